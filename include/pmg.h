@@ -1,0 +1,224 @@
+/*
+ * pmg.h - C ABI of the MI355X (gfx950) geometric-multigrid cell-Vanka preconditioner
+ * for Q2/P1disc Navier-Stokes (Oseen) saddle-point systems inside FGMRES.
+ *
+ * The entry points replace the reference's NSE type-4 multigrid path.  Each declaration
+ * names the reference member or free function it stands in for (paths relative to the
+ * ParMooN source tree).  All arrays crossing this boundary are plain C arrays; no C++,
+ * HIP or torch types appear in the signatures.  Every call returns a pmg_status; none of
+ * them calls exit() (the reference exits with 4711 on divergence, see
+ * src/FE/NSE_MultiGrid.C:629-633 and src/FE/FgmresIte.C:564-567).
+ *
+ * Vector layout (same as the reference, src/FE/NSE_MGLevel4.C:257-268):
+ *   x = [ u1 (n_udof) | u2 (n_udof) | u3 (n_udof) | p (n_pdof) ]
+ * Velocity numbering: active [0, n_active), hanging [n_active, hanging_bound),
+ * Dirichlet [hanging_bound, n_udof).  P1disc pressure is cell-major, 4 per cell,
+ * constant mode first.
+ */
+#ifndef PMG_H
+#define PMG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------------------- */
+/* status codes                                                                          */
+/* ----------------------------------------------------------------------------------- */
+typedef enum {
+  PMG_OK = 0,
+  PMG_ERR_INVALID_ARGUMENT = 1,
+  PMG_ERR_DEVICE = 2,           /* a HIP runtime call failed                            */
+  PMG_ERR_OUT_OF_MEMORY = 3,
+  PMG_ERR_NOT_READY = 4,        /* levels missing / setup not done                      */
+  PMG_ERR_SINGULAR_PATCH = 5,   /* zero pivot in a local Vanka patch (dgetrf info>0)     */
+  PMG_ERR_SINGULAR_COARSE = 6,  /* zero pivot in the coarse dense LU                     */
+  PMG_ERR_NONFINITE = 7,        /* NaN/Inf in an input vector or residual                */
+  PMG_ERR_DIVERGED = 8,         /* FGMRES residual > div_factor * res0 (reference: exit) */
+  PMG_ERR_NOT_CONVERGED = 9,    /* maxit reached without meeting the tolerances          */
+  PMG_ERR_UNSUPPORTED = 10
+} pmg_status;
+
+/* arithmetic of the smoother / level operator (the outer FGMRES is always FP64) */
+typedef enum {
+  PMG_PREC_FP64 = 0,            /* plain FP64 path (parity with the FP64 oracle)         */
+  PMG_PREC_FP32 = 1,            /* FP32 operator copy + FP32 patch factors               */
+  PMG_PREC_FP32_BF16 = 2        /* FP32 operator copy + BF16 patch factors, FP32 refine  */
+} pmg_precision;
+
+/* where vector arguments live */
+typedef enum { PMG_MEM_HOST = 0, PMG_MEM_DEVICE = 1 } pmg_memspace;
+
+/*
+ * Solver knobs.  Field names follow TDatabase::ParamDB (src/General/Database.C:640-749,871).
+ * pmg_params_default() returns the reference defaults.
+ */
+typedef struct {
+  int    sc_smoother_saddle;                   /* 2 = colored cell Vanka (1: same, GS order) */
+  int    sc_pre_smooth_saddle;                 /* nu1 */
+  int    sc_post_smooth_saddle;                /* nu2 */
+  double sc_smooth_damp_factor_saddle;         /* global post-sweep relaxation alpha      */
+  double sc_smooth_damp_factor_coarse_saddle;  /* per-cell Vanka damping 'damp'           */
+  double sc_gmg_damp_factor_saddle;            /* coarse-grid-correction damping beta     */
+  int    sc_coarse_smoother_saddle;            /* 17 = dense direct solve                 */
+  int    internal_project_pressure;            /* 1 = mean-zero P1disc constants          */
+  int    sc_gmres_restart;
+  int    sc_lin_maxit_saddle;
+  double sc_lin_red_factor_saddle;
+  double sc_lin_res_norm_min_saddle;
+  double sc_div_factor;
+  int    sc_minit;
+  int    sc_amg_prec_it;
+  int    precision;                            /* pmg_precision                           */
+  int    bf16_refine_steps;                    /* FP32 refinement steps per BF16 local solve */
+} pmg_params;
+
+/*
+ * One multigrid level, in the reference's own data layout (borrowed during pmg_level_add,
+ * copied to the device).  CSR arrays are the TSquareStructure3D / TStructure3D arrays of
+ * include/FE/Matrix.h:91-129: A rows 0..n_udof-1 (Dirichlet rows hold only the diagonal),
+ * B rows 0..n_pdof-1 over velocity columns, BT rows 0..n_udof-1 over pressure columns.
+ */
+typedef struct {
+  int n_udof, n_active, hanging_bound, n_dirichlet, n_pdof;
+  const int *a_rowptr, *a_kcol; const double *a[9];   /* A11 A12 A13 A21 A22 A23 A31 A32 A33 */
+  int a_nnz;
+  const int *b_rowptr, *b_kcol; const double *b[3];   /* B1 B2 B3   */
+  int b_nnz;
+  const int *bt_rowptr, *bt_kcol; const double *bt[3];/* B1T B2T B3T */
+  int bt_nnz;
+  /* cell -> dof maps: UGlobalNumbers + UBeginIndex[cell] (27 each), same for P (4 each) */
+  int n_cells, n_ulocal, n_plocal;
+  const int *u_begin, *u_global, *p_begin, *p_global;
+  /* optional cell coloring (colorCell output); NULL -> the library computes it */
+  int n_colors; const int *ptr_colors; const int *cell_reorder;
+  /* transfer to the next coarser level (ignored on level 0): parent cell on the coarser
+     level and child number for every cell of this level, and the per-child prolongation
+     matrices QQ[child][k*n_coarse+l] (fine functional k of coarse basis l), i.e.
+     TFEDatabase3D::GetProlongationMatrix3D (src/FE/FEDatabase3D.C:1630-1720) */
+  const int *parent, *child;
+  int n_children;
+  const double *qq_u;       /* n_children * n_ulocal * n_ulocal */
+  const double *qq_p;       /* n_children * n_plocal * n_plocal */
+} pmg_level_desc;
+
+typedef struct pmg_ctx pmg_ctx;
+
+/* ----------------------------------------------------------------------------------- */
+/* context and levels                                                                    */
+/* ----------------------------------------------------------------------------------- */
+void        pmg_params_default(pmg_params *p);
+const char *pmg_status_string(int status);
+
+/* Replaces: TNSE_MultiGrid::TNSE_MultiGrid (src/FE/NSE_MultiGrid.C:81-104) plus the
+   device set-up TNSE_MultiGrid::InitGPU (src/FE/NSE_MultiGrid.C:105-185). */
+int  pmg_ctx_create(int device, const pmg_params *params, pmg_ctx **out);
+/* Replaces: TNSE_MultiGrid::~TNSE_MultiGrid / FreeGPU. */
+int  pmg_ctx_destroy(pmg_ctx *ctx);
+const char *pmg_last_error_string(const pmg_ctx *ctx);
+int  pmg_set_params(pmg_ctx *ctx, const pmg_params *params);
+
+/* Replaces: TNSE_MultiGrid::AddLevel (src/FE/NSE_MultiGrid.C:94-102) together with the
+   TNSE_MGLevel4 constructor (src/FE/NSE_MGLevel4.C:119-326).  Levels are added coarsest
+   first; the returned index equals the reference's level number. */
+int  pmg_level_add(pmg_ctx *ctx, const pmg_level_desc *desc, int *level_out);
+/* Replaces: re-assembly with a fixed pattern (SystemTNSE3D.C:1400-1415 call sites):
+   new matrix values, same structure.  Marks the level's patch factors stale. */
+int  pmg_level_update_values(pmg_ctx *ctx, int level, const pmg_level_desc *desc);
+/* Value-dependent set-up: batched patch LU of every level >= 1 and the coarse dense LU.
+   The reference re-factors inside every sweep (NSE_MGLevel4.C:1400-1609) and every coarse
+   visit (NSE_MGLevel4.C:5728-5846); the factors are the same, so they are kept. */
+int  pmg_setup(pmg_ctx *ctx);
+int  pmg_level_info(const pmg_ctx *ctx, int level, int *n_dof, int *n_colors, int *max_cells_per_color);
+
+/* ----------------------------------------------------------------------------------- */
+/* level operations (TNSE_MGLevel / TNSE_MGLevel4 virtuals and MGComponents3D.C)         */
+/* vectors are n_dof = 3*n_udof + n_pdof long, in memory 'mem'                           */
+/* ----------------------------------------------------------------------------------- */
+/* Replaces: TNSE_MGLevel4::Defect (src/FE/NSE_MGLevel4.C:465-539): CoupledDefect, IntoL20
+   of the pressure constants, Dirichlet rows zeroed, res = ||d||_2 (res may be NULL). */
+int  pmg_defect(pmg_ctx *ctx, int level, const double *u, const double *f, double *d,
+                double *res, int mem);
+/* Replaces: MatVect_NSE4 / CoupledMatVect (src/FE/MGComponents3D.C:2872-3035). */
+int  pmg_matvec(pmg_ctx *ctx, int level, const double *x, double *y, int mem);
+/* Replaces: Defect_NSE4 (src/FE/MGComponents3D.C:3210-3231): r = b - K x, IntoL20 on p. */
+int  pmg_defect_nse4(pmg_ctx *ctx, int level, const double *x, const double *b, double *r,
+                     int mem);
+/* Replaces: TNSE_MGLevel4::CellVanka, colored variant (src/FE/NSE_MGLevel4.C:603-1649),
+   called nsweeps times.  u is updated in place; f is the level right-hand side. */
+int  pmg_vanka_smooth(pmg_ctx *ctx, int level, double *u, const double *f, int nsweeps,
+                      int mem);
+/* Replaces: DefectRestriction for U (3 functions, MGComponents3D.C:996-1240) and P
+   (MGComponents3D.C:711-993): f_coarse = R d_fine.  'level' is the fine level. */
+int  pmg_restrict_defect(pmg_ctx *ctx, int level, const double *d_fine, double *f_coarse,
+                         int mem);
+/* Replaces: Prolongate for U (MGComponents3D.C:468-708) and P (MGComponents3D.C:184-466):
+   d_fine = P u_coarse.  'level' is the fine level. */
+int  pmg_prolongate(pmg_ctx *ctx, int level, const double *u_coarse, double *d_fine,
+                    int mem);
+/* Replaces: TNSE_MGLevel4::CorrectNodes (NSE_MGLevel4.C:542-601): u_D := 0. */
+int  pmg_correct_nodes(pmg_ctx *ctx, int level, double *u, int mem);
+/* Replaces: TNSE_MultiGrid::SetDirichletNodes (NSE_MultiGrid.C:1781-1802): u_D := f_D. */
+int  pmg_set_dirichlet(pmg_ctx *ctx, int level, double *u, const double *f, int mem);
+/* Replaces: TNSE_MGLevel::CorrectDefect (NSE_MGLevel.C:202-278): rows >= n_active := 0,
+   IntoL20 of the pressure constants. */
+int  pmg_correct_defect(pmg_ctx *ctx, int level, double *d, int mem);
+/* Replaces: TNSE_MGLevel::Update (NSE_MGLevel.C:121-129): u += beta d. */
+int  pmg_update(pmg_ctx *ctx, int level, double *u, const double *d, int mem);
+/* Replaces: TNSE_MGLevel4::SolveExact (NSE_MGLevel4.C:5728-5846) on level 0: dense LU
+   with pressure dof 0 pinned (kept factors, see pmg_setup). */
+int  pmg_coarse_solve(pmg_ctx *ctx, double *u, const double *f, int mem);
+
+/* ----------------------------------------------------------------------------------- */
+/* cycle and Krylov drivers                                                              */
+/* ----------------------------------------------------------------------------------- */
+/* Replaces: TMultiGridIte::Iterate (src/FE/MultiGridIte.C:76-130) with zero start:
+   one V-cycle TNSE_MultiGrid::Cycle (NSE_MultiGrid.C:434-1779) on the finest level,
+   z = M^{-1} v. */
+int  pmg_vcycle(pmg_ctx *ctx, const double *v, double *z, int mem);
+/* Replaces: TFgmresIte::Iterate, flexible branch (src/FE/FgmresIte.C:250-677) with the
+   GMG preconditioner.  sol is updated in place.  iters gets the reference's return value
+   j; final_res the last |s[i+1]| (or restart residual).  Returns PMG_ERR_DIVERGED or
+   PMG_ERR_NOT_CONVERGED where the reference exits or prints "FGMRES not converged". */
+int  pmg_fgmres_solve(pmg_ctx *ctx, double *sol, const double *rhs, int *iters,
+                      double *final_res, int mem);
+/* Residual history of the last pmg_fgmres_solve (|s[i+1]| per iteration, plus
+   restart residuals); returns the number of entries written (<= cap). */
+int  pmg_fgmres_history(const pmg_ctx *ctx, double *out, int cap);
+
+/* ----------------------------------------------------------------------------------- */
+/* coloring                                                                              */
+/* ----------------------------------------------------------------------------------- */
+/* Replaces: TParFEMapper3D::colorCell (src/Parallel/ParFEMapper3D.C:750-1084): cells are
+   adjacent iff they share a velocity dof; greedy first-fit in cell-index order; output
+   cells grouped by color, increasing index inside a color.  Linear time.  ptr_colors must
+   hold n_cells+1 ints, cell_reorder n_cells ints. */
+int  pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_global,
+                     int n_udof, int *n_colors, int *ptr_colors, int *cell_reorder);
+
+/* ----------------------------------------------------------------------------------- */
+/* device memory helpers for callers without a device allocator                          */
+/* ----------------------------------------------------------------------------------- */
+int  pmg_device_alloc(pmg_ctx *ctx, size_t bytes, void **ptr);
+int  pmg_device_free(pmg_ctx *ctx, void *ptr);
+int  pmg_memcpy(pmg_ctx *ctx, void *dst, const void *src, size_t bytes, int kind); /* 0 H2D 1 D2H 2 D2D */
+int  pmg_synchronize(pmg_ctx *ctx);
+
+/* kernel timing of the last call (ms), measured with HIP events on the library's stream */
+typedef struct {
+  double defect_ms;   int defect_calls;
+  double sweep_ms;    int sweep_calls;      /* one colored Vanka sweep (all colors)      */
+  double transfer_ms; int transfer_calls;
+  double coarse_ms;   int coarse_calls;
+  double patch_lu_ms; double coarse_lu_ms;
+} pmg_timers;
+int  pmg_timers_get(const pmg_ctx *ctx, pmg_timers *t);
+int  pmg_timers_reset(pmg_ctx *ctx);
+int  pmg_set_profiling(pmg_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMG_H */

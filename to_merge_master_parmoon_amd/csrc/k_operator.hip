@@ -1,0 +1,373 @@
+// k_operator.hip - type-4 block operator (CoupledDefect / CoupledMatVect / Defect_NSE4 /
+// TNSE_MGLevel4::Defect) and the vector kernels around it, for gfx950.
+//
+// Mapping: a group of 16 lanes owns one CSR row; lanes stride over the row's entries so
+// that every value stream (9 A blocks, 3 B or BT blocks, column indices) is read as
+// contiguous 16-entry segments; the three velocity components of a row are produced
+// together so x is gathered once per entry.  Pressure rows follow the velocity rows in
+// the same launch.  Reductions that the reference performs after the product (IntoL20
+// mean of the P1disc constants, the Euclidean norm) are folded into the same launch with a
+// deterministic last-block finish: every block writes its partial sums, the block that
+// arrives last sums them in a fixed order and applies the projection.
+#include "pmg_internal.hpp"
+
+namespace pmg {
+
+namespace {
+
+template <typename T>
+__device__ inline T red16(T v) {
+  v += __shfl_xor(v, 8, 16);
+  v += __shfl_xor(v, 4, 16);
+  v += __shfl_xor(v, 2, 16);
+  v += __shfl_xor(v, 1, 16);
+  return v;
+}
+
+// block-wide sum of two doubles (blockDim.x == 256); result valid in thread 0
+__device__ inline void block_sum2(double &a, double &b, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = a;
+  sh[256 + t] = b;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      sh[t] += sh[t + o];
+      sh[256 + t] += sh[256 + t + o];
+    }
+    __syncthreads();
+  }
+  a = sh[0];
+  b = sh[256];
+}
+
+// Last block of the grid sums partials[0..nblk) (two values each) in a fixed order.
+// Returns true in every thread of the last block.
+__device__ inline bool last_block_sum2(double pa, double pb, double *partials, unsigned *counter,
+                                       double &ta, double &tb, double *sh) {
+  __shared__ bool am_last;
+  block_sum2(pa, pb, sh);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = pa;
+    partials[2 * blockIdx.x + 1] = pb;
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < int(gridDim.x); i += 256) {
+    a += static_cast<volatile double *>(partials)[2 * i];
+    b += static_cast<volatile double *>(partials)[2 * i + 1];
+  }
+  __syncthreads();
+  block_sum2(a, b, sh);
+  ta = a;
+  tb = b;
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+template <int MODE, typename T>
+__global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict__ x,
+                                               const T *__restrict__ bv, T *__restrict__ y,
+                                               int project, int want_norm, double *partials,
+                                               unsigned *counter, double *res) {
+  __shared__ double sh[512];
+  const int lane = threadIdx.x & 15;
+  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int ngrp = gridDim.x * 16;
+  const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
+  const long nnzA = op.a_nnz, nnzB = op.b_nnz, nnzBT = op.bt_nnz;
+  const T *__restrict__ A = op.a;
+  const T *__restrict__ B = op.b;
+  const T *__restrict__ BT = op.bt;
+  const T *__restrict__ x1 = x;
+  const T *__restrict__ x2 = x + N;
+  const T *__restrict__ x3 = x + 2 * N;
+  const T *__restrict__ xp = x + 3 * N;
+  double pc = 0.0, sq = 0.0;
+  const int nrows = N + NP;
+  for (int r = grp; r < nrows; r += ngrp) {
+    if (r < N) {
+      T s1 = 0, s2 = 0, s3 = 0;
+      const int k0 = op.a_rowptr[r], k1 = op.a_rowptr[r + 1];
+      if (r < NA) {
+        for (int k = k0 + lane; k < k1; k += 16) {
+          const int c = op.a_kcol[k];
+          const T u1 = x1[c], u2 = x2[c], u3 = x3[c];
+          s1 += A[k] * u1 + A[nnzA + k] * u2 + A[2 * nnzA + k] * u3;
+          s2 += A[3 * nnzA + k] * u1 + A[4 * nnzA + k] * u2 + A[5 * nnzA + k] * u3;
+          s3 += A[6 * nnzA + k] * u1 + A[7 * nnzA + k] * u2 + A[8 * nnzA + k] * u3;
+        }
+        const int q0 = op.bt_rowptr[r], q1 = op.bt_rowptr[r + 1];
+        for (int k = q0 + lane; k < q1; k += 16) {
+          const T pv = xp[op.bt_kcol[k]];
+          s1 += BT[k] * pv;
+          s2 += BT[nnzBT + k] * pv;
+          s3 += BT[2 * nnzBT + k] * pv;
+        }
+      } else {
+        // rows N_Active..N_UDOF: diagonal blocks only (MGComponents3D.C:3136-3156)
+        for (int k = k0 + lane; k < k1; k += 16) {
+          const int c = op.a_kcol[k];
+          s1 += A[k] * x1[c];
+          s2 += A[4 * nnzA + k] * x2[c];
+          s3 += A[8 * nnzA + k] * x3[c];
+        }
+      }
+      s1 = red16(s1);
+      s2 = red16(s2);
+      s3 = red16(s3);
+      if (lane == 0) {
+        T y1, y2, y3;
+        if (MODE == 0) {
+          y1 = s1; y2 = s2; y3 = s3;
+        } else {
+          y1 = bv[r] - s1; y2 = bv[N + r] - s2; y3 = bv[2 * N + r] - s3;
+        }
+        if (MODE == 2 && r >= NA) y1 = y2 = y3 = 0;
+        y[r] = y1; y[N + r] = y2; y[2 * N + r] = y3;
+        if (want_norm) sq += double(y1) * y1 + double(y2) * y2 + double(y3) * y3;
+      }
+    } else {
+      const int pr = r - N;
+      T s = 0;
+      const int k0 = op.b_rowptr[pr], k1 = op.b_rowptr[pr + 1];
+      for (int k = k0 + lane; k < k1; k += 16) {
+        const int c = op.b_kcol[k];
+        s += B[k] * x1[c] + B[nnzB + k] * x2[c] + B[2 * nnzB + k] * x3[c];
+      }
+      s = red16(s);
+      if (lane == 0) {
+        const T v = (MODE == 0) ? s : T(bv[3 * N + pr] - s);
+        y[3 * N + pr] = v;
+        if (project && (pr & 3) == 0)
+          pc += double(v);
+        else if (want_norm)
+          sq += double(v) * v;
+      }
+    }
+  }
+  if (!(project || want_norm)) return;
+  double tpc, tsq;
+  if (!last_block_sum2(pc, sq, partials, counter, tpc, tsq, sh)) return;
+  // IntoL20Vector3D case -11 (MGComponents3D.C:1766-1786): s = sum/Length*4
+  double add = 0.0;
+  if (project) {
+    const double m = tpc / double(NP) * 4.0;
+    T *p = y + 3 * N;
+    for (int i = threadIdx.x; i < NP / 4; i += 256) {
+      const T v = T(p[4 * i] - T(m));
+      p[4 * i] = v;
+      add += double(v) * v;
+    }
+  }
+  if (want_norm) {
+    double dummy = 0.0;
+    block_sum2(add, dummy, sh);
+    if (threadIdx.x == 0) *res = sqrt(tsq + add);
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T *x, T v, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) x[i] = v;
+}
+template <typename T>
+__global__ void k_copy(T *__restrict__ d, const T *__restrict__ s, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) d[i] = s[i];
+}
+template <typename T>
+__global__ void k_axpy(T *__restrict__ y, T a, const T *__restrict__ x, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) y[i] += a * x[i];
+}
+template <typename T>
+__global__ void k_relax(T *__restrict__ u, const T *__restrict__ uo, T a, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    u[i] = uo[i] + a * (u[i] - uo[i]);
+}
+__global__ void k_d2f(float *__restrict__ d, const double *__restrict__ s, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) d[i] = float(s[i]);
+}
+__global__ void k_f2d(double *__restrict__ d, const float *__restrict__ s, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) d[i] = double(s[i]);
+}
+// u[c*N + start + i] = f ? f[c*N + start + i] : 0, c = 0..2
+template <typename T>
+__global__ void k_dir(T *__restrict__ u, const T *__restrict__ f, int N, int start, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * count) return;
+  const int c = i / count, j = i - c * count;
+  const long o = long(c) * N + start + j;
+  u[o] = f ? f[o] : T(0);
+}
+// standalone IntoL20 of the P1disc constants (single block)
+template <typename T>
+__global__ __launch_bounds__(256) void k_l20(T *p, int NP) {
+  __shared__ double sh[512];
+  double s = 0, d = 0;
+  for (int i = threadIdx.x; i < NP / 4; i += 256) s += double(p[4 * i]);
+  block_sum2(s, d, sh);
+  const double m = s / double(NP) * 4.0;
+  for (int i = threadIdx.x; i < NP / 4; i += 256) p[4 * i] = T(p[4 * i] - T(m));
+}
+
+// ------------------------------------------------------------------ BLAS-1 for FGMRES
+__global__ __launch_bounds__(256) void k_dot(const double *__restrict__ x, const double *__restrict__ y,
+                                             long n, double *out, double *partials, unsigned *counter) {
+  __shared__ double sh[512];
+  double s = 0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += long(gridDim.x) * 256) s += x[i] * y[i];
+  double t, u;
+  if (last_block_sum2(s, 0.0, partials, counter, t, u, sh) && threadIdx.x == 0) *out = t;
+}
+// w -= h[0] v ; out = <w, vnext>
+__global__ __launch_bounds__(256) void k_axpy_dot(double *__restrict__ w, const double *h,
+                                                  const double *__restrict__ v,
+                                                  const double *__restrict__ vn, long n, double *out,
+                                                  double *partials, unsigned *counter) {
+  __shared__ double sh[512];
+  const double a = -h[0];
+  double s = 0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += long(gridDim.x) * 256) {
+    const double wi = w[i] + a * v[i];
+    w[i] = wi;
+    if (vn) s += wi * vn[i];
+  }
+  if (!vn) return;
+  double t, u;
+  if (last_block_sum2(s, 0.0, partials, counter, t, u, sh) && threadIdx.x == 0) *out = t;
+}
+__global__ void k_scale_dev(double *__restrict__ y, const double *__restrict__ x, const double *den, long n) {
+  const double d = 1.0 / den[0];
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    y[i] = x[i] * d;
+}
+__global__ void k_lincomb(double *__restrict__ x, const double *__restrict__ z, long stride,
+                          const double *__restrict__ coef, int m, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) {
+    double t = 0.0;
+    for (int j = 0; j < m; ++j) t += z[j * stride + i] * coef[j];
+    x[i] += t;
+  }
+}
+__global__ void k_scal(double *__restrict__ y, const double *__restrict__ x, double a, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    y[i] = x[i] * a;
+}
+
+inline int grid_for(long n, int bs = 256, int cap = 8192) {
+  long g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return int(g);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project, double *res,
+                  RedScratch &red, hipStream_t s) {
+  const long rows = long(op.n_udof) + op.n_pdof;
+  int grid = int((rows + 15) / 16);
+  if (grid > RED_BLOCKS) grid = RED_BLOCKS;
+  const int want_norm = res != nullptr;
+  const int proj = project ? 1 : 0;
+  if (mode == 0)
+    k_type4<0, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, want_norm, red.partials, red.counter, res);
+  else if (mode == 1)
+    k_type4<1, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res);
+  else
+    k_type4<2, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_type4<double>(const OpView<double> &, int, const double *, const double *, double *, bool,
+                                   double *, RedScratch &, hipStream_t);
+template void launch_type4<float>(const OpView<float> &, int, const float *, const float *, float *, bool,
+                                  double *, RedScratch &, hipStream_t);
+
+template <typename T>
+void launch_fill(T *x, T v, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill<T><<<grid_for(n), 256, 0, s>>>(x, v, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_copy(T *d, const T *src, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_copy<T><<<grid_for(n), 256, 0, s>>>(d, src, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_axpy(T *y, T a, const T *x, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_axpy<T><<<grid_for(n), 256, 0, s>>>(y, a, x, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_relax(T *u, const T *uo, T a, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_relax<T><<<grid_for(n), 256, 0, s>>>(u, uo, a, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_d2f(float *d, const double *src, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_d2f<<<grid_for(n), 256, 0, s>>>(d, src, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_f2d(double *d, const float *src, long n, hipStream_t s) {
+  if (n <= 0) return;
+  k_f2d<<<grid_for(n), 256, 0, s>>>(d, src, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_dirichlet_slices(T *u, const T *f, int N, int start, int count, hipStream_t s) {
+  if (count <= 0) return;
+  k_dir<T><<<(3 * count + 255) / 256, 256, 0, s>>>(u, f, N, start, count);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_l20(T *p, int NP, RedScratch &, hipStream_t s) {
+  if (NP <= 0) return;
+  k_l20<T><<<1, 256, 0, s>>>(p, NP);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+#define PMG_INST_VEC(T)                                                        \
+  template void launch_fill<T>(T *, T, long, hipStream_t);                    \
+  template void launch_copy<T>(T *, const T *, long, hipStream_t);            \
+  template void launch_axpy<T>(T *, T, const T *, long, hipStream_t);         \
+  template void launch_relax<T>(T *, const T *, T, long, hipStream_t);        \
+  template void launch_dirichlet_slices<T>(T *, const T *, int, int, int, hipStream_t); \
+  template void launch_l20<T>(T *, int, RedScratch &, hipStream_t);
+PMG_INST_VEC(double)
+PMG_INST_VEC(float)
+template void launch_fill<int>(int *, int, long, hipStream_t);
+
+void launch_dot(const double *x, const double *y, long n, double *out, RedScratch &red, hipStream_t s) {
+  k_dot<<<grid_for(n, 256, RED_BLOCKS), 256, 0, s>>>(x, y, n, out, red.partials, red.counter);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_axpy_dev_dot(double *w, const double *h, const double *v, const double *vnext, long n, double *out,
+                         RedScratch &red, hipStream_t s) {
+  k_axpy_dot<<<grid_for(n, 256, RED_BLOCKS), 256, 0, s>>>(w, h, v, vnext, n, out, red.partials, red.counter);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_scale_dev(double *y, const double *x, const double *den, long n, hipStream_t s) {
+  k_scale_dev<<<grid_for(n), 256, 0, s>>>(y, x, den, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_lincomb(double *x, const double *z, long stride, const double *coef, int m, long n, hipStream_t s) {
+  if (m <= 0) return;
+  k_lincomb<<<grid_for(n), 256, 0, s>>>(x, z, stride, coef, m, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_scal(double *y, const double *x, double a, long n, hipStream_t s) {
+  k_scal<<<grid_for(n), 256, 0, s>>>(y, x, a, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace pmg

@@ -1,0 +1,384 @@
+// k_vanka.hip - colored cell-Vanka smoother for Q2/P1disc hexahedra on gfx950.
+//
+// Restates TNSE_MGLevel4::CellVanka, colored variant (src/FE/NSE_MGLevel4.C:603-1649).
+// The reference re-assembles every 85x85 patch matrix by linear search and calls
+// dgetrf/dgetrs for it in every sweep; the matrices depend only on the operator values,
+// so here they are assembled and LU-factored once per operator (k_patch_lu) and every
+// sweep only applies the stored factors:
+//   k_vanka_res   : per color, local residuals of all patches from the pre-color state
+//                   (phase 1, :819-1064), written in pivot-permuted order;
+//   k_vanka_solve : per color, one wavefront per patch: unit-lower and upper column sweeps
+//                   streaming the factors from HBM, then u += damp * delta (phase 3,
+//                   :1400-1609) - race free because same-color patches share no dof.
+// The factorization and the sweeps perform the same fma/div sequence as the FP64 oracle
+// (dgetf2 pivoting, reciprocal scaling, column-oriented dtrsm), so FP64 factors and
+// local solves agree with it bit for bit.
+#include "pmg_internal.hpp"
+
+namespace pmg {
+
+namespace {
+
+template <typename T>
+__device__ inline T red16(T v) {
+  v += __shfl_xor(v, 8, 16);
+  v += __shfl_xor(v, 4, 16);
+  v += __shfl_xor(v, 2, 16);
+  v += __shfl_xor(v, 1, 16);
+  return v;
+}
+
+__device__ inline double bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__device__ inline double tfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ inline float tfma(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename T, typename F>
+struct Ld;
+template <typename T>
+struct Ld<T, T> {
+  static __device__ inline T get(const T *p, int i) { return p[i]; }
+};
+template <>
+struct Ld<float, double> {
+  static __device__ inline float get(const double *p, int i) { return float(p[i]); }
+};
+template <>
+struct Ld<float, uint16_t> {
+  static __device__ inline float get(const uint16_t *p, int i) {
+    return __uint_as_float(uint32_t(p[i]) << 16);
+  }
+};
+
+// ------------------------------------------------------------------ patch matrix + LU
+// One workgroup per patch position (cells in color order).  The local matrix is built in
+// LDS exactly as NSE_MGLevel4.C:1066-1290 builds System[col*85+row] (a CSR entry lands in
+// the column of the first local dof with the same global number; Dirichlet rows drop the
+// off-diagonal blocks and the BT columns), then factored with dgetf2 semantics.
+__global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *__restrict__ cells,
+                                                  const int *__restrict__ u_global,
+                                                  const int *__restrict__ p_global, int ncells,
+                                                  double *__restrict__ lu, uint8_t *__restrict__ invperm,
+                                                  int *__restrict__ info) {
+  extern __shared__ double S[];  // NLOC * NLOC, column-major
+  __shared__ int ud[NU_LOC], pd[NP_LOC], sigma[NLOC];
+  __shared__ int piv_s, info_s, nz_s;
+  const int pos = blockIdx.x;
+  if (pos >= ncells) return;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int cell = cells[pos];
+  const long nnzA = op.a_nnz, nnzB = op.b_nnz, nnzBT = op.bt_nnz;
+  if (t < NU_LOC) ud[t] = u_global[long(cell) * NU_LOC + t];
+  if (t < NP_LOC) pd[t] = p_global[long(cell) * NP_LOC + t];
+  if (t < NLOC) sigma[t] = t;
+  if (t == 0) info_s = 0;
+  for (int i = t; i < NLOC * NLOC; i += 256) S[i] = 0.0;
+  __syncthreads();
+  // velocity rows j, j+27, j+54
+  for (int j = wave; j < NU_LOC; j += 4) {
+    const int UDOF = ud[j];
+    const bool dir = UDOF >= op.n_active;
+    for (int k = op.a_rowptr[UDOF] + lane; k < op.a_rowptr[UDOF + 1]; k += 64) {
+      const int l = op.a_kcol[k];
+      int m = -1;
+      for (int q = 0; q < NU_LOC; ++q)
+        if (ud[q] == l) { m = q; break; }
+      if (m < 0) continue;
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double v = op.a[(3 * r + c) * nnzA + k];
+          if (dir && r != c) v = 0.0;
+          S[(m + c * NU_LOC) * NLOC + j + r * NU_LOC] = v;
+        }
+    }
+    if (!dir) {
+      for (int k = op.bt_rowptr[UDOF] + lane; k < op.bt_rowptr[UDOF + 1]; k += 64) {
+        const int l = op.bt_kcol[k];
+        int m = -1;
+        for (int q = 0; q < NP_LOC; ++q)
+          if (pd[q] == l) { m = q; break; }
+        if (m < 0) continue;
+        for (int r = 0; r < 3; ++r) S[(3 * NU_LOC + m) * NLOC + j + r * NU_LOC] = op.bt[r * nnzBT + k];
+      }
+    }
+  }
+  // pressure rows 81+j
+  for (int j = wave; j < NP_LOC; j += 4) {
+    const int PDOF = pd[j];
+    for (int k = op.b_rowptr[PDOF] + lane; k < op.b_rowptr[PDOF + 1]; k += 64) {
+      const int l = op.b_kcol[k];
+      int m = -1;
+      for (int q = 0; q < NU_LOC; ++q)
+        if (ud[q] == l) { m = q; break; }
+      if (m < 0) continue;
+      for (int c = 0; c < 3; ++c) S[(m + c * NU_LOC) * NLOC + 3 * NU_LOC + j] = op.b[c * nnzB + k];
+    }
+  }
+  __syncthreads();
+  // dgetf2
+  for (int j = 0; j < NLOC; ++j) {
+    double *colj = S + j * NLOC;
+    if (wave == 0) {
+      const int i0 = j + lane, i1 = j + lane + 64;
+      double v0 = i0 < NLOC ? fabs(colj[i0]) : -1.0;
+      double v1 = i1 < NLOC ? fabs(colj[i1]) : -1.0;
+      double bv = v0;
+      int bi = i0;
+      if (v1 > bv) { bv = v1; bi = i1; }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) {
+        piv_s = bi;
+        nz_s = bv != 0.0;
+      }
+    }
+    __syncthreads();
+    const int p = piv_s;
+    const bool nonzero = nz_s != 0;
+    if (nonzero) {
+      if (p != j) {
+        if (t < NLOC) {
+          const double a = S[t * NLOC + j];
+          S[t * NLOC + j] = S[t * NLOC + p];
+          S[t * NLOC + p] = a;
+        }
+        if (t == 0) {
+          const int s0 = sigma[j];
+          sigma[j] = sigma[p];
+          sigma[p] = s0;
+        }
+        __syncthreads();
+      }
+      const double r = 1.0 / colj[j];
+      if (t > j && t < NLOC) colj[t] *= r;
+    } else if (t == 0 && info_s == 0) {
+      info_s = j + 1;
+    }
+    __syncthreads();
+    const int n = NLOC - 1 - j;
+    for (int idx = t; idx < n * n; idx += 256) {
+      const int c = j + 1 + idx / n, i = j + 1 + idx % n;
+      S[c * NLOC + i] = fma(-colj[i], S[c * NLOC + j], S[c * NLOC + i]);
+    }
+    __syncthreads();
+  }
+  double *out = lu + long(pos) * LU_STRIDE;
+  for (int i = t; i < NLOC * NLOC; i += 256) out[i] = S[i];
+  if (t < NLOC) invperm[long(pos) * PERM_STRIDE + sigma[t]] = uint8_t(t);
+  if (t == 0) info[pos] = info_s;
+}
+
+// ------------------------------------------------------------------ per-color residual
+// (slot, local row) items, 16 lanes each: rows 0..26 produce the three velocity
+// components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061)
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__restrict__ cells, int ncells,
+                                                   int pos0, const int *__restrict__ u_global,
+                                                   const int *__restrict__ p_global,
+                                                   const uint8_t *__restrict__ invperm,
+                                                   const T *__restrict__ u, const T *__restrict__ f,
+                                                   T *__restrict__ rloc) {
+  const int lane = threadIdx.x & 15;
+  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4;
+  const int ngrp = gridDim.x * 16;
+  const int N = op.n_udof, NA = op.n_active;
+  const long nnzA = op.a_nnz, nnzB = op.b_nnz, nnzBT = op.bt_nnz;
+  const T *__restrict__ A = op.a;
+  const T *__restrict__ u1 = u;
+  const T *__restrict__ u2 = u + N;
+  const T *__restrict__ u3 = u + 2 * N;
+  const T *__restrict__ up = u + 3 * N;
+  const int items = ncells * (NU_LOC + NP_LOC);
+  for (int it = grp; it < items; it += ngrp) {
+    const int slot = it / (NU_LOC + NP_LOC), r = it - slot * (NU_LOC + NP_LOC);
+    const int pos = pos0 + slot;
+    const int cell = cells[pos];
+    const uint8_t *ip = invperm + long(pos) * PERM_STRIDE;
+    T *out = rloc + long(slot) * RLOC_STRIDE;
+    if (r < NU_LOC) {
+      const int UDOF = u_global[long(cell) * NU_LOC + r];
+      T s1 = 0, s2 = 0, s3 = 0;
+      const int k0 = op.a_rowptr[UDOF], k1 = op.a_rowptr[UDOF + 1];
+      if (UDOF < NA) {
+        for (int k = k0 + lane; k < k1; k += 16) {
+          const int c = op.a_kcol[k];
+          const T x1 = u1[c], x2 = u2[c], x3 = u3[c];
+          s1 += A[k] * x1 + A[nnzA + k] * x2 + A[2 * nnzA + k] * x3;
+          s2 += A[3 * nnzA + k] * x1 + A[4 * nnzA + k] * x2 + A[5 * nnzA + k] * x3;
+          s3 += A[6 * nnzA + k] * x1 + A[7 * nnzA + k] * x2 + A[8 * nnzA + k] * x3;
+        }
+        for (int k = op.bt_rowptr[UDOF] + lane; k < op.bt_rowptr[UDOF + 1]; k += 16) {
+          const T pv = up[op.bt_kcol[k]];
+          s1 += op.bt[k] * pv;
+          s2 += op.bt[nnzBT + k] * pv;
+          s3 += op.bt[2 * nnzBT + k] * pv;
+        }
+      } else {
+        for (int k = k0 + lane; k < k1; k += 16) {
+          const int c = op.a_kcol[k];
+          s1 += A[k] * u1[c];
+          s2 += A[4 * nnzA + k] * u2[c];
+          s3 += A[8 * nnzA + k] * u3[c];
+        }
+      }
+      s1 = red16(s1);
+      s2 = red16(s2);
+      s3 = red16(s3);
+      if (lane == 0) {
+        out[ip[r]] = f[UDOF] - s1;
+        out[ip[NU_LOC + r]] = f[N + UDOF] - s2;
+        out[ip[2 * NU_LOC + r]] = f[2 * N + UDOF] - s3;
+      }
+    } else {
+      const int m = r - NU_LOC;
+      const int PDOF = p_global[long(cell) * NP_LOC + m];
+      T s = 0;
+      for (int k = op.b_rowptr[PDOF] + lane; k < op.b_rowptr[PDOF + 1]; k += 16) {
+        const int c = op.b_kcol[k];
+        s += op.b[k] * u1[c] + op.b[nnzB + k] * u2[c] + op.b[2 * nnzB + k] * u3[c];
+      }
+      s = red16(s);
+      if (lane == 0) out[ip[3 * NU_LOC + m]] = f[3 * N + PDOF] - s;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ per-color local solves
+// one wavefront per patch; lane i owns local rows i and i+64 (i < 21)
+template <typename T, typename F>
+__global__ __launch_bounds__(256) void k_vanka_solve(const F *__restrict__ lu, const int *__restrict__ cells,
+                                                     int ncells, int pos0, const T *__restrict__ rloc,
+                                                     const int *__restrict__ u_global,
+                                                     const int *__restrict__ p_global, int N,
+                                                     T *__restrict__ u, T damp) {
+  const int lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (slot >= ncells) return;
+  const int pos = pos0 + slot;
+  const F *__restrict__ M = lu + long(pos) * LU_STRIDE;
+  const T *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
+  const bool two = lane < NLOC - 64;
+  T x0 = rs[lane];
+  T x1 = two ? rs[64 + lane] : T(0);
+  // unit lower sweep (dtrsm L, column oriented)
+#pragma unroll 4
+  for (int k = 0; k < NLOC - 1; ++k) {
+    const T bk = bcast(k < 64 ? x0 : x1, k & 63);
+    const F *col = M + k * NLOC;
+    if (lane > k) x0 = tfma(-bk, Ld<T, F>::get(col, lane), x0);
+    if (two && lane + 64 > k) x1 = tfma(-bk, Ld<T, F>::get(col, 64 + lane), x1);
+  }
+  // upper sweep (dtrsm U): x_k /= U_kk, then eliminate column k above the diagonal
+#pragma unroll 4
+  for (int k = NLOC - 1; k >= 0; --k) {
+    const F *col = M + k * NLOC;
+    const T xk = bcast(k < 64 ? x0 : x1, k & 63) / Ld<T, F>::get(col, k);
+    if (k < 64) {
+      if (lane == k) x0 = xk;
+    } else if (lane == k - 64) {
+      x1 = xk;
+    }
+    if (lane < k) x0 = tfma(-xk, Ld<T, F>::get(col, lane), x0);
+    if (two && lane + 64 < k) x1 = tfma(-xk, Ld<T, F>::get(col, 64 + lane), x1);
+  }
+  // u[UDOF] += damp * delta, p[PDOF] += damp * delta
+  const int cell = cells[pos];
+  {
+    const int i = lane;  // < 64 < 81: velocity
+    const int c = i / NU_LOC, j = i - c * NU_LOC;
+    const long o = long(c) * N + u_global[long(cell) * NU_LOC + j];
+    u[o] += damp * x0;
+  }
+  if (two) {
+    const int i = 64 + lane;
+    if (i < 3 * NU_LOC) {
+      const int c = i / NU_LOC, j = i - c * NU_LOC;
+      const long o = long(c) * N + u_global[long(cell) * NU_LOC + j];
+      u[o] += damp * x1;
+    } else {
+      const long o = 3L * N + p_global[long(cell) * NP_LOC + (i - 3 * NU_LOC)];
+      u[o] += damp * x1;
+    }
+  }
+}
+
+__global__ void k_lu_f32(const double *__restrict__ a, float *__restrict__ b, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+// round-to-nearest-even bf16
+__global__ void k_lu_bf16(const double *__restrict__ a, uint16_t *__restrict__ b, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) {
+    const uint32_t u = __float_as_uint(float(a[i]));
+    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
+    b[i] = uint16_t(r >> 16);
+  }
+}
+
+}  // namespace
+
+void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s) {
+  if (L.n_cells <= 0) return;
+  const size_t shm = size_t(NLOC) * NLOC * sizeof(double);
+  k_patch_lu<<<L.n_cells, 256, shm, s>>>(L.op64(), L.cell_reorder, L.u_global, L.p_global, L.n_cells, lu,
+                                         L.invperm, info);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0, const int *u_global,
+                           const int *p_global, const uint8_t *invperm, const T *u, const T *f, T *rloc,
+                           hipStream_t s) {
+  if (ncells <= 0) return;
+  const long groups = long(ncells) * (NU_LOC + NP_LOC);
+  long grid = (groups + 15) / 16;
+  if (grid > 8192) grid = 8192;
+  k_vanka_res<T><<<int(grid), 256, 0, s>>>(op, cells, ncells, pos0, u_global, p_global, invperm, u, f, rloc);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_vanka_residual<double>(const OpView<double> &, const int *, int, int, const int *,
+                                                    const int *, const uint8_t *, const double *,
+                                                    const double *, double *, hipStream_t);
+template void launch_vanka_residual<float>(const OpView<float> &, const int *, int, int, const int *,
+                                                  const int *, const uint8_t *, const float *, const float *,
+                                                  float *, hipStream_t);
+
+template <typename T, typename F>
+void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc, const int *u_global,
+                        const int *p_global, int n_udof, T *u, T damp, hipStream_t s) {
+  if (ncells <= 0) return;
+  k_vanka_solve<T, F><<<(ncells + 3) / 4, 256, 0, s>>>(lu, cells, ncells, pos0, rloc, u_global, p_global,
+                                                       n_udof, u, damp);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_vanka_solve<double, double>(const double *, const int *, int, int, const double *,
+                                                 const int *, const int *, int, double *, double, hipStream_t);
+template void launch_vanka_solve<float, float>(const float *, const int *, int, int, const float *, const int *,
+                                               const int *, int, float *, float, hipStream_t);
+template void launch_vanka_solve<float, uint16_t>(const uint16_t *, const int *, int, int, const float *,
+                                                  const int *, const int *, int, float *, float, hipStream_t);
+
+void launch_convert_lu_f32(const double *lu, float *lu32, long n, hipStream_t s) {
+  long g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  k_lu_f32<<<int(g), 256, 0, s>>>(lu, lu32, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+void launch_convert_lu_bf16(const double *lu, uint16_t *lu16, long n, hipStream_t s) {
+  long g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  k_lu_bf16<<<int(g), 256, 0, s>>>(lu, lu16, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace pmg

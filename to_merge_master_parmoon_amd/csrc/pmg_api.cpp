@@ -1,0 +1,1090 @@
+// pmg_api.cpp - host side of libpmg: device-resident multigrid hierarchy, V-cycle and
+// FGMRES drivers behind the C ABI of include/pmg.h.  Compiled by hipcc for gfx950.
+//
+// Everything that touches vectors runs on the device, on the context's stream; the host
+// only sequences kernels and, in FGMRES, performs the (restart+1)-sized Givens/Hessenberg
+// arithmetic of src/FE/FgmresIte.C:199-247 on scalars read back once per iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pmg_internal.hpp"
+
+using namespace pmg;
+
+struct pmg_ctx {
+  int device = 0;
+  pmg_params prm{};
+  hipStream_t stream = nullptr;
+  std::vector<DevLevel> L;
+  // host copies of the dof maps (needed to build the transfers to the next level)
+  std::vector<std::vector<int>> h_u_global, h_p_global;
+  Coarse coarse;
+  RedScratch red;
+  std::string err;
+  bool setup_done = false;
+  // FGMRES workspace
+  int restart_alloc = 0;
+  double *V = nullptr, *Z = nullptr, *W = nullptr, *Hdev = nullptr, *coef = nullptr;
+  double *hpin = nullptr;   // pinned host scalars
+  int hpin_len = 0;
+  std::vector<double> hist;
+  // staging buffers for PMG_MEM_HOST calls
+  double *stage[3] = {nullptr, nullptr, nullptr};
+  long stage_len = 0;
+  // timers
+  bool profiling = false;
+  pmg_timers tm{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<void *> allocs;
+};
+
+namespace {
+
+template <typename T>
+T *dalloc(pmg_ctx *c, size_t n) {
+  if (n == 0) n = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) throw DeviceError(std::string("hipMalloc(") + std::to_string(n * sizeof(T)) + "): " +
+                                         hipGetErrorString(e));
+  c->allocs.push_back(p);
+  return static_cast<T *>(p);
+}
+void dfree(pmg_ctx *c, void *p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+template <typename T>
+void h2d(pmg_ctx *c, T *d, const T *h, size_t n) {
+  if (n) PMG_HIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+}
+
+struct Fail {
+  int code;
+  std::string msg;
+};
+
+int guard(pmg_ctx *c, const std::function<void()> &f);
+
+// ------------------------------------------------------------------ event timing
+struct Timed {
+  pmg_ctx *c;
+  double *acc;
+  int *cnt;
+  Timed(pmg_ctx *c_, double *a, int *n) : c(c_), acc(a), cnt(n) {
+    if (c->profiling) PMG_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  }
+  ~Timed() noexcept(false) {
+    if (!c->profiling) return;
+    PMG_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+    PMG_HIP_CHECK(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    PMG_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *acc += ms;
+    if (cnt) ++*cnt;
+  }
+};
+
+// ------------------------------------------------------------------ transfers
+// P (fine rows) accumulated cellwise from QQ and divided by the fine-dof multiplicity, as
+// Prolongate does (MGComponents3D.C:468-708); R = P^T, which is what DefectRestriction
+// computes (MGComponents3D.C:996-1240).
+void build_transfer(pmg_ctx *c, int nloc_f, int nloc_c, int n_fine_cells, const int *parent, const int *child,
+                    const double *qq, const std::vector<int> &fglob, const std::vector<int> &cglob, int n_fdof,
+                    int n_cdof, Csr &P, Csr &R) {
+  struct Trip {
+    int r, c;
+    double v;
+  };
+  std::vector<Trip> t;
+  t.reserve(size_t(n_fine_cells) * nloc_f * 8);
+  std::vector<int> mult(n_fdof, 0);
+  for (int fc = 0; fc < n_fine_cells; ++fc) {
+    const int pc = parent[fc], ch = child[fc];
+    const double *Q = qq + size_t(ch) * nloc_f * nloc_c;
+    const int *FineDOF = &fglob[size_t(fc) * nloc_f];
+    const int *CoarseDOF = &cglob[size_t(pc) * nloc_c];
+    for (int k = 0; k < nloc_f; ++k) {
+      mult[FineDOF[k]]++;
+      for (int l = 0; l < nloc_c; ++l) {
+        const double v = Q[k * nloc_c + l];
+        if (v != 0.0) t.push_back({FineDOF[k], CoarseDOF[l], v});
+      }
+    }
+  }
+  std::sort(t.begin(), t.end(), [](const Trip &a, const Trip &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+  std::vector<int> rp(n_fdof + 1, 0), col;
+  std::vector<double> val;
+  for (size_t i = 0; i < t.size();) {
+    size_t j = i;
+    double s = 0;
+    while (j < t.size() && t[j].r == t[i].r && t[j].c == t[i].c) s += t[j++].v;
+    col.push_back(t[i].c);
+    val.push_back(s / mult[t[i].r]);
+    rp[t[i].r + 1]++;
+    i = j;
+  }
+  for (int i = 0; i < n_fdof; ++i) rp[i + 1] += rp[i];
+  // transpose
+  std::vector<int> rrp(n_cdof + 1, 0), rcol(col.size());
+  std::vector<double> rval(col.size());
+  for (int cc : col) rrp[cc + 1]++;
+  for (int i = 0; i < n_cdof; ++i) rrp[i + 1] += rrp[i];
+  std::vector<int> pos(rrp.begin(), rrp.end() - 1);
+  for (int r = 0; r < n_fdof; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int q = pos[col[k]]++;
+      rcol[q] = r;
+      rval[q] = val[k];
+    }
+  auto up = [&](Csr &m, int nr, int ncl, const std::vector<int> &r_, const std::vector<int> &c_,
+                const std::vector<double> &v_) {
+    m.n_rows = nr;
+    m.n_cols = ncl;
+    m.nnz = int(c_.size());
+    m.rowptr = dalloc<int>(c, r_.size());
+    m.col = dalloc<int>(c, c_.size());
+    m.val = dalloc<double>(c, v_.size());
+    h2d(c, m.rowptr, r_.data(), r_.size());
+    h2d(c, m.col, c_.data(), c_.size());
+    h2d(c, m.val, v_.data(), v_.size());
+  };
+  up(P, n_fdof, n_cdof, rp, col, val);
+  up(R, n_cdof, n_fdof, rrp, rcol, rval);
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
+  for (int b = 0; b < 9; ++b) h2d(c, D.a + size_t(b) * D.a_nnz, d->a[b], D.a_nnz);
+  for (int b = 0; b < 3; ++b) {
+    h2d(c, D.b + size_t(b) * D.b_nnz, d->b[b], D.b_nnz);
+    h2d(c, D.bt + size_t(b) * D.bt_nnz, d->bt[b], D.bt_nnz);
+  }
+  if (D.a32) {
+    launch_d2f(D.a32, D.a, 9 * D.a_nnz, c->stream);
+    launch_d2f(D.b32, D.b, 3 * D.b_nnz, c->stream);
+    launch_d2f(D.bt32, D.bt, 3 * D.bt_nnz, c->stream);
+  }
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+bool mixed(const pmg_ctx *c) { return c->prm.precision != PMG_PREC_FP64; }
+
+void ensure_stage(pmg_ctx *c, long n) {
+  if (n <= c->stage_len) return;
+  for (auto &p : c->stage) {
+    dfree(c, p);
+    p = dalloc<double>(c, n);
+  }
+  c->stage_len = n;
+}
+
+// ------------------------------------------------------------------ level operations
+void correct_nodes(pmg_ctx *c, const DevLevel &D, double *u) {
+  launch_dirichlet_slices<double>(u, nullptr, D.n_udof, D.hanging_bound, D.n_dirichlet, c->stream);
+}
+void set_dirichlet(pmg_ctx *c, const DevLevel &D, double *u, const double *f) {
+  launch_dirichlet_slices<double>(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, c->stream);
+}
+void correct_defect(pmg_ctx *c, const DevLevel &D, double *d) {
+  launch_dirichlet_slices<double>(d, nullptr, D.n_udof, D.n_active, D.n_udof - D.n_active, c->stream);
+  if (c->prm.internal_project_pressure) launch_l20<double>(d + 3L * D.n_udof, D.n_pdof, c->red, c->stream);
+}
+template <typename T>
+void correct_nodes_t(pmg_ctx *c, const DevLevel &D, T *u) {
+  launch_dirichlet_slices<T>(u, nullptr, D.n_udof, D.hanging_bound, D.n_dirichlet, c->stream);
+}
+template <typename T>
+void set_dirichlet_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
+  launch_dirichlet_slices<T>(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, c->stream);
+}
+template <typename T>
+void correct_defect_t(pmg_ctx *c, const DevLevel &D, T *d) {
+  launch_dirichlet_slices<T>(d, nullptr, D.n_udof, D.n_active, D.n_udof - D.n_active, c->stream);
+  if (c->prm.internal_project_pressure) launch_l20<T>(d + 3L * D.n_udof, D.n_pdof, c->red, c->stream);
+}
+
+template <typename T>
+OpView<T> opview(const DevLevel &D);
+template <>
+OpView<double> opview<double>(const DevLevel &D) { return D.op64(); }
+template <>
+OpView<float> opview<float>(const DevLevel &D) { return D.op32(); }
+
+// TNSE_MGLevel4::CellVanka, colored (NSE_MGLevel4.C:603-1649), one sweep
+template <typename T>
+void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
+  Timed tm(c, &c->tm.sweep_ms, &c->tm.sweep_calls);
+  const double alpha = c->prm.sc_smooth_damp_factor_saddle;
+  const T damp = T(c->prm.sc_smooth_damp_factor_coarse_saddle);
+  const bool relax = std::fabs(1 - alpha) > 1e-3;
+  set_dirichlet_t<T>(c, D, u, f);
+  if (relax) launch_copy<T>(aux, u, D.n_dof, c->stream);
+  const OpView<T> op = opview<T>(D);
+  for (int col = 0; col < D.n_colors; ++col) {
+    const int p0 = D.ptr_colors[col], nc = D.ptr_colors[col + 1] - p0;
+    launch_vanka_residual<T>(op, D.cell_reorder, nc, p0, D.u_global, D.p_global, D.invperm, u, f, rloc,
+                             c->stream);
+    if constexpr (sizeof(T) == 8) {
+      launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global, D.n_udof, u,
+                                         damp, c->stream);
+    } else {
+      if (c->prm.precision == PMG_PREC_FP32_BF16)
+        launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global,
+                                            D.n_udof, u, damp, c->stream);
+      else
+        launch_vanka_solve<float, float>(D.lu32, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global, D.n_udof,
+                                         u, damp, c->stream);
+    }
+  }
+  if (relax) launch_relax<T>(u, aux, T(alpha), D.n_dof, c->stream);
+  set_dirichlet_t<T>(c, D, u, f);
+}
+
+template <typename T>
+void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, double *res_dev) {
+  Timed tm(c, &c->tm.defect_ms, &c->tm.defect_calls);
+  launch_type4<T>(opview<T>(D), 2, u, f, d, c->prm.internal_project_pressure != 0, res_dev, c->red, c->stream);
+}
+
+template <typename T>
+void restrict_defect(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *d, T *fc) {
+  Timed tm(c, &c->tm.transfer_ms, &c->tm.transfer_calls);
+  launch_csr_spmv<T>(F.ru, 3, d, F.n_udof, fc, C.n_udof, c->stream);
+  launch_csr_spmv<T>(F.rp, 1, d + 3L * F.n_udof, 0, fc + 3L * C.n_udof, 0, c->stream);
+}
+template <typename T>
+void prolongate(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *uc, T *d) {
+  Timed tm(c, &c->tm.transfer_ms, &c->tm.transfer_calls);
+  launch_csr_spmv<T>(F.pu, 3, uc, C.n_udof, d, F.n_udof, c->stream);
+  launch_csr_spmv<T>(F.pp, 1, uc + 3L * C.n_udof, 0, d + 3L * F.n_udof, 0, c->stream);
+}
+
+// coarse solve in FP64 (vectors of type T converted at the boundary)
+template <typename T>
+void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
+  Timed tm(c, &c->tm.coarse_ms, &c->tm.coarse_calls);
+  const int pin = c->prm.internal_project_pressure ? 3 * D.n_udof : -1;
+  if constexpr (sizeof(T) == 8) {
+    coarse_lu_solve(c->coarse, f, u, pin, c->stream);
+  } else {
+    launch_f2d(D.aux0, f, D.n_dof, c->stream);
+    coarse_lu_solve(c->coarse, D.aux0, D.sol, pin, c->stream);
+    launch_d2f(u, D.sol, D.n_dof, c->stream);
+  }
+}
+
+template <typename T>
+struct Vec {
+  static T *sol(DevLevel &D) { return D.sol; }
+  static T *rhs(DevLevel &D) { return D.rhs; }
+  static T *aux0(DevLevel &D) { return D.aux0; }
+  static T *aux1(DevLevel &D) { return D.aux1; }
+  static T *rloc(DevLevel &D) { return D.rloc; }
+};
+template <>
+struct Vec<float> {
+  static float *sol(DevLevel &D) { return D.sol32; }
+  static float *rhs(DevLevel &D) { return D.rhs32; }
+  static float *aux0(DevLevel &D) { return D.aux0_32; }
+  static float *aux1(DevLevel &D) { return D.aux1_32; }
+  static float *rloc(DevLevel &D) { return D.rloc32; }
+};
+
+// TNSE_MultiGrid::Cycle (NSE_MultiGrid.C:434-1779).  Defects whose only consumer is the
+// residual printout (:503, :1550, :1753 and the coarse-level one) are skipped: they do not
+// change the iterate.
+template <typename T>
+void cycle(pmg_ctx *c, int i) {
+  DevLevel &D = c->L[i];
+  T *u = Vec<T>::sol(D), *f = Vec<T>::rhs(D), *d = Vec<T>::aux0(D), *aux = Vec<T>::aux1(D);
+  correct_nodes_t<T>(c, D, u);
+  set_dirichlet_t<T>(c, D, u, f);
+  if (i == 0) {
+    coarse_solve_t<T>(c, D, u, f);
+    return;
+  }
+  DevLevel &C = c->L[i - 1];
+  for (int j = 0; j < c->prm.sc_pre_smooth_saddle; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D));
+  correct_nodes_t<T>(c, D, u);
+  mg_defect<T>(c, D, u, f, d, nullptr);
+  restrict_defect<T>(c, D, C, d, Vec<T>::rhs(C));
+  correct_defect_t<T>(c, C, Vec<T>::rhs(C));
+  launch_fill<T>(Vec<T>::sol(C), T(0), C.n_dof, c->stream);
+  cycle<T>(c, i - 1);
+  prolongate<T>(c, D, C, Vec<T>::sol(C), d);
+  correct_nodes_t<T>(c, D, d);
+  launch_axpy<T>(u, T(c->prm.sc_gmg_damp_factor_saddle), d, D.n_dof, c->stream);
+  correct_nodes_t<T>(c, D, u);
+  for (int j = 0; j < c->prm.sc_post_smooth_saddle; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D));
+}
+
+// TMultiGridIte::Iterate with zero start (MultiGridIte.C:76-130): z = M^{-1} v, device vectors
+void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
+  DevLevel &F = c->L.back();
+  const int top = int(c->L.size()) - 1;
+  if (!mixed(c)) {
+    launch_fill<double>(F.sol, 0.0, F.n_dof, c->stream);
+    launch_copy<double>(F.rhs, v, F.n_dof, c->stream);
+    set_dirichlet(c, F, F.sol, F.rhs);
+    cycle<double>(c, top);
+    launch_copy<double>(z, F.sol, F.n_dof, c->stream);
+  } else {
+    launch_fill<float>(F.sol32, 0.0f, F.n_dof, c->stream);
+    launch_d2f(F.rhs32, v, F.n_dof, c->stream);
+    set_dirichlet_t<float>(c, F, F.sol32, F.rhs32);
+    cycle<float>(c, top);
+    launch_f2d(z, F.sol32, F.n_dof, c->stream);
+  }
+}
+
+// ------------------------------------------------------------------ set-up
+void setup_factors(pmg_ctx *c) {
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    DevLevel &D = c->L[l];
+    if (D.factors_ready) continue;
+    auto t0 = std::chrono::steady_clock::now();
+    launch_patch_lu(D, D.lu, D.patch_info, c->stream);
+    if (D.lu32) launch_convert_lu_f32(D.lu, D.lu32, long(D.n_cells) * LU_STRIDE, c->stream);
+    if (D.lu16) launch_convert_lu_bf16(D.lu, D.lu16, long(D.n_cells) * LU_STRIDE, c->stream);
+    std::vector<int> info(D.n_cells);
+    PMG_HIP_CHECK(hipMemcpyAsync(info.data(), D.patch_info, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost,
+                                 c->stream));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->tm.patch_lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int p = 0; p < D.n_cells; ++p)
+      if (info[p]) {
+        const int pos = p;
+        throw Fail{PMG_ERR_SINGULAR_PATCH, "zero pivot " + std::to_string(info[pos]) + " in the patch at color position " +
+                                               std::to_string(pos) + " of level " + std::to_string(l)};
+      }
+    D.factors_ready = true;
+  }
+  if (!c->coarse.ready) {
+    auto t0 = std::chrono::steady_clock::now();
+    DevLevel &D = c->L[0];
+    Coarse &C = c->coarse;
+    if (!C.a) {
+      C.n = D.n_dof;
+      C.a = dalloc<double>(c, size_t(C.n) * C.n);
+      C.ipiv = dalloc<int>(c, C.n);
+      C.sigma = dalloc<int>(c, C.n);
+      C.b = dalloc<double>(c, C.n);
+      C.y = dalloc<double>(c, C.n);
+      C.info = dalloc<int>(c, 1);
+    }
+    launch_coarse_build(D, C.a, c->prm.internal_project_pressure != 0, c->stream);
+    const int info = coarse_lu_factor(C, c->stream);
+    c->tm.coarse_lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (info) throw Fail{PMG_ERR_SINGULAR_COARSE, "zero pivot " + std::to_string(info) + " in the coarse LU"};
+    C.ready = true;
+  }
+  c->setup_done = true;
+}
+
+void ensure_fgmres(pmg_ctx *c) {
+  const int m = std::max(1, std::min(c->prm.sc_gmres_restart > 0 ? c->prm.sc_gmres_restart : 10,
+                                     std::max(1, c->prm.sc_lin_maxit_saddle)));
+  if (m <= c->restart_alloc) return;
+  const long n = c->L.back().n_dof;
+  dfree(c, c->V);
+  dfree(c, c->Z);
+  dfree(c, c->W);
+  dfree(c, c->Hdev);
+  dfree(c, c->coef);
+  c->V = dalloc<double>(c, size_t(m + 1) * n);
+  c->Z = dalloc<double>(c, size_t(m + 1) * n);
+  c->W = dalloc<double>(c, n);
+  c->Hdev = dalloc<double>(c, m + 2);
+  c->coef = dalloc<double>(c, m + 1);
+  if (2 * m + 8 > c->hpin_len) {
+    if (c->hpin) (void)hipHostFree(c->hpin);
+    c->hpin = nullptr;
+    PMG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->hpin), sizeof(double) * (2 * m + 8)));
+    c->hpin_len = 2 * m + 8;
+  }
+  c->restart_alloc = m;
+}
+
+double dev_norm2(pmg_ctx *c, const double *x, long n) {
+  launch_dot(x, x, n, c->Hdev, c->red, c->stream);
+  PMG_HIP_CHECK(hipMemcpyAsync(c->hpin, c->Hdev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  return c->hpin[0];
+}
+
+void defect_nse4_dev(pmg_ctx *c, const double *x, const double *b, double *r) {
+  DevLevel &F = c->L.back();
+  launch_type4<double>(F.op64(), 1, x, b, r, c->prm.internal_project_pressure != 0, nullptr, c->red, c->stream);
+}
+
+// GeneratePlaneRotation / ApplyPlaneRotation (FgmresIte.C:199-226)
+void gen_rot(double dx, double dy, double &cs, double &sn) {
+  if (dy == 0.0) {
+    cs = 1.0; sn = 0.0;
+  } else if (std::fabs(dy) > std::fabs(dx)) {
+    const double temp = dx / dy;
+    sn = 1.0 / std::sqrt(1.0 + temp * temp);
+    cs = temp * sn;
+  } else {
+    const double temp = dy / dx;
+    cs = 1.0 / std::sqrt(1.0 + temp * temp);
+    sn = temp * cs;
+  }
+}
+void app_rot(double &dx, double &dy, double cs, double sn) {
+  const double temp = cs * dx + sn * dy;
+  dy = -sn * dx + cs * dy;
+  dx = temp;
+}
+
+// UpdateGmresIterate (FgmresIte.C:228-247) back-solve on the host, then x += Z y on device
+void update_iterate(pmg_ctx *c, double *x, int k, std::vector<std::vector<double>> &H, std::vector<double> &s) {
+  for (int i = k; i >= 0; i--) {
+    s[i] /= H[i][i];
+    for (int j = i - 1; j >= 0; j--) s[j] -= H[j][i] * s[i];
+  }
+  const long n = c->L.back().n_dof;
+  for (int j = 0; j <= k; ++j) c->hpin[j] = s[j];
+  PMG_HIP_CHECK(hipMemcpyAsync(c->coef, c->hpin, sizeof(double) * (k + 1), hipMemcpyHostToDevice, c->stream));
+  launch_lincomb(x, c->Z, n, c->coef, k + 1, n, c->stream);
+}
+
+// TFgmresIte::Iterate, flexible branch (FgmresIte.C:250-677), device vectors
+int fgmres_dev(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *final_res) {
+  ensure_fgmres(c);
+  const pmg_params &P = c->prm;
+  DevLevel &F = c->L.back();
+  const long n = F.n_dof;
+  int maxit = P.sc_lin_maxit_saddle;
+  int restart = P.sc_gmres_restart;
+  if (restart > maxit) restart = maxit;
+  if (restart <= 0) restart = 10;
+  const double res_norm_min = P.sc_lin_res_norm_min_saddle, red_factor = P.sc_lin_red_factor_saddle;
+  const int minit = P.sc_minit;
+  std::vector<double> s(restart + 1), cs(restart + 1), sn(restart + 1);
+  std::vector<std::vector<double>> H(restart + 1, std::vector<double>(restart, 0.0));
+  auto v = [&](int i) { return c->V + size_t(i) * n; };
+  auto z = [&](int i) { return c->Z + size_t(i) * n; };
+  c->hist.clear();
+  defect_nse4_dev(c, sol, rhs, v(0));
+  double res = std::sqrt(dev_norm2(c, v(0), n));
+  double res0 = res, beta = res;
+  int maxite = maxit;
+  if (!std::isfinite(res)) {
+    *iters = 0; *final_res = res;
+    throw Fail{PMG_ERR_NONFINITE, "non-finite initial FGMRES residual"};
+  }
+  if (beta <= res_norm_min) {
+    if (minit == 0 || beta < 1e-20) {
+      *iters = 0; *final_res = res;
+      return PMG_OK;
+    }
+    maxite = minit;
+  }
+  c->hist.push_back(beta);
+  int j = 1;
+  while (j <= maxite) {
+    launch_scal(v(0), v(0), 1.0 / beta, n, c->stream);
+    std::fill(s.begin(), s.end(), 0.0);
+    s[0] = beta;
+    for (int i = 0; i < restart && j <= maxite; i++, j++) {
+      for (int k = 0; k < std::max(1, P.sc_amg_prec_it); ++k) vcycle_dev(c, v(i), z(i));
+      launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
+      // modified Gram-Schmidt, H[k][i] -> Hdev[k], ||w||^2 -> Hdev[i+1]
+      launch_dot(c->W, v(0), n, c->Hdev, c->red, c->stream);
+      for (int k = 0; k <= i; ++k) {
+        const double *vn = (k < i) ? v(k + 1) : c->W;
+        launch_axpy_dev_dot(c->W, c->Hdev + k, v(k), vn, n, c->Hdev + k + 1, c->red, c->stream);
+      }
+      PMG_HIP_CHECK(hipMemcpyAsync(c->hpin, c->Hdev, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, c->stream));
+      PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+      for (int k = 0; k <= i; ++k) H[k][i] = c->hpin[k];
+      H[i + 1][i] = std::sqrt(c->hpin[i + 1]);
+      if (H[i + 1][i] != 0)
+        launch_scal(v(i + 1), c->W, 1 / H[i + 1][i], n, c->stream);
+      else
+        launch_copy<double>(v(i + 1), c->W, n, c->stream);
+      for (int k = 0; k < i; k++) app_rot(H[k][i], H[k + 1][i], cs[k], sn[k]);
+      gen_rot(H[i][i], H[i + 1][i], cs[i], sn[i]);
+      app_rot(H[i][i], H[i + 1][i], cs[i], sn[i]);
+      app_rot(s[i], s[i + 1], cs[i], sn[i]);
+      res = std::fabs(s[i + 1]);
+      c->hist.push_back(res);
+      if (!std::isfinite(res)) {
+        *iters = j; *final_res = res;
+        throw Fail{PMG_ERR_NONFINITE, "non-finite FGMRES residual at iteration " + std::to_string(j)};
+      }
+      if ((j >= minit) && ((res < res_norm_min) || (res < red_factor * res0))) {
+        update_iterate(c, sol, i, H, s);
+        PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+        *iters = j; *final_res = res;
+        return PMG_OK;
+      }
+      if (res > P.sc_div_factor * res0) {
+        *iters = j; *final_res = res;
+        throw Fail{PMG_ERR_DIVERGED, "fgmres iteration diverges (res " + std::to_string(res) + ")"};
+      }
+    }
+    update_iterate(c, sol, restart - 1, H, s);
+    defect_nse4_dev(c, sol, rhs, v(0));
+    res = res0 = beta = std::sqrt(dev_norm2(c, v(0), n));
+    c->hist.push_back(res);
+  }
+  if (j > maxite) j = maxite;
+  *iters = j; *final_res = res;
+  if (res > res_norm_min && res > res0 * red_factor) return PMG_ERR_NOT_CONVERGED;
+  return PMG_OK;
+}
+
+int guard(pmg_ctx *c, const std::function<void()> &f) {
+  try {
+    f();
+    return PMG_OK;
+  } catch (const Fail &e) {
+    if (c) c->err = e.msg;
+    return e.code;
+  } catch (const DeviceError &e) {
+    if (c) c->err = e.what;
+    return PMG_ERR_DEVICE;
+  } catch (const std::bad_alloc &) {
+    if (c) c->err = "host allocation failed";
+    return PMG_ERR_OUT_OF_MEMORY;
+  }
+}
+
+DevLevel &level_of(pmg_ctx *c, int l) {
+  if (l < 0 || l >= int(c->L.size())) throw Fail{PMG_ERR_INVALID_ARGUMENT, "level out of range"};
+  return c->L[l];
+}
+
+// copy-in/out helpers for PMG_MEM_HOST
+struct HostIO {
+  pmg_ctx *c;
+  int mem;
+  long n;
+  HostIO(pmg_ctx *c_, int m, long n_) : c(c_), mem(m), n(n_) {
+    if (mem != PMG_MEM_HOST && mem != PMG_MEM_DEVICE) throw Fail{PMG_ERR_INVALID_ARGUMENT, "bad memspace"};
+    if (mem == PMG_MEM_HOST) ensure_stage(c, n);
+  }
+  double *in(int slot, const double *p) {
+    if (!p) return nullptr;
+    if (mem == PMG_MEM_DEVICE) return const_cast<double *>(p);
+    PMG_HIP_CHECK(hipMemcpyAsync(c->stage[slot], p, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    return c->stage[slot];
+  }
+  void out(int slot, double *p) {
+    if (mem == PMG_MEM_DEVICE || !p) return;
+    PMG_HIP_CHECK(hipMemcpyAsync(p, c->stage[slot], sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  void sync() { PMG_HIP_CHECK(hipStreamSynchronize(c->stream)); }
+};
+
+}  // namespace
+
+// ===================================================================== C ABI
+extern "C" {
+
+void pmg_params_default(pmg_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  // src/General/Database.C:674-749, 871
+  p->sc_smoother_saddle = 2;
+  p->sc_pre_smooth_saddle = 2;
+  p->sc_post_smooth_saddle = 2;
+  p->sc_smooth_damp_factor_saddle = 1.0;
+  p->sc_smooth_damp_factor_coarse_saddle = 1.0;
+  p->sc_gmg_damp_factor_saddle = 1.0;
+  p->sc_coarse_smoother_saddle = 17;
+  p->internal_project_pressure = 1;
+  p->sc_gmres_restart = 10;
+  p->sc_lin_maxit_saddle = 10000;
+  p->sc_lin_red_factor_saddle = 0.0;
+  p->sc_lin_res_norm_min_saddle = 1e-10;
+  p->sc_div_factor = 1e10;
+  p->sc_minit = 0;
+  p->sc_amg_prec_it = 1;
+  p->precision = PMG_PREC_FP64;
+  p->bf16_refine_steps = 0;
+}
+
+const char *pmg_status_string(int s) {
+  switch (s) {
+    case PMG_OK: return "ok";
+    case PMG_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case PMG_ERR_DEVICE: return "device error";
+    case PMG_ERR_OUT_OF_MEMORY: return "out of memory";
+    case PMG_ERR_NOT_READY: return "not ready";
+    case PMG_ERR_SINGULAR_PATCH: return "singular local patch";
+    case PMG_ERR_SINGULAR_COARSE: return "singular coarse matrix";
+    case PMG_ERR_NONFINITE: return "non-finite value";
+    case PMG_ERR_DIVERGED: return "diverged";
+    case PMG_ERR_NOT_CONVERGED: return "not converged";
+    case PMG_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+static int check_params(const pmg_params *p, std::string &why) {
+  if (p->sc_smoother_saddle != 1 && p->sc_smoother_saddle != 2) { why = "only cell Vanka smoothers 1/2"; return PMG_ERR_UNSUPPORTED; }
+  if (p->sc_coarse_smoother_saddle != 17) { why = "only the dense coarse solve (17)"; return PMG_ERR_UNSUPPORTED; }
+  if (p->precision < 0 || p->precision > 2) { why = "bad precision"; return PMG_ERR_INVALID_ARGUMENT; }
+  if (p->sc_pre_smooth_saddle < 0 || p->sc_post_smooth_saddle < 0) { why = "negative smoothing steps"; return PMG_ERR_INVALID_ARGUMENT; }
+  return PMG_OK;
+}
+
+int pmg_ctx_create(int device, const pmg_params *params, pmg_ctx **out) {
+  if (!out) return PMG_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto c = std::make_unique<pmg_ctx>();
+  if (params) c->prm = *params; else pmg_params_default(&c->prm);
+  std::string why;
+  if (int st = check_params(&c->prm, why)) return st;
+  c->device = device;
+  int rc = guard(c.get(), [&] {
+    int n = 0;
+    PMG_HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Fail{PMG_ERR_INVALID_ARGUMENT, "no such device"};
+    PMG_HIP_CHECK(hipSetDevice(device));
+    PMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    PMG_HIP_CHECK(hipEventCreate(&c->ev0));
+    PMG_HIP_CHECK(hipEventCreate(&c->ev1));
+    c->red.partials = dalloc<double>(c.get(), 2 * RED_BLOCKS + 16);
+    c->red.counter = dalloc<unsigned>(c.get(), 16);
+    c->red.result = dalloc<double>(c.get(), 16);
+    PMG_HIP_CHECK(hipMemsetAsync(c->red.counter, 0, 16 * sizeof(unsigned), c->stream));
+    PMG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->hpin), sizeof(double) * 256));
+    c->hpin_len = 256;
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+  if (rc) return rc;
+  *out = c.release();
+  return PMG_OK;
+}
+
+int pmg_ctx_destroy(pmg_ctx *c) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->hpin) (void)hipHostFree(c->hpin);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PMG_OK;
+}
+
+const char *pmg_last_error_string(const pmg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
+  if (!c || !p) return PMG_ERR_INVALID_ARGUMENT;
+  std::string why;
+  if (int st = check_params(p, why)) { c->err = why; return st; }
+  const bool prec_change = p->precision != c->prm.precision;
+  const bool pin_change = p->internal_project_pressure != c->prm.internal_project_pressure;
+  if (prec_change && !c->L.empty()) { c->err = "precision is fixed once levels exist"; return PMG_ERR_UNSUPPORTED; }
+  c->prm = *p;
+  if (pin_change) c->coarse.ready = false;
+  return PMG_OK;
+}
+
+int pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_global, int n_udof, int *n_colors,
+                    int *ptr_colors, int *cell_reorder);
+
+int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
+  if (!c || !d) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    PMG_HIP_CHECK(hipSetDevice(c->device));
+    if (d->n_ulocal != NU_LOC || d->n_plocal != NP_LOC)
+      throw Fail{PMG_ERR_UNSUPPORTED, "only Q2/P1disc hexahedra (27 + 4 local dofs)"};
+    if (d->n_udof <= 0 || d->n_active < 0 || d->n_active > d->n_udof || d->n_pdof <= 0 || d->n_cells <= 0)
+      throw Fail{PMG_ERR_INVALID_ARGUMENT, "bad level sizes"};
+    if (d->hanging_bound != d->n_active) throw Fail{PMG_ERR_UNSUPPORTED, "hanging nodes are not supported"};
+    if (d->hanging_bound + d->n_dirichlet != d->n_udof) throw Fail{PMG_ERR_INVALID_ARGUMENT, "dof bounds inconsistent"};
+    const int lvl = int(c->L.size());
+    if (lvl > 0 && (!d->parent || !d->child || !d->qq_u || !d->qq_p || d->n_children <= 0))
+      throw Fail{PMG_ERR_INVALID_ARGUMENT, "fine level without parent/child/QQ data"};
+    // validate the CSR
+    if (d->a_rowptr[0] != 0 || d->a_rowptr[d->n_udof] != d->a_nnz || d->b_rowptr[d->n_pdof] != d->b_nnz ||
+        d->bt_rowptr[d->n_udof] != d->bt_nnz)
+      throw Fail{PMG_ERR_INVALID_ARGUMENT, "CSR row pointers inconsistent with nnz"};
+    auto check_csr = [](const int *rp, const int *kc, int rows, int ncols, const char *what) {
+      for (int r = 0; r < rows; ++r)
+        if (rp[r + 1] < rp[r]) throw Fail{PMG_ERR_INVALID_ARGUMENT, std::string(what) + ": row pointers decrease"};
+      for (int k = 0; k < rp[rows]; ++k)
+        if (kc[k] < 0 || kc[k] >= ncols) throw Fail{PMG_ERR_INVALID_ARGUMENT, std::string(what) + ": column out of range"};
+    };
+    check_csr(d->a_rowptr, d->a_kcol, d->n_udof, d->n_udof, "A");
+    check_csr(d->b_rowptr, d->b_kcol, d->n_pdof, d->n_udof, "B");
+    check_csr(d->bt_rowptr, d->bt_kcol, d->n_udof, d->n_pdof, "BT");
+    c->L.emplace_back();
+    DevLevel &D = c->L.back();
+    D.n_udof = d->n_udof; D.n_active = d->n_active; D.hanging_bound = d->hanging_bound;
+    D.n_dirichlet = d->n_dirichlet; D.n_pdof = d->n_pdof; D.n_dof = 3 * d->n_udof + d->n_pdof;
+    D.n_cells = d->n_cells;
+    D.a_nnz = d->a_nnz; D.b_nnz = d->b_nnz; D.bt_nnz = d->bt_nnz;
+    D.a_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.a_rowptr, d->a_rowptr, D.n_udof + 1);
+    D.a_kcol = dalloc<int>(c, D.a_nnz); h2d(c, D.a_kcol, d->a_kcol, D.a_nnz);
+    D.b_rowptr = dalloc<int>(c, D.n_pdof + 1); h2d(c, D.b_rowptr, d->b_rowptr, D.n_pdof + 1);
+    D.b_kcol = dalloc<int>(c, D.b_nnz); h2d(c, D.b_kcol, d->b_kcol, D.b_nnz);
+    D.bt_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.bt_rowptr, d->bt_rowptr, D.n_udof + 1);
+    D.bt_kcol = dalloc<int>(c, D.bt_nnz); h2d(c, D.bt_kcol, d->bt_kcol, D.bt_nnz);
+    D.a = dalloc<double>(c, 9 * size_t(D.a_nnz));
+    D.b = dalloc<double>(c, 3 * size_t(D.b_nnz));
+    D.bt = dalloc<double>(c, 3 * size_t(D.bt_nnz));
+    if (mixed(c)) {
+      D.a32 = dalloc<float>(c, 9 * size_t(D.a_nnz));
+      D.b32 = dalloc<float>(c, 3 * size_t(D.b_nnz));
+      D.bt32 = dalloc<float>(c, 3 * size_t(D.bt_nnz));
+    }
+    upload_values(c, D, d);
+    // dof maps (UGlobalNumbers + UBeginIndex[cell])
+    std::vector<int> ug(size_t(D.n_cells) * NU_LOC), pg(size_t(D.n_cells) * NP_LOC);
+    for (int cell = 0; cell < D.n_cells; ++cell) {
+      for (int j = 0; j < NU_LOC; ++j) {
+        const int g = d->u_global[d->u_begin[cell] + j];
+        if (g < 0 || g >= D.n_udof) throw Fail{PMG_ERR_INVALID_ARGUMENT, "velocity dof out of range"};
+        ug[size_t(cell) * NU_LOC + j] = g;
+      }
+      for (int j = 0; j < NP_LOC; ++j) {
+        const int g = d->p_global[d->p_begin[cell] + j];
+        if (g < 0 || g >= D.n_pdof) throw Fail{PMG_ERR_INVALID_ARGUMENT, "pressure dof out of range"};
+        pg[size_t(cell) * NP_LOC + j] = g;
+      }
+    }
+    D.u_global = dalloc<int>(c, ug.size()); h2d(c, D.u_global, ug.data(), ug.size());
+    D.p_global = dalloc<int>(c, pg.size()); h2d(c, D.p_global, pg.data(), pg.size());
+    // coloring (colorCell) - given or computed
+    std::vector<int> ptr(D.n_cells + 1), reo(D.n_cells);
+    int nc = 0;
+    if (d->ptr_colors && d->cell_reorder && d->n_colors > 0) {
+      nc = d->n_colors;
+      std::copy(d->ptr_colors, d->ptr_colors + nc + 1, ptr.begin());
+      std::copy(d->cell_reorder, d->cell_reorder + D.n_cells, reo.begin());
+    } else {
+      std::vector<int> ub(D.n_cells);
+      for (int i = 0; i < D.n_cells; ++i) ub[i] = i * NU_LOC;
+      int st = pmg_color_cells(D.n_cells, NU_LOC, ub.data(), ug.data(), D.n_udof, &nc, ptr.data(), reo.data());
+      if (st) throw Fail{st, "coloring failed"};
+    }
+    D.n_colors = nc;
+    D.ptr_colors.assign(ptr.begin(), ptr.begin() + nc + 1);
+    D.max_cells_per_color = 0;
+    for (int k = 0; k < nc; ++k) D.max_cells_per_color = std::max(D.max_cells_per_color, ptr[k + 1] - ptr[k]);
+    D.cell_reorder = dalloc<int>(c, D.n_cells); h2d(c, D.cell_reorder, reo.data(), D.n_cells);
+    // vectors
+    D.sol = dalloc<double>(c, D.n_dof); D.rhs = dalloc<double>(c, D.n_dof);
+    D.aux0 = dalloc<double>(c, D.n_dof); D.aux1 = dalloc<double>(c, D.n_dof);
+    launch_fill<double>(D.sol, 0.0, D.n_dof, c->stream);
+    launch_fill<double>(D.rhs, 0.0, D.n_dof, c->stream);
+    if (mixed(c)) {
+      D.sol32 = dalloc<float>(c, D.n_dof); D.rhs32 = dalloc<float>(c, D.n_dof);
+      D.aux0_32 = dalloc<float>(c, D.n_dof); D.aux1_32 = dalloc<float>(c, D.n_dof);
+    }
+    if (lvl > 0) {
+      D.lu = dalloc<double>(c, size_t(D.n_cells) * LU_STRIDE);
+      if (c->prm.precision == PMG_PREC_FP32) D.lu32 = dalloc<float>(c, size_t(D.n_cells) * LU_STRIDE);
+      if (c->prm.precision == PMG_PREC_FP32_BF16) D.lu16 = dalloc<uint16_t>(c, size_t(D.n_cells) * LU_STRIDE);
+      D.invperm = dalloc<uint8_t>(c, size_t(D.n_cells) * PERM_STRIDE);
+      D.patch_info = dalloc<int>(c, D.n_cells);
+      D.rloc = dalloc<double>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
+      if (mixed(c)) D.rloc32 = dalloc<float>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
+      // transfers from the coarser level
+      const DevLevel &C = c->L[lvl - 1];
+      for (int fc = 0; fc < D.n_cells; ++fc)
+        if (d->parent[fc] < 0 || d->parent[fc] >= C.n_cells || d->child[fc] < 0 || d->child[fc] >= d->n_children)
+          throw Fail{PMG_ERR_INVALID_ARGUMENT, "parent/child map out of range"};
+      build_transfer(c, NU_LOC, NU_LOC, D.n_cells, d->parent, d->child, d->qq_u, ug, c->h_u_global[lvl - 1],
+                     D.n_udof, C.n_udof, D.pu, D.ru);
+      build_transfer(c, NP_LOC, NP_LOC, D.n_cells, d->parent, d->child, d->qq_p, pg, c->h_p_global[lvl - 1],
+                     D.n_pdof, C.n_pdof, D.pp, D.rp);
+    }
+    c->h_u_global.push_back(std::move(ug));
+    c->h_p_global.push_back(std::move(pg));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->setup_done = false;
+    c->coarse.ready = false;
+    if (level_out) *level_out = lvl;
+  });
+}
+
+int pmg_level_update_values(pmg_ctx *c, int level, const pmg_level_desc *d) {
+  if (!c || !d) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    if (d->a_nnz != D.a_nnz || d->b_nnz != D.b_nnz || d->bt_nnz != D.bt_nnz)
+      throw Fail{PMG_ERR_INVALID_ARGUMENT, "pattern changed"};
+    upload_values(c, D, d);
+    D.factors_ready = false;
+    if (level == 0) c->coarse.ready = false;
+    c->setup_done = false;
+  });
+}
+
+int pmg_setup(pmg_ctx *c) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (c->L.empty()) throw Fail{PMG_ERR_NOT_READY, "no levels"};
+    PMG_HIP_CHECK(hipSetDevice(c->device));
+    setup_factors(c);
+  });
+}
+
+int pmg_level_info(const pmg_ctx *c, int level, int *n_dof, int *n_colors, int *mcpc) {
+  if (!c || level < 0 || level >= int(c->L.size())) return PMG_ERR_INVALID_ARGUMENT;
+  const DevLevel &D = c->L[level];
+  if (n_dof) *n_dof = D.n_dof;
+  if (n_colors) *n_colors = D.n_colors;
+  if (mcpc) *mcpc = D.max_cells_per_color;
+  return PMG_OK;
+}
+
+#define PMG_NEED_READY()                                                          \
+  do {                                                                            \
+    if (!c->setup_done) throw Fail{PMG_ERR_NOT_READY, "pmg_setup not called"};    \
+  } while (0)
+
+int pmg_defect(pmg_ctx *c, int level, const double *u, const double *f, double *d, double *res, int mem) {
+  if (!c || !u || !f || !d) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *du = io.in(0, u), *df = io.in(1, f), *dd = (mem == PMG_MEM_HOST) ? c->stage[2] : d;
+    mg_defect<double>(c, D, du, df, dd, c->red.result);
+    io.out(2, d);
+    PMG_HIP_CHECK(hipMemcpyAsync(c->hpin, c->red.result, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    io.sync();
+    if (res) *res = c->hpin[0];
+  });
+}
+
+int pmg_matvec(pmg_ctx *c, int level, const double *x, double *y, int mem) {
+  if (!c || !x || !y) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *dx = io.in(0, x), *dy = (mem == PMG_MEM_HOST) ? c->stage[2] : y;
+    launch_type4<double>(D.op64(), 0, dx, nullptr, dy, false, nullptr, c->red, c->stream);
+    io.out(2, y);
+    io.sync();
+  });
+}
+
+int pmg_defect_nse4(pmg_ctx *c, int level, const double *x, const double *b, double *r, int mem) {
+  if (!c || !x || !b || !r) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *dx = io.in(0, x), *db = io.in(1, b), *dr = (mem == PMG_MEM_HOST) ? c->stage[2] : r;
+    launch_type4<double>(D.op64(), 1, dx, db, dr, c->prm.internal_project_pressure != 0, nullptr, c->red,
+                         c->stream);
+    io.out(2, r);
+    io.sync();
+  });
+}
+
+int pmg_vanka_smooth(pmg_ctx *c, int level, double *u, const double *f, int nsweeps, int mem) {
+  if (!c || !u || !f || nsweeps < 0) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    PMG_NEED_READY();
+    if (level < 1) throw Fail{PMG_ERR_INVALID_ARGUMENT, "the coarsest level is solved directly"};
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *du = io.in(0, u), *df = io.in(1, f);
+    if (!mixed(c)) {
+      for (int s = 0; s < nsweeps; ++s) vanka_sweep<double>(c, D, du, df, D.aux1, D.rloc);
+    } else {
+      launch_d2f(D.sol32, du, D.n_dof, c->stream);
+      launch_d2f(D.rhs32, df, D.n_dof, c->stream);
+      for (int s = 0; s < nsweeps; ++s) vanka_sweep<float>(c, D, D.sol32, D.rhs32, D.aux1_32, D.rloc32);
+      launch_f2d(du, D.sol32, D.n_dof, c->stream);
+    }
+    io.out(0, u);
+    io.sync();
+  });
+}
+
+int pmg_restrict_defect(pmg_ctx *c, int level, const double *d_fine, double *f_coarse, int mem) {
+  if (!c || !d_fine || !f_coarse) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (level < 1) throw Fail{PMG_ERR_INVALID_ARGUMENT, "level must be >= 1"};
+    DevLevel &F = level_of(c, level), &C = level_of(c, level - 1);
+    HostIO io(c, mem, F.n_dof);
+    double *dd = io.in(0, d_fine), *dc = (mem == PMG_MEM_HOST) ? c->stage[2] : f_coarse;
+    restrict_defect<double>(c, F, C, dd, dc);
+    if (mem == PMG_MEM_HOST)
+      PMG_HIP_CHECK(hipMemcpyAsync(f_coarse, dc, sizeof(double) * C.n_dof, hipMemcpyDeviceToHost, c->stream));
+    io.sync();
+  });
+}
+
+int pmg_prolongate(pmg_ctx *c, int level, const double *u_coarse, double *d_fine, int mem) {
+  if (!c || !u_coarse || !d_fine) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (level < 1) throw Fail{PMG_ERR_INVALID_ARGUMENT, "level must be >= 1"};
+    DevLevel &F = level_of(c, level), &C = level_of(c, level - 1);
+    HostIO io(c, mem, F.n_dof);
+    double *uc = const_cast<double *>(u_coarse);
+    if (mem == PMG_MEM_HOST) {
+      PMG_HIP_CHECK(hipMemcpyAsync(c->stage[0], u_coarse, sizeof(double) * C.n_dof, hipMemcpyHostToDevice, c->stream));
+      uc = c->stage[0];
+    }
+    double *dd = (mem == PMG_MEM_HOST) ? c->stage[2] : d_fine;
+    prolongate<double>(c, F, C, uc, dd);
+    io.out(2, d_fine);
+    io.sync();
+  });
+}
+
+int pmg_correct_nodes(pmg_ctx *c, int level, double *u, int mem) {
+  if (!c || !u) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *du = io.in(0, u);
+    correct_nodes(c, D, du);
+    io.out(0, u);
+    io.sync();
+  });
+}
+
+int pmg_set_dirichlet(pmg_ctx *c, int level, double *u, const double *f, int mem) {
+  if (!c || !u || !f) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *du = io.in(0, u), *df = io.in(1, f);
+    set_dirichlet(c, D, du, df);
+    io.out(0, u);
+    io.sync();
+  });
+}
+
+int pmg_correct_defect(pmg_ctx *c, int level, double *d, int mem) {
+  if (!c || !d) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *dd = io.in(0, d);
+    correct_defect(c, D, dd);
+    io.out(0, d);
+    io.sync();
+  });
+}
+
+int pmg_update(pmg_ctx *c, int level, double *u, const double *d, int mem) {
+  if (!c || !u || !d) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    HostIO io(c, mem, D.n_dof);
+    double *du = io.in(0, u), *dd = io.in(1, d);
+    launch_axpy<double>(du, c->prm.sc_gmg_damp_factor_saddle, dd, D.n_dof, c->stream);
+    io.out(0, u);
+    io.sync();
+  });
+}
+
+int pmg_coarse_solve(pmg_ctx *c, double *u, const double *f, int mem) {
+  if (!c || !u || !f) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    PMG_NEED_READY();
+    DevLevel &D = level_of(c, 0);
+    HostIO io(c, mem, D.n_dof);
+    double *du = (mem == PMG_MEM_HOST) ? c->stage[0] : u, *df = io.in(1, f);
+    coarse_solve_t<double>(c, D, du, df);
+    io.out(0, u);
+    io.sync();
+  });
+}
+
+int pmg_vcycle(pmg_ctx *c, const double *v, double *z, int mem) {
+  if (!c || !v || !z) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    PMG_NEED_READY();
+    DevLevel &F = c->L.back();
+    HostIO io(c, mem, F.n_dof);
+    double *dv = io.in(0, v), *dz = (mem == PMG_MEM_HOST) ? c->stage[2] : z;
+    vcycle_dev(c, dv, dz);
+    io.out(2, z);
+    io.sync();
+  });
+}
+
+int pmg_fgmres_solve(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *final_res, int mem) {
+  if (!c || !sol || !rhs) return PMG_ERR_INVALID_ARGUMENT;
+  int it = 0;
+  double fr = 0;
+  int st = PMG_OK;
+  int rc = guard(c, [&] {
+    PMG_NEED_READY();
+    DevLevel &F = c->L.back();
+    HostIO io(c, mem, F.n_dof);
+    double *ds = io.in(0, sol), *dr = io.in(1, rhs);
+    st = fgmres_dev(c, ds, dr, &it, &fr);
+    io.out(0, sol);
+    io.sync();
+  });
+  if (iters) *iters = it;
+  if (final_res) *final_res = fr;
+  if (rc) return rc;
+  if (st == PMG_ERR_NOT_CONVERGED) c->err = "FGMRES not converged";
+  return st;
+}
+
+int pmg_fgmres_history(const pmg_ctx *c, double *out, int cap) {
+  if (!c || !out) return 0;
+  const int n = std::min<int>(cap, int(c->hist.size()));
+  for (int i = 0; i < n; ++i) out[i] = c->hist[i];
+  return n;
+}
+
+int pmg_device_alloc(pmg_ctx *c, size_t bytes, void **ptr) {
+  if (!c || !ptr) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] { *ptr = dalloc<char>(c, bytes); });
+}
+int pmg_device_free(pmg_ctx *c, void *ptr) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  dfree(c, ptr);
+  return PMG_OK;
+}
+int pmg_memcpy(pmg_ctx *c, void *dst, const void *src, size_t bytes, int kind) {
+  if (!c || kind < 0 || kind > 2) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    PMG_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+int pmg_synchronize(pmg_ctx *c) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] { PMG_HIP_CHECK(hipStreamSynchronize(c->stream)); });
+}
+int pmg_timers_get(const pmg_ctx *c, pmg_timers *t) {
+  if (!c || !t) return PMG_ERR_INVALID_ARGUMENT;
+  *t = c->tm;
+  return PMG_OK;
+}
+int pmg_timers_reset(pmg_ctx *c) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  c->tm = pmg_timers{};
+  return PMG_OK;
+}
+int pmg_set_profiling(pmg_ctx *c, int on) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  c->profiling = on != 0;
+  return PMG_OK;
+}
+
+}  // extern "C"

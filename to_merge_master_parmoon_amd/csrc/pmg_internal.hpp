@@ -1,0 +1,176 @@
+// pmg_internal.hpp - device data structures and kernel launchers of libpmg (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pmg.h"
+
+namespace pmg {
+
+constexpr int NU_LOC = 27;          // Q2 hex velocity dofs per cell (per component)
+constexpr int NP_LOC = 4;           // P1disc hex pressure dofs per cell
+constexpr int NLOC = 3 * NU_LOC + NP_LOC;  // 85: local Vanka system size
+constexpr int LU_STRIDE = 7232;     // 85*85 = 7225 padded to a multiple of 16 entries
+constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
+constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
+constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
+
+// Type-4 operator in structure-of-arrays form: A values a[b*a_nnz + k] for the 9 blocks
+// A11 A12 A13 A21 A22 A23 A31 A32 A33, B/BT values b[c*b_nnz + k], bt[c*bt_nnz + k].
+template <typename T>
+struct OpView {
+  int n_udof, n_active, n_pdof;
+  const int *a_rowptr, *a_kcol;
+  const T *a;
+  long a_nnz;
+  const int *b_rowptr, *b_kcol;
+  const T *b;
+  long b_nnz;
+  const int *bt_rowptr, *bt_kcol;
+  const T *bt;
+  long bt_nnz;
+};
+
+struct Csr {
+  int n_rows = 0, n_cols = 0, nnz = 0;
+  int *rowptr = nullptr, *col = nullptr;
+  double *val = nullptr;
+  float *val32 = nullptr;
+};
+
+struct DevLevel {
+  int n_udof = 0, n_active = 0, hanging_bound = 0, n_dirichlet = 0, n_pdof = 0, n_dof = 0;
+  int n_cells = 0;
+  long a_nnz = 0, b_nnz = 0, bt_nnz = 0;
+  // operator, FP64 and (mixed precision) FP32 copies
+  int *a_rowptr = nullptr, *a_kcol = nullptr, *b_rowptr = nullptr, *b_kcol = nullptr;
+  int *bt_rowptr = nullptr, *bt_kcol = nullptr;
+  double *a = nullptr, *b = nullptr, *bt = nullptr;
+  float *a32 = nullptr, *b32 = nullptr, *bt32 = nullptr;
+  // cells
+  int *u_global = nullptr;           // n_cells * 27 (UGlobalNumbers + UBeginIndex[cell])
+  int *p_global = nullptr;           // n_cells * 4
+  // coloring (host copy of the color pointers; device cell list in color order)
+  int n_colors = 0, max_cells_per_color = 0;
+  std::vector<int> ptr_colors;
+  int *cell_reorder = nullptr;
+  // patch factors in color order (position = index into cell_reorder)
+  double *lu = nullptr;
+  float *lu32 = nullptr;
+  uint16_t *lu16 = nullptr;          // bf16 factors
+  uint8_t *invperm = nullptr;        // n_cells * PERM_STRIDE
+  int *patch_info = nullptr;         // n_cells
+  bool factors_ready = false;
+  // transfers to the next coarser level (this level = fine)
+  Csr pu, ru, pp, rp;                // P_U (fine rows), R_U = P_U^T, P_P, R_P
+  // MG vectors: sol, rhs, aux0 (defect), aux1 (aux) as the reference level owns them
+  double *sol = nullptr, *rhs = nullptr, *aux0 = nullptr, *aux1 = nullptr;
+  float *sol32 = nullptr, *rhs32 = nullptr, *aux0_32 = nullptr, *aux1_32 = nullptr;
+  double *rloc = nullptr;            // max_cells_per_color * RLOC_STRIDE
+  float *rloc32 = nullptr;
+  // host copies kept for value updates / coarse assembly
+  std::vector<int> h_a_rowptr, h_a_kcol, h_b_rowptr, h_b_kcol, h_bt_rowptr, h_bt_kcol;
+
+  OpView<double> op64() const {
+    return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
+                          b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz};
+  }
+  OpView<float> op32() const {
+    return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a32, a_nnz,
+                         b_rowptr, b_kcol, b32, b_nnz, bt_rowptr, bt_kcol, bt32, bt_nnz};
+  }
+};
+
+struct Coarse {
+  int n = 0;
+  double *a = nullptr;     // dense column-major LU
+  int *ipiv = nullptr;     // device
+  int *sigma = nullptr;    // b'[i] = b[sigma[i]]
+  double *b = nullptr;     // work vectors of the two sweeps
+  double *y = nullptr;
+  int *info = nullptr;
+  bool ready = false;
+};
+
+// reduction scratch: partial sums + a completion counter per reduction slot
+struct RedScratch {
+  double *partials = nullptr;   // RED_BLOCKS * 4
+  unsigned *counter = nullptr;  // one per slot
+  double *result = nullptr;     // result slots
+};
+
+// ------------------------------------------------------------------ kernel launchers
+// type-4 operator: mode 0 y = K x, 1 y = b - K x (Defect_NSE4), 2 MG defect (Dirichlet rows 0)
+// project: IntoL20 of the pressure constants; res (device, may be null): ||y||_2
+template <typename T>
+void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project,
+                  double *res, RedScratch &red, hipStream_t s);
+
+// Vanka
+void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s);
+template <typename T>
+void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0,
+                           const int *u_global, const int *p_global, const uint8_t *invperm,
+                           const T *u, const T *f, T *rloc, hipStream_t s);
+template <typename T, typename F>
+void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc,
+                        const int *u_global, const int *p_global, int n_udof, T *u, T damp,
+                        hipStream_t s);
+void launch_convert_lu_f32(const double *lu, float *lu32, long n, hipStream_t s);
+void launch_convert_lu_bf16(const double *lu, uint16_t *lu16, long n, hipStream_t s);
+
+// vectors
+template <typename T>
+void launch_fill(T *x, T v, long n, hipStream_t s);
+template <typename T>
+void launch_copy(T *dst, const T *src, long n, hipStream_t s);
+template <typename T>
+void launch_axpy(T *y, T a, const T *x, long n, hipStream_t s);         // y += a x
+template <typename T>
+void launch_relax(T *u, const T *uold, T alpha, long n, hipStream_t s); // u = uold + a (u-uold)
+void launch_d2f(float *dst, const double *src, long n, hipStream_t s);
+void launch_f2d(double *dst, const float *src, long n, hipStream_t s);
+template <typename T>
+void launch_dirichlet_slices(T *u, const T *f, int n_udof, int start, int count, hipStream_t s);
+template <typename T>
+void launch_l20(T *p, int n_pdof, RedScratch &red, hipStream_t s);
+
+// transfers: y[c*ny + r] = sum_k val[k] x[c*nx + col[k]], c < ncomp
+template <typename T>
+void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long x_stride, T *y, long y_stride,
+                     hipStream_t s);
+
+// dense coarse LU / solve
+void launch_coarse_build(const DevLevel &L, double *a, bool pin, hipStream_t s);
+int  coarse_lu_factor(Coarse &C, hipStream_t s);
+void coarse_lu_solve(Coarse &C, const double *f, double *u, int pin_index, hipStream_t s);
+
+// BLAS-1 for FGMRES (deterministic two-stage reductions)
+void launch_dot(const double *x, const double *y, long n, double *out, RedScratch &red, hipStream_t s);
+// w -= h[0] * v ; then out = dot(w, vnext) (vnext may be null)
+void launch_axpy_dev_dot(double *w, const double *h, const double *v, const double *vnext, long n,
+                         double *out, RedScratch &red, hipStream_t s);
+void launch_scale_dev(double *y, const double *x, const double *den, long n, hipStream_t s);  // y = x/den
+// x += (sum_j coef[j] z_j), z_j = zbase + j*stride  (UpdateGmresIterate + Daxpy)
+void launch_lincomb(double *x, const double *zbase, long stride, const double *coef, int m, long n,
+                    hipStream_t s);
+void launch_scal(double *y, const double *x, double a, long n, hipStream_t s);  // y = a x
+
+}  // namespace pmg
+
+#define PMG_HIP_CHECK(expr)                                                          \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) throw pmg::DeviceError(std::string(#expr) + ": " +          \
+                                                 hipGetErrorString(e_));             \
+  } while (0)
+
+namespace pmg {
+struct DeviceError {
+  std::string what;
+  explicit DeviceError(std::string w) : what(std::move(w)) {}
+};
+}  // namespace pmg
