@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""Oseen3D 32^3 Q2/P1disc: FGMRES(30) + GMG cell-Vanka solve to rtol 1e-10 on one MI355X.
+
+One step = one complete solve of the Oseen system with the operator resident in HBM:
+the value-dependent set-up (batched LU of every 85x85 Vanka patch on levels 8^3..32^3 and
+the dense LU of the 4^3 level) followed by FGMRES from x = 0 until the restart-cycle
+relative residual drops below 1e-10, each FGMRES iteration preconditioned by one V(2,2)
+cycle.  ``value`` is seconds per solve (lower is better).
+
+A separate profiling pass then times every hot-kernel launch with HIP events on the
+library's stream and divides the kernel's algorithmic bytes by its time (roofline), and
+the FP64 CPU oracle (reference-faithful: patch matrices re-assembled and LU-factored in
+every sweep, coarse matrix refactored at every visit) is timed for a bounded number of
+FGMRES iterations on the host cores and scaled to the GPU's iteration count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Oseen3D 32^3 Q2/P1disc solve s to 1e-10: mixed/batched vs FP64 GMG-Vanka"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def solver_params(precision: int):
+    from to_merge_master_parmoon_amd import _abi
+    # config 2/3: FGMRES(30), SC_LIN_RED_FACTOR_SADDLE = 1e-10, SC_LIN_RES_NORM_MIN_SADDLE = 0,
+    # SC_LIN_MAXIT_SADDLE = 1000, SC_AMG_PREC_IT = 1, V(2,2), damp = alpha = beta = 1
+    return _abi.default_params(sc_gmres_restart=30, sc_lin_red_factor_saddle=1e-10,
+                               sc_lin_res_norm_min_saddle=0.0, sc_lin_maxit_saddle=1000,
+                               sc_pre_smooth_saddle=2, sc_post_smooth_saddle=2, precision=precision)
+
+
+def run_gpu(prob, precision: int, steps: int, warmup: int, log):
+    from to_merge_master_parmoon_amd.multigrid import DeviceVector, build_hierarchy
+    t0 = time.time()
+    mg = build_hierarchy(prob, solver_params(precision))
+    log(f"hierarchy on device ({time.time() - t0:.1f}s), levels {mg.n_dof}")
+    n = mg.n_dof[-1]
+    rhs = DeviceVector(mg, n, prob.rhs(-1))
+    zero = DeviceVector(mg, n, np.zeros(n))
+    sol = DeviceVector(mg, n)
+    lib, ctx = mg._lib, mg._ctx
+
+    def step():
+        mg._check(lib.pmg_memcpy(ctx, sol.ptr, zero.ptr, 8 * n, 2))
+        mg.refactor()
+        return mg.fgmres_device(sol, rhs)
+
+    for w in range(warmup):
+        it, res = step()
+        log(f"warmup {w}: {it} iterations, res {res:.3e}")
+    mg.synchronize()
+    times, iters = [], []
+    for k in range(steps):
+        mg.synchronize()
+        t = time.perf_counter()
+        it, res = step()
+        mg.synchronize()
+        times.append(time.perf_counter() - t)
+        iters.append(it)
+        log(f"step {k}: {times[-1] * 1e3:.1f} ms, {it} iterations, res {res:.3e}")
+    x = sol.download()
+    hist = mg.fgmres_history()
+    # profiling pass: per-launch HIP events (synchronizing; not part of the timed steps)
+    mg.reset_timers()
+    mg.set_profiling(True)
+    step()
+    mg.set_profiling(False)
+    tm = mg.timers()
+    return dict(times=times, iters=iters, x=x, hist=hist, timers=tm, mg=mg)
+
+
+def roofline_from(tm: dict):
+    kern = {k: v for k, v in tm.items() if isinstance(v, dict) and v["calls"] > 0}
+    solve_kernels = {k: v for k, v in kern.items() if k not in ("patch_lu", "coarse_lu")}
+    dom = max(solve_kernels, key=lambda k: solve_kernels[k]["ms"])
+    v = kern[dom]
+    avg_ms = v["ms"] / v["calls"]
+    bytes_per_launch = v["bytes"] / v["calls"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    breakdown = {k: dict(ms=round(x["ms"], 3), calls=int(x["calls"]),
+                         avg_us=round(1e3 * x["ms"] / x["calls"], 2),
+                         gbs=round(x["bytes"] / (x["ms"] * 1e-3) / 1e9, 1) if x["ms"] > 0 else None)
+                 for k, x in kern.items()}
+    names = {"vanka_res": "k_vanka_res", "vanka_solve": "k_vanka_solve", "defect": "k_type4<2>",
+             "matvec": "k_type4<0/1>", "transfer": "k_csr", "coarse_solve": "k_trsv_fwd/bwd"}
+    roof = {"bound": "hbm", "kernel": names.get(dom, dom), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": round(bytes_per_launch), "avg_launch_us": round(avg_ms * 1e3, 2)}
+    return roof, breakdown
+
+
+def cpu_baseline(prob, gpu_iters: int, cpu_iters: int, log):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pmg_oracle
+    threads = min(16, os.cpu_count() or 1)
+    pmg_oracle.set_threads(threads)
+    prm = solver_params(0)
+    prm.sc_lin_maxit_saddle = cpu_iters
+    orc = pmg_oracle.Oracle(prm, prob.descs, refactor=1)
+    f = prob.rhs(-1).copy()
+    x = np.zeros_like(f)
+    t = time.perf_counter()
+    it, res, st = orc.fgmres(x, f)
+    dt = time.perf_counter() - t
+    orc.close()
+    per_it = dt / max(it, 1)
+    log(f"cpu oracle: {it} iterations in {dt:.1f}s ({per_it:.2f} s/iteration, {threads} threads)")
+    return {"value": round(per_it * gpu_iters, 3), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"{it} FGMRES iterations of the reference-faithful FP64 oracle on the 32^3 problem "
+                      f"(per-sweep patch re-assembly + LU, per-visit coarse LU) timed: {dt:.2f} s; "
+                      f"scaled to the GPU's {gpu_iters} iterations"}
+
+
+def rel_err(x, ref, n_u3):
+    """relative L2 differences of the velocity block and of the pressure block; the
+    pressure is compared modulo its constant mode (mean of the P1disc constants), which
+    the all-Dirichlet Oseen operator leaves undetermined"""
+    du = np.linalg.norm(x[:n_u3] - ref[:n_u3]) / max(np.linalg.norm(ref[:n_u3]), 1e-300)
+    px, pr = x[n_u3:].copy(), ref[n_u3:].copy()
+    px[0::4] -= px[0::4].mean()
+    pr[0::4] -= pr[0::4].mean()
+    dp = np.linalg.norm(px - pr) / max(np.linalg.norm(pr), 1e-300)
+    return float(du), float(dp)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=32, help="cells per edge of the finest level")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp32",
+                    help="smoother/level-operator arithmetic of the measured solve (FGMRES is FP64)")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-ref", action="store_true", help="skip the FP64 GPU solve timed beside it")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition "
+              "of this solver is described in DESIGN.md and not launched.", file=sys.stderr)
+        sys.exit(2)
+    from to_merge_master_parmoon_amd import problem
+    prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]
+    t0 = time.time()
+    prob = problem.oseen3d(args.n)
+    sz = prob.sizes(-1)
+    n_u3 = 3 * sz["n_udof"]
+    log(f"problem {args.n}^3: {sz['n_dof']} dofs, nnz(A) {sz['a_nnz']} ({time.time() - t0:.1f}s)")
+    ref = None
+    if prec != 0 and not args.no_fp64_ref:
+        log("FP64 GPU solve (plain form, timed beside the measured one)")
+        ref = run_gpu(prob, 0, max(1, args.steps), min(args.warmup, 1), log)
+        ref["mg"].close()
+    log(f"measured solve: {args.precision}")
+    g = run_gpu(prob, prec, args.steps, args.warmup, log)
+    ms = 1e3 * float(np.mean(g["times"]))
+    roof, breakdown = roofline_from(g["timers"])
+    iters = int(g["iters"][-1])
+    cfg = {"workload": f"oseen3d_{args.n}^3_q2p1disc_fgmres30_vcycle22", "levels": len(g['mg'].n_dof),
+           "dofs": sz["n_dof"], "precision": args.precision, "nu": 1e-2,
+           "fgmres_iterations": iters, "final_residual": float(g["hist"][-1]) if len(g["hist"]) else None,
+           "setup_ms": round(max(g["timers"]["patch_lu"]["ms"], g["timers"]["coarse_lu"]["ms"]), 3)}
+    if ref is not None:
+        fp64_ms = 1e3 * float(np.mean(ref["times"]))
+        du, dp = rel_err(g["x"], ref["x"], n_u3)
+        cfg.update({"fp64_solve_s": round(fp64_ms / 1e3, 5), "fp64_iterations": int(ref["iters"][-1]),
+                    "speedup_vs_fp64": round(fp64_ms / ms, 3),
+                    "rel_err_vs_fp64": {"velocity": du, "pressure": dp}})
+    out = {
+        "metric": METRIC,
+        "value": round(ms / 1e3, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "fp32": "f32 smoother+level operator / f64 FGMRES+coarse",
+                  "bf16": "bf16 patch factors / f32 smoother / f64 FGMRES+coarse"}[args.precision],
+        "data": "synthetic",
+        "config": cfg,
+        "roofline": roof,
+        "kernels": breakdown,
+    }
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(prob, iters, args.cpu_iters, log)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

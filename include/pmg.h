@@ -131,6 +131,11 @@ int  pmg_level_update_values(pmg_ctx *ctx, int level, const pmg_level_desc *desc
    The reference re-factors inside every sweep (NSE_MGLevel4.C:1400-1609) and every coarse
    visit (NSE_MGLevel4.C:5728-5846); the factors are the same, so they are kept. */
 int  pmg_setup(pmg_ctx *ctx);
+/* Recompute every value-dependent factor (patch LUs of all levels >= 1, coarse LU) from the
+   operator already resident on the device.  The reference repeats this work inside every
+   sweep and coarse visit; callers that change values in device memory, or that time a
+   complete solve including its set-up, call this instead of re-adding levels. */
+int  pmg_refactor(pmg_ctx *ctx);
 int  pmg_level_info(const pmg_ctx *ctx, int level, int *n_dof, int *n_colors, int *max_cells_per_color);
 
 /* ----------------------------------------------------------------------------------- */
@@ -206,13 +211,24 @@ int  pmg_device_free(pmg_ctx *ctx, void *ptr);
 int  pmg_memcpy(pmg_ctx *ctx, void *dst, const void *src, size_t bytes, int kind); /* 0 H2D 1 D2H 2 D2D */
 int  pmg_synchronize(pmg_ctx *ctx);
 
-/* kernel timing of the last call (ms), measured with HIP events on the library's stream */
+/* Per-kernel timing, measured with HIP events on the library's stream around every launch
+   while profiling is on (pmg_set_profiling; it synchronizes after each launch, so it is
+   meant for a separate measurement pass).  'bytes' accumulates the algorithmic HBM bytes
+   of the timed launches (every operator value, index, factor and vector entry the
+   algorithm needs read or written once; gathers of x counted as cache hits). */
 typedef struct {
-  double defect_ms;   int defect_calls;
-  double sweep_ms;    int sweep_calls;      /* one colored Vanka sweep (all colors)      */
-  double transfer_ms; int transfer_calls;
-  double coarse_ms;   int coarse_calls;
-  double patch_lu_ms; double coarse_lu_ms;
+  double ms; long long calls; double bytes; double flops;
+} pmg_kstat;
+typedef struct {
+  pmg_kstat defect;        /* k_type4 in the V-cycle and the level defect                */
+  pmg_kstat matvec;        /* k_type4 as FGMRES operator / Defect_NSE4                    */
+  pmg_kstat vanka_res;     /* per-color local residuals                                   */
+  pmg_kstat vanka_solve;   /* per-color stored-factor local solves + update               */
+  pmg_kstat transfer;      /* restriction + prolongation SpMVs                            */
+  pmg_kstat coarse_solve;  /* coarse triangular sweeps                                    */
+  pmg_kstat patch_lu;      /* batched patch LU (set-up)                                   */
+  pmg_kstat coarse_lu;     /* coarse dense LU (set-up)                                    */
+  double sweep_ms; long long sweep_calls;   /* whole colored sweeps                       */
 } pmg_timers;
 int  pmg_timers_get(const pmg_ctx *ctx, pmg_timers *t);
 int  pmg_timers_reset(pmg_ctx *ctx);

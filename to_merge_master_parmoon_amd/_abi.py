@@ -60,14 +60,15 @@ class PmgLevelDesc(C.Structure):
     ]
 
 
+class PmgKstat(C.Structure):
+    _fields_ = [("ms", C.c_double), ("calls", C.c_longlong), ("bytes", C.c_double), ("flops", C.c_double)]
+
+
+KSTATS = ["defect", "matvec", "vanka_res", "vanka_solve", "transfer", "coarse_solve", "patch_lu", "coarse_lu"]
+
+
 class PmgTimers(C.Structure):
-    _fields_ = [
-        ("defect_ms", C.c_double), ("defect_calls", C.c_int),
-        ("sweep_ms", C.c_double), ("sweep_calls", C.c_int),
-        ("transfer_ms", C.c_double), ("transfer_calls", C.c_int),
-        ("coarse_ms", C.c_double), ("coarse_calls", C.c_int),
-        ("patch_lu_ms", C.c_double), ("coarse_lu_ms", C.c_double),
-    ]
+    _fields_ = [(k, PmgKstat) for k in KSTATS] + [("sweep_ms", C.c_double), ("sweep_calls", C.c_longlong)]
 
 
 class PmgpConfig(C.Structure):
@@ -97,6 +98,7 @@ PMG_FUNCS = {
     "pmg_level_add": (C.c_int, [C.c_void_p, C.POINTER(PmgLevelDesc), c_int_p]),
     "pmg_level_update_values": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PmgLevelDesc)]),
     "pmg_setup": (C.c_int, [C.c_void_p]),
+    "pmg_refactor": (C.c_int, [C.c_void_p]),
     "pmg_level_info": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_defect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, C.c_int]),
     "pmg_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),

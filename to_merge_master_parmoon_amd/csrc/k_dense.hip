@@ -3,19 +3,24 @@
 //
 // SolveExact builds the dense column-major matrix of the coarsest level with the row of
 // pressure dof 0 replaced by the identity and calls LAPACK dgetrf/dgetrs at every visit.
-// Here the matrix is built and factored on the device once per operator, with a blocked
-// right-looking LU that applies, to every entry, the same fma sequence as the unblocked
-// dgetf2 of the oracle (panel factorization, row swaps, triangular solve of the U block
-// row, and a trailing update whose per-entry fma chain runs over the panel in order), so
-// the factors agree bit for bit.  Each visit then runs blocked column sweeps that keep
-// the oracle's per-entry update order.
+// Its Dirichlet rows are identity rows too (only the diagonal entry of the diagonal
+// blocks), so x_D = f_D / diag and x_pin = 0 are known, and the remaining unknowns solve
+// the dense system of the active velocity and free pressure dofs with the right-hand side
+// f_I - K_ID x_D: the same solution with 3*N_Dirichlet + 1 fewer unknowns (for the 4^3
+// level 1284 instead of 2443, 7x fewer LU flops).  That system is built and factored on the
+// device once per operator by a blocked right-looking LU with partial pivoting whose
+// per-entry update order is that of the unblocked dgetf2 (panel factorization, row swaps,
+// triangular solve of the U block row, trailing update as an fma chain over the panel in
+// order); each visit runs blocked column sweeps in the dgetrs order.
 #include "pmg_internal.hpp"
 
 namespace pmg {
 
 namespace {
 
-constexpr int NB = 32;     // LU panel width
+constexpr int NB = 32;     // max LU panel width (trsm / gemm tiles)
+constexpr int NBR = 16;    // panel width of the register-resident panel factorization
+constexpr int PNT = 512;   // threads of the register-resident panel factorization
 constexpr int TB = 64;     // triangular-solve block (one wavefront)
 
 __device__ inline double bcast(double v, int lane) {
@@ -25,39 +30,36 @@ __device__ inline double bcast(double v, int lane) {
 }
 
 // ------------------------------------------------------------------ dense assembly
-// :5739-5833: A blocks (active rows all nine, other rows the diagonal blocks), BT, B
-__global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long n) {
+// reduced numbering: velocity (component b, active dof r) -> b*na + r, pressure q -> 3na+q-pin
+__device__ inline long cvel(int b, int r, int na) { return long(b) * na + r; }
+__device__ inline long cpre(int q, int na, int pin) { return 3L * na + q - pin; }
+
+// :5739-5833 restricted to the kept rows and columns: A blocks of active rows/columns,
+// BT columns except the pinned one, B rows except the pinned one
+__global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long n, int pin) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
-  const long G = 3L * N;
-  if (r < N) {
+  if (r < NA) {
     for (int k = op.a_rowptr[r]; k < op.a_rowptr[r + 1]; ++k) {
-      const long c = op.a_kcol[k];
-      if (r < NA) {
-        for (int br = 0; br < 3; ++br)
-          for (int bc = 0; bc < 3; ++bc)
-            a[(c + bc * long(N)) * n + r + br * long(N)] = op.a[(3 * br + bc) * op.a_nnz + k];
-      } else {
-        for (int b = 0; b < 3; ++b) a[(c + b * long(N)) * n + r + b * long(N)] = op.a[(4 * b) * op.a_nnz + k];
-      }
+      const int c = op.a_kcol[k];
+      if (c >= NA) continue;
+      for (int br = 0; br < 3; ++br)
+        for (int bc = 0; bc < 3; ++bc) a[cvel(bc, c, NA) * n + cvel(br, r, NA)] = op.a[(3 * br + bc) * op.a_nnz + k];
     }
-    if (r < NA)
-      for (int k = op.bt_rowptr[r]; k < op.bt_rowptr[r + 1]; ++k) {
-        const long l = (op.bt_kcol[k] + G) * n;
-        for (int b = 0; b < 3; ++b) a[l + r + b * long(N)] = op.bt[b * op.bt_nnz + k];
-      }
-  } else if (r < N + NP) {
+    for (int k = op.bt_rowptr[r]; k < op.bt_rowptr[r + 1]; ++k) {
+      const int q = op.bt_kcol[k];
+      if (pin && q == 0) continue;
+      for (int b = 0; b < 3; ++b) a[cpre(q, NA, pin) * n + cvel(b, r, NA)] = op.bt[b * op.bt_nnz + k];
+    }
+  } else if (r >= N && r < N + NP) {
     const int i = r - N;
+    if (pin && i == 0) return;
     for (int k = op.b_rowptr[i]; k < op.b_rowptr[i + 1]; ++k) {
-      const long c = op.b_kcol[k];
-      for (int b = 0; b < 3; ++b) a[(c + b * long(N)) * n + i + G] = op.b[b * op.b_nnz + k];
+      const int c = op.b_kcol[k];
+      if (c >= NA) continue;
+      for (int b = 0; b < 3; ++b) a[cvel(b, c, NA) * n + cpre(i, NA, pin)] = op.b[b * op.b_nnz + k];
     }
   }
-}
-// :5824-5830: row of pressure dof 0 := e_{3N}
-__global__ void k_coarse_pin(double *a, long n, long pin) {
-  const long c = blockIdx.x * long(blockDim.x) + threadIdx.x;
-  if (c < n) a[c * n + pin] = (c == pin) ? 1.0 : 0.0;
 }
 
 __global__ void k_fill_d(double *x, double v, long n) {
@@ -68,31 +70,40 @@ __global__ void k_fill_d(double *x, double v, long n) {
 // panel [j0, j1): dgetf2 restricted to the panel columns (single workgroup)
 __global__ __launch_bounds__(1024) void k_lu_panel(double *__restrict__ a, int n, int j0, int j1,
                                                    int *__restrict__ ipiv, int *__restrict__ info) {
-  __shared__ double sv[1024];
-  __shared__ int si[1024];
-  const int t = threadIdx.x;
+  __shared__ double sv[16];
+  __shared__ int si[16];
+  __shared__ int piv_s, nz_s;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for (int j = j0; j < j1; ++j) {
     double *colj = a + long(j) * n;
+    // first index of max |a(i,j)|, i >= j: per thread in increasing row order, then
+    // (value, -index) maxima across lanes and waves
     double bv = -1.0;
     int bi = n;
     for (int i = j + t; i < n; i += 1024) {
       const double v = fabs(colj[i]);
-      if (v > bv) { bv = v; bi = i; }   // rows visited in increasing order per thread
+      if (v > bv) { bv = v; bi = i; }
     }
-    sv[t] = bv;
-    si[t] = bi;
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { sv[wave] = bv; si[wave] = bi; }
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-      if (t < o) {
-        const double ov = sv[t + o];
-        const int oi = si[t + o];
-        if (ov > sv[t] || (ov == sv[t] && oi < si[t])) { sv[t] = ov; si[t] = oi; }
+    if (wave == 0) {
+      bv = lane < 16 ? sv[lane] : -1.0;
+      bi = lane < 16 ? si[lane] : n;
+      for (int o = 8; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
       }
-      __syncthreads();
+      if (lane == 0) { piv_s = bi; nz_s = bv != 0.0; }
     }
-    const int p = si[0];
-    const bool nz = colj[p] != 0.0;
     __syncthreads();
+    const int p = piv_s;
+    const bool nz = nz_s != 0;
     if (t == 0) ipiv[j] = p;
     if (nz) {
       if (p != j && t < j1 - j0) {
@@ -116,6 +127,109 @@ __global__ __launch_bounds__(1024) void k_lu_panel(double *__restrict__ a, int n
     __syncthreads();
   }
 }
+// Same dgetf2 steps with the panel held in registers: thread t owns panel rows
+// j0 + t + 1024 q (q < PR), so the column loop touches global memory only to load and
+// store the panel.  The pivot row travels through LDS (it is also the U row of the step).
+template <int PR>
+__global__ __launch_bounds__(PNT) void k_lu_panel_reg(double *__restrict__ a, int n, int j0, int j1,
+                                                       int *__restrict__ ipiv, int *__restrict__ info) {
+  __shared__ double bufP[NBR], bufJ[NBR];
+  __shared__ double sv[16];
+  __shared__ int si[16];
+  __shared__ int piv_s, nz_s;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int w = j1 - j0;
+  double v[PR][NBR];
+#pragma unroll
+  for (int q = 0; q < PR; ++q) {
+    const int r = j0 + t + PNT * q;
+#pragma unroll
+    for (int cc = 0; cc < NBR; ++cc) v[q][cc] = (r < n && cc < w) ? a[long(j0 + cc) * n + r] : 0.0;
+  }
+#pragma unroll
+  for (int jj = 0; jj < NBR; ++jj) {
+    const int j = j0 + jj;
+    if (jj < w) {
+    double bv = -1.0;
+    int bi = n;
+#pragma unroll
+    for (int q = 0; q < PR; ++q) {
+      const int r = j0 + t + PNT * q;
+      if (r >= j && r < n) {
+        const double x = fabs(v[q][jj]);
+        if (x > bv) { bv = x; bi = r; }
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { sv[wave] = bv; si[wave] = bi; }
+    __syncthreads();
+    if (wave == 0) {
+      bv = lane < PNT / 64 ? sv[lane] : -1.0;
+      bi = lane < PNT / 64 ? si[lane] : n;
+      for (int o = 4; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) { piv_s = bi; nz_s = bv != 0.0; }
+    }
+    __syncthreads();
+    const int p = piv_s;
+    if (t == 0) ipiv[j] = p;
+    if (nz_s) {
+      const int tp = (p - j0) % PNT, qp = (p - j0) / PNT;
+      // pivot row -> bufP, row j -> bufJ (row j is owned by thread jj, q = 0)
+#pragma unroll
+      for (int q = 0; q < PR; ++q)
+        if (t == tp && q == qp)
+#pragma unroll
+          for (int cc = 0; cc < NBR; ++cc) bufP[cc] = v[q][cc];
+      if (t == jj)
+#pragma unroll
+        for (int cc = 0; cc < NBR; ++cc) bufJ[cc] = v[0][cc];
+      __syncthreads();
+      if (p != j) {
+        if (t == jj)
+#pragma unroll
+          for (int cc = 0; cc < NBR; ++cc) v[0][cc] = bufP[cc];
+#pragma unroll
+        for (int q = 0; q < PR; ++q)
+          if (t == tp && q == qp)
+#pragma unroll
+            for (int cc = 0; cc < NBR; ++cc) v[q][cc] = bufJ[cc];
+      }
+      const double rinv = 1.0 / bufP[jj];
+#pragma unroll
+      for (int q = 0; q < PR; ++q) {
+        const int r = j0 + t + PNT * q;
+        if (r > j && r < n) {
+          const double l = v[q][jj] * rinv;
+          v[q][jj] = l;
+#pragma unroll
+          for (int cc = 0; cc < NBR; ++cc)
+            if (cc > jj) v[q][cc] = fma(-l, bufP[cc], v[q][cc]);
+        }
+      }
+    } else if (t == 0 && info[0] == 0) {
+      info[0] = j + 1;
+    }
+    __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PR; ++q) {
+    const int r = j0 + t + PNT * q;
+    if (r < n)
+#pragma unroll
+      for (int cc = 0; cc < NBR; ++cc)
+        if (cc < w) a[long(j0 + cc) * n + r] = v[q][cc];
+  }
+}
+
 // apply the panel's row interchanges to the columns outside [j0, j1)
 __global__ void k_lu_swap(double *__restrict__ a, int n, int j0, int j1, const int *__restrict__ ipiv) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -216,14 +330,52 @@ __global__ void k_sigma(const int *ipiv, int *sigma, int n) {
 }
 
 // ------------------------------------------------------------------ per-visit solve
-__global__ void k_coarse_rhs(const double *__restrict__ f, const int *__restrict__ sigma, double *__restrict__ b,
-                             int n, int pin) {
+// reduced right-hand side, already permuted: b[i] = (f_I - K_ID x_D)[sigma[i]] with
+// x_D = f_D / diag(A_cc) on the Dirichlet velocity dofs
+__device__ inline double dirichlet_value(const OpView<double> &op, const double *f, int b, int c) {
+  const int k = op.a_rowptr[c];   // Dirichlet rows hold only their diagonal
+  return f[long(b) * op.n_udof + c] / op.a[(4 * b) * op.a_nnz + k];
+}
+__global__ void k_coarse_rhs(OpView<double> op, const double *__restrict__ f, const int *__restrict__ sigma,
+                             double *__restrict__ bout, int n, int pin) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    const int s = sigma[i];
-    b[i] = (s == pin) ? 0.0 : f[s];
+  if (i >= n) return;
+  const int k = sigma[i];
+  const int N = op.n_udof, NA = op.n_active;
+  double v;
+  if (k < 3 * NA) {
+    const int br = k / NA, r = k - br * NA;
+    v = f[long(br) * N + r];
+    for (int q = op.a_rowptr[r]; q < op.a_rowptr[r + 1]; ++q) {
+      const int c = op.a_kcol[q];
+      if (c < NA) continue;
+      for (int bc = 0; bc < 3; ++bc) v -= op.a[(3 * br + bc) * op.a_nnz + q] * dirichlet_value(op, f, bc, c);
+    }
+  } else {
+    const int pi = k - 3 * NA + pin;
+    v = f[3L * N + pi];
+    for (int q = op.b_rowptr[pi]; q < op.b_rowptr[pi + 1]; ++q) {
+      const int c = op.b_kcol[q];
+      if (c < NA) continue;
+      for (int b = 0; b < 3; ++b) v -= op.b[b * op.b_nnz + q] * dirichlet_value(op, f, b, c);
+    }
+  }
+  bout[i] = v;
+}
+// scatter the reduced solution back: kept dofs, Dirichlet dofs f_D/diag, pinned pressure 0
+__global__ void k_coarse_out(OpView<double> op, const double *__restrict__ f, const double *__restrict__ xr,
+                             double *__restrict__ u, int pin) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
+  if (i < 3L * N) {
+    const int b = int(i / N), r = int(i - long(b) * N);
+    u[i] = r < NA ? xr[cvel(b, r, NA)] : dirichlet_value(op, f, b, r);
+  } else if (i < 3L * N + NP) {
+    const int q = int(i - 3L * N);
+    u[i] = (pin && q == 0) ? 0.0 : xr[cpre(q, NA, pin)];
   }
 }
+
 // forward sweep, block [K0, K0+TB): every workgroup solves the diagonal block in wave 0
 // (rows of the block in b are final), workgroup 0 stores it in y, all workgroups update
 // the rows >= K0+TB of b
@@ -233,10 +385,18 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ a, 
   const int t = threadIdx.x;
   const int kb = min(TB, n - K0);
   if (t < 64) {
+    // the lane's row of the diagonal block, loaded before the dependent chain starts
+    const int tr = K0 + min(t, kb - 1);
+    double row[TB];
+#pragma unroll
+    for (int k = 0; k < TB; ++k) row[k] = a[long(K0 + min(k, kb - 1)) * n + tr];
     double x = t < kb ? b[K0 + t] : 0.0;
-    for (int k = 0; k < kb; ++k) {
-      const double xk = bcast(x, k);
-      if (t > k && t < kb) x = fma(-xk, a[long(K0 + k) * n + K0 + t], x);
+#pragma unroll
+    for (int k = 0; k < TB; ++k) {
+      if (k < kb) {
+        const double xk = bcast(x, k);
+        if (t > k && t < kb) x = fma(-xk, row[k], x);
+      }
     }
     xs[t] = x;
     if (blockIdx.x == 0 && t < kb) y[K0 + t] = x;
@@ -245,6 +405,7 @@ __global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ a, 
   const int i = K0 + kb + blockIdx.x * 256 + t;
   if (i >= n) return;
   double v = b[i];
+#pragma unroll 16
   for (int k = 0; k < kb; ++k) v = fma(-xs[k], a[long(K0 + k) * n + i], v);
   b[i] = v;
 }
@@ -256,11 +417,19 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ a, 
   const int t = threadIdx.x;
   const int kb = min(TB, n - K0);
   if (t < 64) {
+    const int tr = K0 + min(t, kb - 1);
+    double row[TB];
+#pragma unroll
+    for (int k = 0; k < TB; ++k) row[k] = a[long(K0 + min(k, kb - 1)) * n + tr];
     double x = t < kb ? y[K0 + t] : 0.0;
-    for (int k = kb - 1; k >= 0; --k) {
-      const double xk = bcast(x, k) / a[long(K0 + k) * n + K0 + k];
-      if (t == k) x = xk;
-      if (t < k) x = fma(-xk, a[long(K0 + k) * n + K0 + t], x);
+#pragma unroll
+    for (int k = TB - 1; k >= 0; --k) {
+      if (k < kb) {
+        // U(k,k) sits in lane k's row[k]
+        const double xk = bcast(x, k) / bcast(row[k], k);
+        if (t == k) x = xk;
+        if (t < k) x = fma(-xk, row[k], x);
+      }
     }
     xs[t] = x;
     if (blockIdx.x == 0 && t < kb) xo[K0 + t] = x;
@@ -269,6 +438,7 @@ __global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ a, 
   const int i = blockIdx.x * 256 + t;
   if (i >= K0) return;
   double v = y[i];
+#pragma unroll 16
   for (int k = kb - 1; k >= 0; --k) v = fma(-xs[k], a[long(K0 + k) * n + i], v);
   y[i] = v;
 }
@@ -291,25 +461,30 @@ __global__ void k_csr(int n_rows, const int *__restrict__ rp, const int *__restr
 
 }  // namespace
 
-void launch_coarse_build(const DevLevel &L, double *a, bool pin, hipStream_t s) {
-  const long n = L.n_dof;
-  k_fill_d<<<4096, 256, 0, s>>>(a, 0.0, n * n);
+void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s) {
+  const long n = C.n;
+  k_fill_d<<<2048, 256, 0, s>>>(C.a, 0.0, n * n);
   PMG_HIP_CHECK(hipGetLastError());
   const int rows = L.n_udof + L.n_pdof;
-  k_coarse_scatter<<<(rows + 127) / 128, 128, 0, s>>>(L.op64(), a, n);
+  k_coarse_scatter<<<(rows + 127) / 128, 128, 0, s>>>(L.op64(), C.a, n, C.pin);
   PMG_HIP_CHECK(hipGetLastError());
-  if (pin) {
-    k_coarse_pin<<<int((n + 255) / 256), 256, 0, s>>>(a, n, 3L * L.n_udof);
-    PMG_HIP_CHECK(hipGetLastError());
-  }
 }
 
-int coarse_lu_factor(Coarse &C, hipStream_t s) {
+void coarse_lu_factor(Coarse &C, hipStream_t s) {
   const int n = C.n;
   PMG_HIP_CHECK(hipMemsetAsync(C.info, 0, sizeof(int), s));
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    const int j1 = std::min(n, j0 + NB);
-    k_lu_panel<<<1, 1024, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info);
+  for (int j0 = 0; j0 < n; j0 += NBR) {
+    const int j1 = std::min(n, j0 + NBR);
+    const int pr = (n - j0 + PNT - 1) / PNT;
+    switch (pr) {
+      case 1: k_lu_panel_reg<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      case 2: k_lu_panel_reg<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      case 3: k_lu_panel_reg<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      case 4: k_lu_panel_reg<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      case 5: k_lu_panel_reg<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      case 6: k_lu_panel_reg<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      default: k_lu_panel<<<1, 1024, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+    }
     PMG_HIP_CHECK(hipGetLastError());
     const int nout = n - (j1 - j0);
     if (nout > 0) {
@@ -326,15 +501,11 @@ int coarse_lu_factor(Coarse &C, hipStream_t s) {
   }
   k_sigma<<<1, 64, size_t(n) * sizeof(int), s>>>(C.ipiv, C.sigma, n);
   PMG_HIP_CHECK(hipGetLastError());
-  int info = 0;
-  PMG_HIP_CHECK(hipMemcpyAsync(&info, C.info, sizeof(int), hipMemcpyDeviceToHost, s));
-  PMG_HIP_CHECK(hipStreamSynchronize(s));
-  return info;
 }
 
-void coarse_lu_solve(Coarse &C, const double *f, double *u, int pin_index, hipStream_t s) {
+void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s) {
   const int n = C.n;
-  k_coarse_rhs<<<(n + 255) / 256, 256, 0, s>>>(f, C.sigma, C.b, n, pin_index);
+  k_coarse_rhs<<<(n + 255) / 256, 256, 0, s>>>(op, f, C.sigma, C.b, n, C.pin);
   PMG_HIP_CHECK(hipGetLastError());
   for (int K0 = 0; K0 < n; K0 += TB) {
     const int below = n - std::min(n, K0 + TB);
@@ -345,9 +516,12 @@ void coarse_lu_solve(Coarse &C, const double *f, double *u, int pin_index, hipSt
   const int last = ((n - 1) / TB) * TB;
   for (int K0 = last; K0 >= 0; K0 -= TB) {
     const int g = std::max(1, (K0 + 255) / 256);
-    k_trsv_bwd<<<g, 256, 0, s>>>(C.a, C.y, u, n, K0);
+    k_trsv_bwd<<<g, 256, 0, s>>>(C.a, C.y, C.x, n, K0);
     PMG_HIP_CHECK(hipGetLastError());
   }
+  const long nf = 3L * op.n_udof + op.n_pdof;
+  k_coarse_out<<<int((nf + 255) / 256), 256, 0, s>>>(op, f, C.x, u, C.pin);
+  PMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>

@@ -24,21 +24,24 @@ __device__ inline T red16(T v) {
   return v;
 }
 
-// block-wide sum of two doubles (blockDim.x == 256); result valid in thread 0
+// block-wide sum of two doubles (blockDim.x == 256), fixed order: wave shuffles, then the
+// four wave sums in wave order; the result is valid in every thread
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
 __device__ inline void block_sum2(double &a, double &b, double *sh) {
-  const int t = threadIdx.x;
-  sh[t] = a;
-  sh[256 + t] = b;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (t < o) {
-      sh[t] += sh[t + o];
-      sh[256 + t] += sh[256 + t + o];
-    }
-    __syncthreads();
+  if (l == 0) {
+    sh[w] = a;
+    sh[4 + w] = b;
   }
-  a = sh[0];
-  b = sh[256];
+  __syncthreads();
+  a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
 }
 
 // Last block of the grid sums partials[0..nblk) (two values each) in a fixed order.
@@ -74,48 +77,50 @@ template <int MODE, typename T>
 __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict__ x,
                                                const T *__restrict__ bv, T *__restrict__ y,
                                                int project, int want_norm, double *partials,
-                                               unsigned *counter, double *res) {
-  __shared__ double sh[512];
-  const int lane = threadIdx.x & 15;
-  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4;
-  const int ngrp = gridDim.x * 16;
+                                               unsigned *counter, double *res, double *res_slots) {
+  __shared__ double sh[8];
+  const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
-  const long nnzA = op.a_nnz, nnzB = op.b_nnz, nnzBT = op.bt_nnz;
-  const T *__restrict__ A = op.a;
-  const T *__restrict__ B = op.b;
-  const T *__restrict__ BT = op.bt;
+  const AEnt<T> *__restrict__ AE = op.ae;
+  const BEnt<T> *__restrict__ BE = op.be;
+  const BEnt<T> *__restrict__ BTE = op.bte;
   const T *__restrict__ x1 = x;
   const T *__restrict__ x2 = x + N;
   const T *__restrict__ x3 = x + 2 * N;
   const T *__restrict__ xp = x + 3 * N;
   double pc = 0.0, sq = 0.0;
+  // this block's contiguous row range (XCD-aware logical block id)
   const int nrows = N + NP;
-  for (int r = grp; r < nrows; r += ngrp) {
+  const int per = (nrows + gridDim.x - 1) / gridDim.x;
+  const int lb = xcd_block();
+  const int r0 = lb * per, r1 = min(nrows, r0 + per);
+  for (int r = r0 + grp; r < r1; r += 16) {
     if (r < N) {
       T s1 = 0, s2 = 0, s3 = 0;
       const int k0 = op.a_rowptr[r], k1 = op.a_rowptr[r + 1];
       if (r < NA) {
         for (int k = k0 + lane; k < k1; k += 16) {
-          const int c = op.a_kcol[k];
-          const T u1 = x1[c], u2 = x2[c], u3 = x3[c];
-          s1 += A[k] * u1 + A[nnzA + k] * u2 + A[2 * nnzA + k] * u3;
-          s2 += A[3 * nnzA + k] * u1 + A[4 * nnzA + k] * u2 + A[5 * nnzA + k] * u3;
-          s3 += A[6 * nnzA + k] * u1 + A[7 * nnzA + k] * u2 + A[8 * nnzA + k] * u3;
+          const AEnt<T> e = AE[k];
+          const T u1 = x1[e.col], u2 = x2[e.col], u3 = x3[e.col];
+          s1 += e.v[0] * u1 + e.v[1] * u2 + e.v[2] * u3;
+          s2 += e.v[3] * u1 + e.v[4] * u2 + e.v[5] * u3;
+          s3 += e.v[6] * u1 + e.v[7] * u2 + e.v[8] * u3;
         }
         const int q0 = op.bt_rowptr[r], q1 = op.bt_rowptr[r + 1];
         for (int k = q0 + lane; k < q1; k += 16) {
-          const T pv = xp[op.bt_kcol[k]];
-          s1 += BT[k] * pv;
-          s2 += BT[nnzBT + k] * pv;
-          s3 += BT[2 * nnzBT + k] * pv;
+          const BEnt<T> e = BTE[k];
+          const T pv = xp[e.col];
+          s1 += e.v[0] * pv;
+          s2 += e.v[1] * pv;
+          s3 += e.v[2] * pv;
         }
       } else {
         // rows N_Active..N_UDOF: diagonal blocks only (MGComponents3D.C:3136-3156)
         for (int k = k0 + lane; k < k1; k += 16) {
-          const int c = op.a_kcol[k];
-          s1 += A[k] * x1[c];
-          s2 += A[4 * nnzA + k] * x2[c];
-          s3 += A[8 * nnzA + k] * x3[c];
+          const AEnt<T> e = AE[k];
+          s1 += e.v[0] * x1[e.col];
+          s2 += e.v[4] * x2[e.col];
+          s3 += e.v[8] * x3[e.col];
         }
       }
       s1 = red16(s1);
@@ -137,8 +142,8 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
       T s = 0;
       const int k0 = op.b_rowptr[pr], k1 = op.b_rowptr[pr + 1];
       for (int k = k0 + lane; k < k1; k += 16) {
-        const int c = op.b_kcol[k];
-        s += B[k] * x1[c] + B[nnzB + k] * x2[c] + B[2 * nnzB + k] * x3[c];
+        const BEnt<T> e = BE[k];
+        s += e.v[0] * x1[e.col] + e.v[1] * x2[e.col] + e.v[2] * x3[e.col];
       }
       s = red16(s);
       if (lane == 0) {
@@ -154,22 +159,33 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
   if (!(project || want_norm)) return;
   double tpc, tsq;
   if (!last_block_sum2(pc, sq, partials, counter, tpc, tsq, sh)) return;
-  // IntoL20Vector3D case -11 (MGComponents3D.C:1766-1786): s = sum/Length*4
-  double add = 0.0;
-  if (project) {
-    const double m = tpc / double(NP) * 4.0;
-    T *p = y + 3 * N;
-    for (int i = threadIdx.x; i < NP / 4; i += 256) {
-      const T v = T(p[4 * i] - T(m));
-      p[4 * i] = v;
-      add += double(v) * v;
+  if (threadIdx.x == 0) {
+    if (project) {
+      // IntoL20Vector3D case -11 (MGComponents3D.C:1766-1786): s = sum/Length*4, applied by
+      // k_l20_apply; the squares of the constants are added there
+      res_slots[0] = tpc / double(NP) * 4.0;
+      res_slots[1] = tsq;
+    } else {
+      *res = sqrt(tsq);
     }
   }
-  if (want_norm) {
-    double dummy = 0.0;
-    block_sum2(add, dummy, sh);
-    if (threadIdx.x == 0) *res = sqrt(tsq + add);
+}
+
+// v[4i] -= s for the P1disc constants; with res: ||.||^2 = base + sum of the new squares
+template <typename T>
+__global__ __launch_bounds__(256) void k_l20_apply(T *__restrict__ p, int NP, const double *slots, double *res,
+                                                   double *partials, unsigned *counter) {
+  __shared__ double sh[8];
+  const double m = slots[0];
+  double add = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < NP / 4; i += gridDim.x * 256) {
+    const T v = T(p[4 * i] - T(m));
+    p[4 * i] = v;
+    add += double(v) * v;
   }
+  if (!res) return;
+  double t, u;
+  if (last_block_sum2(add, 0.0, partials, counter, t, u, sh) && threadIdx.x == 0) *res = sqrt(slots[1] + t);
 }
 
 template <typename T>
@@ -207,7 +223,7 @@ __global__ void k_dir(T *__restrict__ u, const T *__restrict__ f, int N, int sta
 // standalone IntoL20 of the P1disc constants (single block)
 template <typename T>
 __global__ __launch_bounds__(256) void k_l20(T *p, int NP) {
-  __shared__ double sh[512];
+  __shared__ double sh[8];
   double s = 0, d = 0;
   for (int i = threadIdx.x; i < NP / 4; i += 256) s += double(p[4 * i]);
   block_sum2(s, d, sh);
@@ -218,7 +234,7 @@ __global__ __launch_bounds__(256) void k_l20(T *p, int NP) {
 // ------------------------------------------------------------------ BLAS-1 for FGMRES
 __global__ __launch_bounds__(256) void k_dot(const double *__restrict__ x, const double *__restrict__ y,
                                              long n, double *out, double *partials, unsigned *counter) {
-  __shared__ double sh[512];
+  __shared__ double sh[8];
   double s = 0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += long(gridDim.x) * 256) s += x[i] * y[i];
   double t, u;
@@ -229,7 +245,7 @@ __global__ __launch_bounds__(256) void k_axpy_dot(double *__restrict__ w, const 
                                                   const double *__restrict__ v,
                                                   const double *__restrict__ vn, long n, double *out,
                                                   double *partials, unsigned *counter) {
-  __shared__ double sh[512];
+  __shared__ double sh[8];
   const double a = -h[0];
   double s = 0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += long(gridDim.x) * 256) {
@@ -272,22 +288,70 @@ template <typename T>
 void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project, double *res,
                   RedScratch &red, hipStream_t s) {
   const long rows = long(op.n_udof) + op.n_pdof;
-  int grid = int((rows + 15) / 16);
-  if (grid > RED_BLOCKS) grid = RED_BLOCKS;
+  const int grid = xcd_grid(rows, 64, RED_BLOCKS);
   const int want_norm = res != nullptr;
-  const int proj = project ? 1 : 0;
+  const int proj = (mode != 0 && project) ? 1 : 0;
+  double *slots = red.result + 8;
   if (mode == 0)
-    k_type4<0, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, want_norm, red.partials, red.counter, res);
+    k_type4<0, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, want_norm, red.partials, red.counter, res, slots);
   else if (mode == 1)
-    k_type4<1, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res);
+    k_type4<1, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res, slots);
   else
-    k_type4<2, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res);
+    k_type4<2, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res, slots);
   PMG_HIP_CHECK(hipGetLastError());
+  if (proj) {
+    const int nc = op.n_pdof / 4;
+    const int g = std::max(1, std::min(512, (nc + 255) / 256));
+    k_l20_apply<T><<<g, 256, 0, s>>>(y + 3L * op.n_udof, op.n_pdof, slots, want_norm ? res : nullptr,
+                                     red.partials, red.counter);
+    PMG_HIP_CHECK(hipGetLastError());
+  }
 }
 template void launch_type4<double>(const OpView<double> &, int, const double *, const double *, double *, bool,
                                    double *, RedScratch &, hipStream_t);
 template void launch_type4<float>(const OpView<float> &, int, const float *, const float *, float *, bool,
                                   double *, RedScratch &, hipStream_t);
+
+namespace {
+template <typename T>
+__global__ void k_records_a(const int *__restrict__ col, const double *__restrict__ a, long nnz,
+                            AEnt<T> *__restrict__ out) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < nnz; k += long(gridDim.x) * blockDim.x) {
+    AEnt<T> e;
+    e.col = col[k];
+    if constexpr (sizeof(T) == 8) e.pad = 0;
+#pragma unroll
+    for (int b = 0; b < 9; ++b) e.v[b] = T(a[b * nnz + k]);
+    out[k] = e;
+  }
+}
+template <typename T>
+__global__ void k_records_b(const int *__restrict__ col, const double *__restrict__ a, long nnz,
+                            BEnt<T> *__restrict__ out) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < nnz; k += long(gridDim.x) * blockDim.x) {
+    BEnt<T> e;
+    e.col = col[k];
+    if constexpr (sizeof(T) == 8) e.pad = 0;
+#pragma unroll
+    for (int b = 0; b < 3; ++b) e.v[b] = T(a[b * nnz + k]);
+    out[k] = e;
+  }
+}
+}  // namespace
+
+template <typename T>
+void launch_build_records(const DevLevel &L, AEnt<T> *ae, BEnt<T> *be, BEnt<T> *bte, hipStream_t s) {
+  k_records_a<T><<<grid_for(L.a_nnz), 256, 0, s>>>(L.a_kcol, L.a, L.a_nnz, ae);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_records_b<T><<<grid_for(L.b_nnz), 256, 0, s>>>(L.b_kcol, L.b, L.b_nnz, be);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_records_b<T><<<grid_for(L.bt_nnz), 256, 0, s>>>(L.bt_kcol, L.bt, L.bt_nnz, bte);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_build_records<double>(const DevLevel &, AEnt<double> *, BEnt<double> *, BEnt<double> *,
+                                           hipStream_t);
+template void launch_build_records<float>(const DevLevel &, AEnt<float> *, BEnt<float> *, BEnt<float> *,
+                                          hipStream_t);
 
 template <typename T>
 void launch_fill(T *x, T v, long n, hipStream_t s) {
@@ -348,12 +412,12 @@ PMG_INST_VEC(float)
 template void launch_fill<int>(int *, int, long, hipStream_t);
 
 void launch_dot(const double *x, const double *y, long n, double *out, RedScratch &red, hipStream_t s) {
-  k_dot<<<grid_for(n, 256, RED_BLOCKS), 256, 0, s>>>(x, y, n, out, red.partials, red.counter);
+  k_dot<<<grid_for(n, 256, 512), 256, 0, s>>>(x, y, n, out, red.partials, red.counter);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_axpy_dev_dot(double *w, const double *h, const double *v, const double *vnext, long n, double *out,
                          RedScratch &red, hipStream_t s) {
-  k_axpy_dot<<<grid_for(n, 256, RED_BLOCKS), 256, 0, s>>>(w, h, v, vnext, n, out, red.partials, red.counter);
+  k_axpy_dot<<<grid_for(n, 256, 512), 256, 0, s>>>(w, h, v, vnext, n, out, red.partials, red.counter);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_scale_dev(double *y, const double *x, const double *den, long n, hipStream_t s) {

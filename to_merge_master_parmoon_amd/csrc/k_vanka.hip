@@ -172,35 +172,48 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *
     }
     __syncthreads();
   }
+  // packed store: strictly lower part column by column at [0, LU_L), upper part (with the
+  // diagonal) column by column at [LU_UOFF, LU_UOFF + LU_U)
   double *out = lu + long(pos) * LU_STRIDE;
-  for (int i = t; i < NLOC * NLOC; i += 256) out[i] = S[i];
+  for (int i = t; i < NLOC * NLOC; i += 256) {
+    const int c = i / NLOC, r = i - c * NLOC;
+    if (r > c)
+      out[lu_loff(c) + r - c - 1] = S[i];
+    else
+      out[LU_UOFF + lu_uoff(c) + r] = S[i];
+  }
   if (t < NLOC) invperm[long(pos) * PERM_STRIDE + sigma[t]] = uint8_t(t);
   if (t == 0) info[pos] = info_s;
 }
 
 // ------------------------------------------------------------------ per-color residual
 // (slot, local row) items, 16 lanes each: rows 0..26 produce the three velocity
-// components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061)
+// components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061).
+// items[] lists slot*32 + row sorted by the global row, and each block takes one
+// contiguous run of it (XCD-aware), so neighbouring rows share cache lines and L2.
 template <typename T>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__restrict__ cells, int ncells,
                                                    int pos0, const int *__restrict__ u_global,
                                                    const int *__restrict__ p_global,
                                                    const uint8_t *__restrict__ invperm,
+                                                   const int *__restrict__ items,
                                                    const T *__restrict__ u, const T *__restrict__ f,
                                                    T *__restrict__ rloc) {
-  const int lane = threadIdx.x & 15;
-  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4;
-  const int ngrp = gridDim.x * 16;
+  const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active;
-  const long nnzA = op.a_nnz, nnzB = op.b_nnz, nnzBT = op.bt_nnz;
-  const T *__restrict__ A = op.a;
+  const AEnt<T> *__restrict__ AE = op.ae;
+  const BEnt<T> *__restrict__ BE = op.be;
+  const BEnt<T> *__restrict__ BTE = op.bte;
   const T *__restrict__ u1 = u;
   const T *__restrict__ u2 = u + N;
   const T *__restrict__ u3 = u + 2 * N;
   const T *__restrict__ up = u + 3 * N;
-  const int items = ncells * (NU_LOC + NP_LOC);
-  for (int it = grp; it < items; it += ngrp) {
-    const int slot = it / (NU_LOC + NP_LOC), r = it - slot * (NU_LOC + NP_LOC);
+  const int nitems = ncells * (NU_LOC + NP_LOC);
+  const int per = (nitems + gridDim.x - 1) / gridDim.x;
+  const int i0 = xcd_block() * per, i1 = min(nitems, i0 + per);
+  for (int i = i0 + grp; i < i1; i += 16) {
+    const int it = items[i];
+    const int slot = it >> 5, r = it & 31;
     const int pos = pos0 + slot;
     const int cell = cells[pos];
     const uint8_t *ip = invperm + long(pos) * PERM_STRIDE;
@@ -211,24 +224,26 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
       const int k0 = op.a_rowptr[UDOF], k1 = op.a_rowptr[UDOF + 1];
       if (UDOF < NA) {
         for (int k = k0 + lane; k < k1; k += 16) {
-          const int c = op.a_kcol[k];
-          const T x1 = u1[c], x2 = u2[c], x3 = u3[c];
-          s1 += A[k] * x1 + A[nnzA + k] * x2 + A[2 * nnzA + k] * x3;
-          s2 += A[3 * nnzA + k] * x1 + A[4 * nnzA + k] * x2 + A[5 * nnzA + k] * x3;
-          s3 += A[6 * nnzA + k] * x1 + A[7 * nnzA + k] * x2 + A[8 * nnzA + k] * x3;
+          const AEnt<T> e = AE[k];
+          const T x1 = u1[e.col], x2 = u2[e.col], x3 = u3[e.col];
+          s1 += e.v[0] * x1 + e.v[1] * x2 + e.v[2] * x3;
+          s2 += e.v[3] * x1 + e.v[4] * x2 + e.v[5] * x3;
+          s3 += e.v[6] * x1 + e.v[7] * x2 + e.v[8] * x3;
         }
-        for (int k = op.bt_rowptr[UDOF] + lane; k < op.bt_rowptr[UDOF + 1]; k += 16) {
-          const T pv = up[op.bt_kcol[k]];
-          s1 += op.bt[k] * pv;
-          s2 += op.bt[nnzBT + k] * pv;
-          s3 += op.bt[2 * nnzBT + k] * pv;
+        const int q0 = op.bt_rowptr[UDOF], q1 = op.bt_rowptr[UDOF + 1];
+        for (int k = q0 + lane; k < q1; k += 16) {
+          const BEnt<T> e = BTE[k];
+          const T pv = up[e.col];
+          s1 += e.v[0] * pv;
+          s2 += e.v[1] * pv;
+          s3 += e.v[2] * pv;
         }
       } else {
         for (int k = k0 + lane; k < k1; k += 16) {
-          const int c = op.a_kcol[k];
-          s1 += A[k] * u1[c];
-          s2 += A[4 * nnzA + k] * u2[c];
-          s3 += A[8 * nnzA + k] * u3[c];
+          const AEnt<T> e = AE[k];
+          s1 += e.v[0] * u1[e.col];
+          s2 += e.v[4] * u2[e.col];
+          s3 += e.v[8] * u3[e.col];
         }
       }
       s1 = red16(s1);
@@ -244,8 +259,8 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
       const int PDOF = p_global[long(cell) * NP_LOC + m];
       T s = 0;
       for (int k = op.b_rowptr[PDOF] + lane; k < op.b_rowptr[PDOF + 1]; k += 16) {
-        const int c = op.b_kcol[k];
-        s += op.b[k] * u1[c] + op.b[nnzB + k] * u2[c] + op.b[2 * nnzB + k] * u3[c];
+        const BEnt<T> e = BE[k];
+        s += e.v[0] * u1[e.col] + e.v[1] * u2[e.col] + e.v[2] * u3[e.col];
       }
       s = red16(s);
       if (lane == 0) out[ip[3 * NU_LOC + m]] = f[3 * N + PDOF] - s;
@@ -254,42 +269,101 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
 }
 
 // ------------------------------------------------------------------ per-color local solves
-// one wavefront per patch; lane i owns local rows i and i+64 (i < 21)
+// One wavefront per patch; lane i owns local rows i and i+64 (i < 21).  The sweeps are a
+// chain of 170 dependent steps, so nothing in the chain may wait on HBM: the wave first
+// issues the loads of the whole factor (L part and U part, 16-byte vectors), stores the L
+// part into LDS and runs the unit-lower sweep from LDS while the U part is still arriving
+// in registers, then stores the U part over it and runs the upper sweep.
+template <typename F>
+struct PackedSizes {
+  static constexpr int NVL = int((LU_L * sizeof(F) + 15) / 16);   // 16-byte vectors of L
+  static constexpr int NVU = int((LU_U * sizeof(F) + 15) / 16);   // 16-byte vectors of U
+  static constexpr int RL = (NVL + 63) / 64, RU = (NVU + 63) / 64;
+  // every lane loads RL (RU) vectors unconditionally; the overshoot past the part stays
+  // inside the patch stride or the LU_PAD tail of the allocation
+  static constexpr int LDS_BYTES = 16 * 64 * (RU > RL ? RU : RL);
+  static_assert(LU_UOFF * sizeof(F) + 16 * 64 * RU <= (LU_STRIDE + LU_PAD) * sizeof(F), "pad too small");
+};
+
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));   // register-promotable 16 B
+
 template <typename T, typename F>
-__global__ __launch_bounds__(256) void k_vanka_solve(const F *__restrict__ lu, const int *__restrict__ cells,
-                                                     int ncells, int pos0, const T *__restrict__ rloc,
-                                                     const int *__restrict__ u_global,
-                                                     const int *__restrict__ p_global, int N,
-                                                     T *__restrict__ u, T damp) {
-  const int lane = threadIdx.x & 63;
-  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, const int *__restrict__ cells,
+                                                    int ncells, int pos0, const T *__restrict__ rloc,
+                                                    const int *__restrict__ u_global,
+                                                    const int *__restrict__ p_global, int N,
+                                                    T *__restrict__ u, T damp) {
+  using PS = PackedSizes<F>;
+  extern __shared__ v4u s4[];
+  const F *sl = reinterpret_cast<const F *>(s4);
+  const int lane = threadIdx.x;
+  const int slot = blockIdx.x;
   if (slot >= ncells) return;
   const int pos = pos0 + slot;
-  const F *__restrict__ M = lu + long(pos) * LU_STRIDE;
+  const v4u *__restrict__ gL = reinterpret_cast<const v4u *>(lu + long(pos) * LU_STRIDE);
+  const v4u *__restrict__ gU = reinterpret_cast<const v4u *>(lu + long(pos) * LU_STRIDE + LU_UOFF);
+  {
+    v4u rL[PS::RL];
+#pragma unroll
+    for (int q = 0; q < PS::RL; ++q) rL[q] = gL[lane + 64 * q];
+#pragma unroll
+    for (int q = 0; q < PS::RL; ++q) s4[lane + 64 * q] = rL[q];
+  }
+  v4u rU[PS::RU];
+#pragma unroll
+  for (int q = 0; q < PS::RU; ++q) rU[q] = gU[lane + 64 * q];
   const T *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
   const bool two = lane < NLOC - 64;
   T x0 = rs[lane];
   T x1 = two ? rs[64 + lane] : T(0);
-  // unit lower sweep (dtrsm L, column oriented)
-#pragma unroll 4
+  __syncthreads();
+  // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1
+  // branch-free: every lane reads an in-range LDS entry and keeps its old value where the
+  // row is not below the pivot, so the reads carry no control dependence and issue early
+#pragma unroll
   for (int k = 0; k < NLOC - 1; ++k) {
     const T bk = bcast(k < 64 ? x0 : x1, k & 63);
-    const F *col = M + k * NLOC;
-    if (lane > k) x0 = tfma(-bk, Ld<T, F>::get(col, lane), x0);
-    if (two && lane + 64 > k) x1 = tfma(-bk, Ld<T, F>::get(col, 64 + lane), x1);
+    const int o = lu_loff(k) - k - 1;
+    const bool a0 = lane > k, a1 = two && lane + 64 > k;
+    const T l0 = Ld<T, F>::get(sl, a0 ? o + lane : 0);
+    const T l1 = Ld<T, F>::get(sl, a1 ? o + 64 + lane : 0);
+    const T y0 = tfma(-bk, l0, x0), y1 = tfma(-bk, l1, x1);
+    x0 = a0 ? y0 : x0;
+    x1 = a1 ? y1 : x1;
   }
-  // upper sweep (dtrsm U): x_k /= U_kk, then eliminate column k above the diagonal
-#pragma unroll 4
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PS::RU; ++q) s4[lane + 64 * q] = rU[q];
+  __syncthreads();
+  // upper sweep (dtrsm U): x_k /= U(k,k), then eliminate column k above the diagonal;
+  // U(r,k) at lu_uoff(k) + r
+  // Diagonal of U and its reciprocal per lane, computed in parallel before the chain.  The
+  // quotient x_k / U(k,k) is then q = x r, e = fma(-q, U, x), q' = fma(e, r, q): with r the
+  // correctly rounded reciprocal this is the correctly rounded quotient (Markstein), i.e.
+  // the same value as the division of the column-oriented dtrsm, in three dependent steps.
+  const T dg0 = Ld<T, F>::get(sl, lu_uoff(lane) + lane);
+  const T dg1 = two ? Ld<T, F>::get(sl, lu_uoff(64 + lane) + 64 + lane) : T(1);
+  const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
+#pragma unroll
   for (int k = NLOC - 1; k >= 0; --k) {
-    const F *col = M + k * NLOC;
-    const T xk = bcast(k < 64 ? x0 : x1, k & 63) / Ld<T, F>::get(col, k);
-    if (k < 64) {
-      if (lane == k) x0 = xk;
-    } else if (lane == k - 64) {
-      x1 = xk;
+    const int o = lu_uoff(k);
+    T q0 = x0 * rc0, q1 = x1 * rc1;
+    if constexpr (sizeof(T) == 8) {
+      q0 = tfma(tfma(-q0, dg0, x0), rc0, q0);
+      q1 = tfma(tfma(-q1, dg1, x1), rc1, q1);
     }
-    if (lane < k) x0 = tfma(-xk, Ld<T, F>::get(col, lane), x0);
-    if (two && lane + 64 < k) x1 = tfma(-xk, Ld<T, F>::get(col, 64 + lane), x1);
+    const T xk = bcast(k < 64 ? q0 : q1, k & 63);
+    const bool a0 = lane < k, a1 = two && lane + 64 < k;
+    const T u0 = Ld<T, F>::get(sl, a0 ? o + lane : 0);
+    const T u1 = Ld<T, F>::get(sl, a1 ? o + 64 + lane : 0);
+    const T y0 = tfma(-xk, u0, x0), y1 = tfma(-xk, u1, x1);
+    if (k < 64) {
+      x0 = (lane == k) ? xk : (a0 ? y0 : x0);
+      x1 = a1 ? y1 : x1;
+    } else {
+      x0 = a0 ? y0 : x0;
+      x1 = (lane == k - 64) ? xk : (a1 ? y1 : x1);
+    }
   }
   // u[UDOF] += damp * delta, p[PDOF] += damp * delta
   const int cell = cells[pos];
@@ -337,28 +411,26 @@ void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s) {
 
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0, const int *u_global,
-                           const int *p_global, const uint8_t *invperm, const T *u, const T *f, T *rloc,
-                           hipStream_t s) {
+                           const int *p_global, const uint8_t *invperm, const int *items, const T *u, const T *f,
+                           T *rloc, hipStream_t s) {
   if (ncells <= 0) return;
-  const long groups = long(ncells) * (NU_LOC + NP_LOC);
-  long grid = (groups + 15) / 16;
-  if (grid > 8192) grid = 8192;
-  k_vanka_res<T><<<int(grid), 256, 0, s>>>(op, cells, ncells, pos0, u_global, p_global, invperm, u, f, rloc);
+  const int grid = xcd_grid(long(ncells) * (NU_LOC + NP_LOC), 32, 8192);
+  k_vanka_res<T><<<grid, 256, 0, s>>>(op, cells, ncells, pos0, u_global, p_global, invperm, items, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, const int *, int, int, const int *,
-                                                    const int *, const uint8_t *, const double *,
-                                                    const double *, double *, hipStream_t);
-template void launch_vanka_residual<float>(const OpView<float> &, const int *, int, int, const int *,
-                                                  const int *, const uint8_t *, const float *, const float *,
-                                                  float *, hipStream_t);
+                                            const int *, const uint8_t *, const int *, const double *,
+                                            const double *, double *, hipStream_t);
+template void launch_vanka_residual<float>(const OpView<float> &, const int *, int, int, const int *, const int *,
+                                           const uint8_t *, const int *, const float *, const float *, float *,
+                                           hipStream_t);
 
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc, const int *u_global,
                         const int *p_global, int n_udof, T *u, T damp, hipStream_t s) {
   if (ncells <= 0) return;
-  k_vanka_solve<T, F><<<(ncells + 3) / 4, 256, 0, s>>>(lu, cells, ncells, pos0, rloc, u_global, p_global,
-                                                       n_udof, u, damp);
+  k_vanka_solve<T, F><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(lu, cells, ncells, pos0, rloc, u_global,
+                                                                    p_global, n_udof, u, damp);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_solve<double, double>(const double *, const int *, int, int, const double *,

@@ -24,6 +24,8 @@ struct pmg_ctx {
   int device = 0;
   pmg_params prm{};
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;   // coarse LU, overlapping the batched patch LUs
+  hipEvent_t evs = nullptr;
   std::vector<DevLevel> L;
   // host copies of the dof maps (needed to build the transfers to the next level)
   std::vector<std::vector<int>> h_u_global, h_p_global;
@@ -78,11 +80,13 @@ struct Fail {
 int guard(pmg_ctx *c, const std::function<void()> &f);
 
 // ------------------------------------------------------------------ event timing
+// HIP events around one launch (or one group of launches) on the context's stream; only
+// active while profiling is on, because it waits for the launch to finish.
 struct Timed {
   pmg_ctx *c;
-  double *acc;
-  int *cnt;
-  Timed(pmg_ctx *c_, double *a, int *n) : c(c_), acc(a), cnt(n) {
+  pmg_kstat *k;
+  double bytes, flops;
+  Timed(pmg_ctx *c_, pmg_kstat *k_, double b = 0, double f = 0) : c(c_), k(k_), bytes(b), flops(f) {
     if (c->profiling) PMG_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   }
   ~Timed() noexcept(false) {
@@ -91,8 +95,27 @@ struct Timed {
     PMG_HIP_CHECK(hipEventSynchronize(c->ev1));
     float ms = 0;
     PMG_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    *acc += ms;
-    if (cnt) ++*cnt;
+    k->ms += ms;
+    k->calls += 1;
+    k->bytes += bytes;
+    k->flops += flops;
+  }
+};
+
+// Algorithmic bytes of the type-4 operator rows (values + column indices + row pointers,
+// right-hand side read, output written; gathers of x counted as cache hits), for values of
+// vs bytes.  Dirichlet rows read only the diagonal blocks, as the kernels do.
+struct RowBytes {
+  double vel(const pmg_level_desc *d, int i, int vs) const {
+    const double nA = d->a_rowptr[i + 1] - d->a_rowptr[i];
+    if (i < d->n_active) {
+      const double nBT = d->bt_rowptr[i + 1] - d->bt_rowptr[i];
+      return nA * (9.0 * vs + 4) + nBT * (3.0 * vs + 4) + 16;
+    }
+    return nA * (3.0 * vs + 4) + 8;
+  }
+  double pre(const pmg_level_desc *d, int i, int vs) const {
+    return double(d->b_rowptr[i + 1] - d->b_rowptr[i]) * (3.0 * vs + 4) + 8;
   }
 };
 
@@ -171,11 +194,8 @@ void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
     h2d(c, D.b + size_t(b) * D.b_nnz, d->b[b], D.b_nnz);
     h2d(c, D.bt + size_t(b) * D.bt_nnz, d->bt[b], D.bt_nnz);
   }
-  if (D.a32) {
-    launch_d2f(D.a32, D.a, 9 * D.a_nnz, c->stream);
-    launch_d2f(D.b32, D.b, 3 * D.b_nnz, c->stream);
-    launch_d2f(D.bt32, D.bt, 3 * D.bt_nnz, c->stream);
-  }
+  launch_build_records<double>(D, D.ae64, D.be64, D.bte64, c->stream);
+  if (D.ae32) launch_build_records<float>(D, D.ae32, D.be32, D.bte32, c->stream);
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -223,19 +243,34 @@ template <>
 OpView<float> opview<float>(const DevLevel &D) { return D.op32(); }
 
 // TNSE_MGLevel4::CellVanka, colored (NSE_MGLevel4.C:603-1649), one sweep
+// bytes of one stored-factor patch solve: factors, local residual read, u read+write,
+// dof maps and the cell index
+inline double solve_bytes(int fbytes, int vs) {
+  return double(NLOC) * NLOC * fbytes + NLOC * double(vs) + 2.0 * NLOC * vs + (NU_LOC + NP_LOC + 1) * 4.0;
+}
+
+// TNSE_MGLevel4::CellVanka, colored (NSE_MGLevel4.C:603-1649), one sweep
 template <typename T>
 void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
-  Timed tm(c, &c->tm.sweep_ms, &c->tm.sweep_calls);
+  const bool prof = c->profiling;
+  auto t0 = std::chrono::steady_clock::now();
   const double alpha = c->prm.sc_smooth_damp_factor_saddle;
   const T damp = T(c->prm.sc_smooth_damp_factor_coarse_saddle);
   const bool relax = std::fabs(1 - alpha) > 1e-3;
+  const int vs = sizeof(T);
+  const int fb = sizeof(T) == 8 ? 8 : (c->prm.precision == PMG_PREC_FP32_BF16 ? 2 : 4);
   set_dirichlet_t<T>(c, D, u, f);
   if (relax) launch_copy<T>(aux, u, D.n_dof, c->stream);
   const OpView<T> op = opview<T>(D);
   for (int col = 0; col < D.n_colors; ++col) {
     const int p0 = D.ptr_colors[col], nc = D.ptr_colors[col + 1] - p0;
-    launch_vanka_residual<T>(op, D.cell_reorder, nc, p0, D.u_global, D.p_global, D.invperm, u, f, rloc,
-                             c->stream);
+    {
+      const double rb = D.res_bytes[col] * (vs == 8 ? 1.0 : 0.5);
+      Timed tm(c, &c->tm.vanka_res, rb, 0.0);
+      launch_vanka_residual<T>(op, D.cell_reorder, nc, p0, D.u_global, D.p_global, D.invperm,
+                               D.res_items + size_t(p0) * (NU_LOC + NP_LOC), u, f, rloc, c->stream);
+    }
+    Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));
     if constexpr (sizeof(T) == 8) {
       launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global, D.n_udof, u,
                                          damp, c->stream);
@@ -250,23 +285,27 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
   }
   if (relax) launch_relax<T>(u, aux, T(alpha), D.n_dof, c->stream);
   set_dirichlet_t<T>(c, D, u, f);
+  if (prof) {
+    c->tm.sweep_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->tm.sweep_calls += 1;
+  }
 }
 
 template <typename T>
 void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, double *res_dev) {
-  Timed tm(c, &c->tm.defect_ms, &c->tm.defect_calls);
+  Timed tm(c, &c->tm.defect, D.defect_bytes * (sizeof(T) == 8 ? 1.0 : 0.5), 0.0);
   launch_type4<T>(opview<T>(D), 2, u, f, d, c->prm.internal_project_pressure != 0, res_dev, c->red, c->stream);
 }
 
 template <typename T>
 void restrict_defect(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *d, T *fc) {
-  Timed tm(c, &c->tm.transfer_ms, &c->tm.transfer_calls);
+  Timed tm(c, &c->tm.transfer, F.transfer_bytes, 0.0);
   launch_csr_spmv<T>(F.ru, 3, d, F.n_udof, fc, C.n_udof, c->stream);
   launch_csr_spmv<T>(F.rp, 1, d + 3L * F.n_udof, 0, fc + 3L * C.n_udof, 0, c->stream);
 }
 template <typename T>
 void prolongate(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *uc, T *d) {
-  Timed tm(c, &c->tm.transfer_ms, &c->tm.transfer_calls);
+  Timed tm(c, &c->tm.transfer, F.transfer_bytes, 0.0);
   launch_csr_spmv<T>(F.pu, 3, uc, C.n_udof, d, F.n_udof, c->stream);
   launch_csr_spmv<T>(F.pp, 1, uc + 3L * C.n_udof, 0, d + 3L * F.n_udof, 0, c->stream);
 }
@@ -274,13 +313,13 @@ void prolongate(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *uc, T
 // coarse solve in FP64 (vectors of type T converted at the boundary)
 template <typename T>
 void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
-  Timed tm(c, &c->tm.coarse_ms, &c->tm.coarse_calls);
-  const int pin = c->prm.internal_project_pressure ? 3 * D.n_udof : -1;
+  const double n = c->coarse.n;
+  Timed tm(c, &c->tm.coarse_solve, n * n * 8.0, 2.0 * n * n);
   if constexpr (sizeof(T) == 8) {
-    coarse_lu_solve(c->coarse, f, u, pin, c->stream);
+    coarse_lu_solve(c->coarse, D.op64(), f, u, c->stream);
   } else {
     launch_f2d(D.aux0, f, D.n_dof, c->stream);
-    coarse_lu_solve(c->coarse, D.aux0, D.sol, pin, c->stream);
+    coarse_lu_solve(c->coarse, D.op64(), D.aux0, D.sol, c->stream);
     launch_d2f(u, D.sol, D.n_dof, c->stream);
   }
 }
@@ -350,44 +389,76 @@ void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
 }
 
 // ------------------------------------------------------------------ set-up
+// value-dependent set-up: the dense coarse LU runs on a second stream while the batched
+// patch LUs of all finer levels occupy the main stream
 void setup_factors(pmg_ctx *c) {
-  for (size_t l = 1; l < c->L.size(); ++l) {
-    DevLevel &D = c->L[l];
-    if (D.factors_ready) continue;
-    auto t0 = std::chrono::steady_clock::now();
-    launch_patch_lu(D, D.lu, D.patch_info, c->stream);
-    if (D.lu32) launch_convert_lu_f32(D.lu, D.lu32, long(D.n_cells) * LU_STRIDE, c->stream);
-    if (D.lu16) launch_convert_lu_bf16(D.lu, D.lu16, long(D.n_cells) * LU_STRIDE, c->stream);
-    std::vector<int> info(D.n_cells);
-    PMG_HIP_CHECK(hipMemcpyAsync(info.data(), D.patch_info, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost,
-                                 c->stream));
-    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
-    c->tm.patch_lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    for (int p = 0; p < D.n_cells; ++p)
-      if (info[p]) {
-        const int pos = p;
-        throw Fail{PMG_ERR_SINGULAR_PATCH, "zero pivot " + std::to_string(info[pos]) + " in the patch at color position " +
-                                               std::to_string(pos) + " of level " + std::to_string(l)};
-      }
-    D.factors_ready = true;
-  }
-  if (!c->coarse.ready) {
-    auto t0 = std::chrono::steady_clock::now();
+  auto t0 = std::chrono::steady_clock::now();
+  const bool do_coarse = !c->coarse.ready;
+  Coarse &C = c->coarse;
+  if (do_coarse) {
     DevLevel &D = c->L[0];
-    Coarse &C = c->coarse;
+    const int pin = c->prm.internal_project_pressure ? 1 : 0;
+    const int nr = 3 * D.n_active + D.n_pdof - pin;
+    if (C.a && C.n != nr) {
+      for (void *p : {(void *)C.a, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y, (void *)C.x})
+        dfree(c, p);
+      C.a = nullptr;
+    }
     if (!C.a) {
-      C.n = D.n_dof;
+      C.n = nr;
       C.a = dalloc<double>(c, size_t(C.n) * C.n);
       C.ipiv = dalloc<int>(c, C.n);
       C.sigma = dalloc<int>(c, C.n);
       C.b = dalloc<double>(c, C.n);
       C.y = dalloc<double>(c, C.n);
-      C.info = dalloc<int>(c, 1);
+      C.x = dalloc<double>(c, C.n);
+      if (!C.info) C.info = dalloc<int>(c, 1);
     }
-    launch_coarse_build(D, C.a, c->prm.internal_project_pressure != 0, c->stream);
-    const int info = coarse_lu_factor(C, c->stream);
-    c->tm.coarse_lu_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (info) throw Fail{PMG_ERR_SINGULAR_COARSE, "zero pivot " + std::to_string(info) + " in the coarse LU"};
+    C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;
+    PMG_HIP_CHECK(hipEventRecord(c->evs, c->stream));
+    PMG_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->evs, 0));
+    launch_coarse_build(D, C, c->stream2);
+    coarse_lu_factor(C, c->stream2);
+  }
+  std::vector<std::vector<int>> infos(c->L.size());
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    DevLevel &D = c->L[l];
+    if (D.factors_ready) continue;
+    launch_patch_lu(D, D.lu, D.patch_info, c->stream);
+    if (D.lu32) launch_convert_lu_f32(D.lu, D.lu32, long(D.n_cells) * LU_STRIDE + LU_PAD, c->stream);
+    if (D.lu16) launch_convert_lu_bf16(D.lu, D.lu16, long(D.n_cells) * LU_STRIDE + LU_PAD, c->stream);
+    infos[l].resize(D.n_cells);
+    PMG_HIP_CHECK(hipMemcpyAsync(infos[l].data(), D.patch_info, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost,
+                                 c->stream));
+  }
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  const double t_patch = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  int cinfo = 0;
+  if (do_coarse) {
+    PMG_HIP_CHECK(hipMemcpyAsync(&cinfo, C.info, sizeof(int), hipMemcpyDeviceToHost, c->stream2));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream2));
+  }
+  const double t_all = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    DevLevel &D = c->L[l];
+    if (D.factors_ready) continue;
+    c->tm.patch_lu.calls += 1;
+    c->tm.patch_lu.bytes += double(D.n_cells) * (NLOC * NLOC * 8.0) + D.defect_bytes;
+    c->tm.patch_lu.flops += double(D.n_cells) * (2.0 / 3.0) * NLOC * NLOC * NLOC;
+    for (int p = 0; p < D.n_cells; ++p)
+      if (infos[l][p])
+        throw Fail{PMG_ERR_SINGULAR_PATCH, "zero pivot " + std::to_string(infos[l][p]) +
+                                               " in the patch at color position " + std::to_string(p) +
+                                               " of level " + std::to_string(l)};
+    D.factors_ready = true;
+  }
+  c->tm.patch_lu.ms += t_patch;
+  if (do_coarse) {
+    c->tm.coarse_lu.ms += t_all;   // wall time from the start of set-up (runs concurrently)
+    c->tm.coarse_lu.calls += 1;
+    c->tm.coarse_lu.bytes += double(C.n) * C.n * 16.0;
+    c->tm.coarse_lu.flops += (2.0 / 3.0) * double(C.n) * C.n * C.n;
+    if (cinfo) throw Fail{PMG_ERR_SINGULAR_COARSE, "zero pivot " + std::to_string(cinfo) + " in the coarse LU"};
     C.ready = true;
   }
   c->setup_done = true;
@@ -426,6 +497,7 @@ double dev_norm2(pmg_ctx *c, const double *x, long n) {
 
 void defect_nse4_dev(pmg_ctx *c, const double *x, const double *b, double *r) {
   DevLevel &F = c->L.back();
+  Timed tm(c, &c->tm.matvec, F.defect_bytes, 0.0);
   launch_type4<double>(F.op64(), 1, x, b, r, c->prm.internal_project_pressure != 0, nullptr, c->red, c->stream);
 }
 
@@ -501,7 +573,10 @@ int fgmres_dev(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *f
     s[0] = beta;
     for (int i = 0; i < restart && j <= maxite; i++, j++) {
       for (int k = 0; k < std::max(1, P.sc_amg_prec_it); ++k) vcycle_dev(c, v(i), z(i));
-      launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
+      {
+        Timed tm(c, &c->tm.matvec, F.matvec_bytes, 0.0);
+        launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
+      }
       // modified Gram-Schmidt, H[k][i] -> Hdev[k], ||w||^2 -> Hdev[i+1]
       launch_dot(c->W, v(0), n, c->Hdev, c->red, c->stream);
       for (int k = 0; k <= i; ++k) {
@@ -658,6 +733,8 @@ int pmg_ctx_create(int device, const pmg_params *params, pmg_ctx **out) {
     if (device < 0 || device >= n) throw Fail{PMG_ERR_INVALID_ARGUMENT, "no such device"};
     PMG_HIP_CHECK(hipSetDevice(device));
     PMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    PMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    PMG_HIP_CHECK(hipEventCreateWithFlags(&c->evs, hipEventDisableTiming));
     PMG_HIP_CHECK(hipEventCreate(&c->ev0));
     PMG_HIP_CHECK(hipEventCreate(&c->ev1));
     c->red.partials = dalloc<double>(c.get(), 2 * RED_BLOCKS + 16);
@@ -677,10 +754,13 @@ int pmg_ctx_destroy(pmg_ctx *c) {
   if (!c) return PMG_ERR_INVALID_ARGUMENT;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->hpin) (void)hipHostFree(c->hpin);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evs) (void)hipEventDestroy(c->evs);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PMG_OK;
@@ -744,10 +824,13 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     D.a = dalloc<double>(c, 9 * size_t(D.a_nnz));
     D.b = dalloc<double>(c, 3 * size_t(D.b_nnz));
     D.bt = dalloc<double>(c, 3 * size_t(D.bt_nnz));
+    D.ae64 = dalloc<AEnt<double>>(c, size_t(D.a_nnz));
+    D.be64 = dalloc<BEnt<double>>(c, size_t(D.b_nnz));
+    D.bte64 = dalloc<BEnt<double>>(c, size_t(D.bt_nnz));
     if (mixed(c)) {
-      D.a32 = dalloc<float>(c, 9 * size_t(D.a_nnz));
-      D.b32 = dalloc<float>(c, 3 * size_t(D.b_nnz));
-      D.bt32 = dalloc<float>(c, 3 * size_t(D.bt_nnz));
+      D.ae32 = dalloc<AEnt<float>>(c, size_t(D.a_nnz));
+      D.be32 = dalloc<BEnt<float>>(c, size_t(D.b_nnz));
+      D.bte32 = dalloc<BEnt<float>>(c, size_t(D.bt_nnz));
     }
     upload_values(c, D, d);
     // dof maps (UGlobalNumbers + UBeginIndex[cell])
@@ -784,6 +867,48 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     D.max_cells_per_color = 0;
     for (int k = 0; k < nc; ++k) D.max_cells_per_color = std::max(D.max_cells_per_color, ptr[k + 1] - ptr[k]);
     D.cell_reorder = dalloc<int>(c, D.n_cells); h2d(c, D.cell_reorder, reo.data(), D.n_cells);
+    {
+      // residual work items of each color sorted by the operator row they stream
+      std::vector<int> items(size_t(D.n_cells) * (NU_LOC + NP_LOC));
+      std::vector<std::pair<int, int>> key;
+      for (int k = 0; k < nc; ++k) {
+        key.clear();
+        for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
+          const int cell = reo[q], slot = q - ptr[k];
+          for (int j = 0; j < NU_LOC; ++j) key.emplace_back(ug[size_t(cell) * NU_LOC + j], slot * 32 + j);
+          for (int j = 0; j < NP_LOC; ++j)
+            key.emplace_back(D.n_udof + pg[size_t(cell) * NP_LOC + j], slot * 32 + NU_LOC + j);
+        }
+        std::sort(key.begin(), key.end());
+        for (size_t i = 0; i < key.size(); ++i) items[size_t(ptr[k]) * (NU_LOC + NP_LOC) + i] = key[i].second;
+      }
+      D.res_items = dalloc<int>(c, items.size());
+      h2d(c, D.res_items, items.data(), items.size());
+    }
+    {
+      RowBytes rb;
+      double vb = 0;
+      for (int i = 0; i < D.n_udof; ++i) vb += rb.vel(d, i, 8);
+      double pb = 0;
+      for (int i = 0; i < D.n_pdof; ++i) pb += rb.pre(d, i, 8);
+      // defect: rows + x read + f read + d write; matvec: no f
+      D.defect_bytes = vb + pb + 3.0 * 8.0 * D.n_dof;
+      D.matvec_bytes = vb + pb + 2.0 * 8.0 * D.n_dof;
+      D.res_bytes.assign(nc, 0.0);
+      for (int k = 0; k < nc; ++k) {
+        double b = 0;
+        for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
+          const int cell = reo[q];
+          b += 4;  // cell index
+          for (int j = 0; j < NU_LOC; ++j) {
+            const int g = ug[size_t(cell) * NU_LOC + j];
+            b += rb.vel(d, g, 8) + 3 * 8 + 3 * 8 + 4 + 3;   // row, f, local residual, dof, perm
+          }
+          for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], 8) + 8 + 8 + 4 + 1;
+        }
+        D.res_bytes[k] = b;
+      }
+    }
     // vectors
     D.sol = dalloc<double>(c, D.n_dof); D.rhs = dalloc<double>(c, D.n_dof);
     D.aux0 = dalloc<double>(c, D.n_dof); D.aux1 = dalloc<double>(c, D.n_dof);
@@ -794,9 +919,11 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
       D.aux0_32 = dalloc<float>(c, D.n_dof); D.aux1_32 = dalloc<float>(c, D.n_dof);
     }
     if (lvl > 0) {
-      D.lu = dalloc<double>(c, size_t(D.n_cells) * LU_STRIDE);
-      if (c->prm.precision == PMG_PREC_FP32) D.lu32 = dalloc<float>(c, size_t(D.n_cells) * LU_STRIDE);
-      if (c->prm.precision == PMG_PREC_FP32_BF16) D.lu16 = dalloc<uint16_t>(c, size_t(D.n_cells) * LU_STRIDE);
+      const size_t lun = size_t(D.n_cells) * LU_STRIDE + LU_PAD;
+      D.lu = dalloc<double>(c, lun);
+      launch_fill<double>(D.lu, 0.0, long(lun), c->stream);
+      if (c->prm.precision == PMG_PREC_FP32) D.lu32 = dalloc<float>(c, lun);
+      if (c->prm.precision == PMG_PREC_FP32_BF16) D.lu16 = dalloc<uint16_t>(c, lun);
       D.invperm = dalloc<uint8_t>(c, size_t(D.n_cells) * PERM_STRIDE);
       D.patch_info = dalloc<int>(c, D.n_cells);
       D.rloc = dalloc<double>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
@@ -810,6 +937,14 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
                      D.n_udof, C.n_udof, D.pu, D.ru);
       build_transfer(c, NP_LOC, NP_LOC, D.n_cells, d->parent, d->child, d->qq_p, pg, c->h_p_global[lvl - 1],
                      D.n_pdof, C.n_pdof, D.pp, D.rp);
+    }
+    if (lvl > 0) {
+      const DevLevel &C = c->L[lvl - 1];
+      auto tb = [](const Csr &m, int ncomp) {
+        return double(m.nnz) * 12.0 + (m.n_rows + 1) * 4.0 + double(ncomp) * (m.n_rows + m.n_cols) * 8.0;
+      };
+      D.transfer_bytes = tb(D.ru, 3) + tb(D.rp, 1);
+      (void)C;
     }
     c->h_u_global.push_back(std::move(ug));
     c->h_p_global.push_back(std::move(pg));
@@ -838,6 +973,17 @@ int pmg_setup(pmg_ctx *c) {
   return guard(c, [&] {
     if (c->L.empty()) throw Fail{PMG_ERR_NOT_READY, "no levels"};
     PMG_HIP_CHECK(hipSetDevice(c->device));
+    setup_factors(c);
+  });
+}
+
+int pmg_refactor(pmg_ctx *c) {
+  if (!c) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (c->L.empty()) throw Fail{PMG_ERR_NOT_READY, "no levels"};
+    PMG_HIP_CHECK(hipSetDevice(c->device));
+    for (auto &D : c->L) D.factors_ready = false;
+    c->coarse.ready = false;
     setup_factors(c);
   });
 }

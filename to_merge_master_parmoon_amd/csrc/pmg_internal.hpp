@@ -14,12 +14,41 @@ constexpr int NU_LOC = 27;          // Q2 hex velocity dofs per cell (per compon
 constexpr int NP_LOC = 4;           // P1disc hex pressure dofs per cell
 constexpr int NLOC = 3 * NU_LOC + NP_LOC;  // 85: local Vanka system size
 constexpr int LU_STRIDE = 7232;     // 85*85 = 7225 padded to a multiple of 16 entries
+// packed patch factors: L (strictly lower, 3570 entries) column by column, then U (upper
+// with diagonal, 3655 entries) column by column starting at a 16-byte aligned offset
+constexpr int LU_L = NLOC * (NLOC - 1) / 2;   // 3570
+constexpr int LU_U = NLOC * (NLOC + 1) / 2;   // 3655
+constexpr int LU_UOFF = 3576;                 // LU_L rounded up to 8 entries
+constexpr int LU_PAD = 512;                   // tail entries after the last patch (vector overshoot)
+__host__ __device__ constexpr int lu_loff(int k) { return k * (NLOC - 1) - k * (k - 1) / 2; }  // L column k
+__host__ __device__ constexpr int lu_uoff(int k) { return k * (k + 1) / 2; }                   // U column k
+static_assert(LU_UOFF + LU_U <= LU_STRIDE, "packed factors exceed the patch stride");
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
 
-// Type-4 operator in structure-of-arrays form: A values a[b*a_nnz + k] for the 9 blocks
-// A11 A12 A13 A21 A22 A23 A31 A32 A33, B/BT values b[c*b_nnz + k], bt[c*bt_nnz + k].
+// Operator entries interleaved per nonzero (the streaming kernels read these): column
+// index and the nine A blocks A11 A12 A13 A21 A22 A23 A31 A32 A33 of one CSR entry, or the
+// column and the three B (BT) blocks.  A row is one contiguous run of records, so a row
+// touches only whole cache lines except at its two ends.
+template <typename T>
+struct AEnt;
+template <>
+struct alignas(16) AEnt<double> { int col; int pad; double v[9]; };   // 80 B
+template <>
+struct alignas(8) AEnt<float> { int col; float v[9]; };               // 40 B
+template <typename T>
+struct BEnt;
+template <>
+struct alignas(16) BEnt<double> { int col; int pad; double v[3]; };   // 32 B
+template <>
+struct alignas(16) BEnt<float> { int col; float v[3]; };              // 16 B
+static_assert(sizeof(AEnt<double>) == 80 && sizeof(AEnt<float>) == 40, "record size");
+static_assert(sizeof(BEnt<double>) == 32 && sizeof(BEnt<float>) == 16, "record size");
+
+// Type-4 operator.  Structure-of-arrays values (FP64 only, the reference layout: A values
+// a[b*a_nnz + k] for the 9 blocks, B/BT values b[c*b_nnz + k]) feed the set-up kernels;
+// the interleaved records feed the streaming kernels.
 template <typename T>
 struct OpView {
   int n_udof, n_active, n_pdof;
@@ -32,7 +61,24 @@ struct OpView {
   const int *bt_rowptr, *bt_kcol;
   const T *bt;
   long bt_nnz;
+  const AEnt<T> *ae;
+  const BEnt<T> *be, *bte;
 };
+
+// hardware dispatches workgroup b to XCD b % 8: logical block ids that make the blocks of
+// one XCD cover one contiguous range of rows (one L2 sees neighbouring rows and columns)
+__device__ inline int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int per = G >> 3, rem = G & 7, x = b & 7, i = b >> 3;
+  return x * per + (x < rem ? x : rem) + i;
+}
+// grid size for n_items processed in chunks by 'per_block' workers: a multiple of 8
+inline int xcd_grid(long n_items, long per_block, int cap) {
+  long g = (n_items + per_block - 1) / per_block;
+  if (g < 8) g = 8;
+  if (g > cap) g = cap;
+  return int((g + 7) / 8 * 8);
+}
 
 struct Csr {
   int n_rows = 0, n_cols = 0, nnz = 0;
@@ -49,7 +95,13 @@ struct DevLevel {
   int *a_rowptr = nullptr, *a_kcol = nullptr, *b_rowptr = nullptr, *b_kcol = nullptr;
   int *bt_rowptr = nullptr, *bt_kcol = nullptr;
   double *a = nullptr, *b = nullptr, *bt = nullptr;
-  float *a32 = nullptr, *b32 = nullptr, *bt32 = nullptr;
+  AEnt<double> *ae64 = nullptr;
+  BEnt<double> *be64 = nullptr, *bte64 = nullptr;
+  AEnt<float> *ae32 = nullptr;
+  BEnt<float> *be32 = nullptr, *bte32 = nullptr;
+  // per color, the (slot * 32 + local row) work items of the residual kernel sorted by the
+  // global row they read (velocity rows, then pressure rows)
+  int *res_items = nullptr;
   // cells
   int *u_global = nullptr;           // n_cells * 27 (UGlobalNumbers + UBeginIndex[cell])
   int *p_global = nullptr;           // n_cells * 4
@@ -73,19 +125,29 @@ struct DevLevel {
   float *rloc32 = nullptr;
   // host copies kept for value updates / coarse assembly
   std::vector<int> h_a_rowptr, h_a_kcol, h_b_rowptr, h_b_kcol, h_bt_rowptr, h_bt_kcol;
+  // algorithmic bytes (FP64 operator) of the per-color residual launches and of one defect
+  std::vector<double> res_bytes;
+  double defect_bytes = 0, matvec_bytes = 0;
+  double transfer_bytes = 0;
 
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
-                          b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz};
+                          b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz,
+                          ae64, be64, bte64};
   }
   OpView<float> op32() const {
-    return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a32, a_nnz,
-                         b_rowptr, b_kcol, b32, b_nnz, bt_rowptr, bt_kcol, bt32, bt_nnz};
+    return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, nullptr, a_nnz,
+                         b_rowptr, b_kcol, nullptr, b_nnz, bt_rowptr, bt_kcol, nullptr, bt_nnz,
+                         ae32, be32, bte32};
   }
 };
 
+// Coarse level: the dense system of SolveExact with the Dirichlet rows (identity) and the
+// pinned pressure row eliminated; n = 3*na + np - pin unknowns remain.
 struct Coarse {
   int n = 0;
+  int N = 0, na = 0, np = 0, pin = 0;   // velocity dofs, active velocity dofs, pressure dofs, pinned?
+  double *x = nullptr;     // reduced solution
   double *a = nullptr;     // dense column-major LU
   int *ipiv = nullptr;     // device
   int *sigma = nullptr;    // b'[i] = b[sigma[i]]
@@ -114,7 +176,10 @@ void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s);
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0,
                            const int *u_global, const int *p_global, const uint8_t *invperm,
-                           const T *u, const T *f, T *rloc, hipStream_t s);
+                           const int *items, const T *u, const T *f, T *rloc, hipStream_t s);
+// interleaved operator records from the structure-of-arrays FP64 values
+template <typename T>
+void launch_build_records(const DevLevel &L, AEnt<T> *ae, BEnt<T> *be, BEnt<T> *bte, hipStream_t s);
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc,
                         const int *u_global, const int *p_global, int n_udof, T *u, T damp,
@@ -144,9 +209,9 @@ void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long x_stride, T *y, l
                      hipStream_t s);
 
 // dense coarse LU / solve
-void launch_coarse_build(const DevLevel &L, double *a, bool pin, hipStream_t s);
-int  coarse_lu_factor(Coarse &C, hipStream_t s);
-void coarse_lu_solve(Coarse &C, const double *f, double *u, int pin_index, hipStream_t s);
+void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s);
+void coarse_lu_factor(Coarse &C, hipStream_t s);   // asynchronous; C.info gets the dgetrf info
+void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s);
 
 // BLAS-1 for FGMRES (deterministic two-stage reductions)
 void launch_dot(const double *x, const double *y, long n, double *out, RedScratch &red, hipStream_t s);

@@ -102,6 +102,10 @@ class NSEMultiGrid:
     def setup(self):
         self._check(self._lib.pmg_setup(self._ctx), "pmg_setup")
 
+    def refactor(self):
+        """Recompute all value-dependent factors from the resident operator."""
+        self._check(self._lib.pmg_refactor(self._ctx), "pmg_refactor")
+
     def level_info(self, level: int) -> dict:
         n, nc, mc = C.c_int(), C.c_int(), C.c_int()
         self._check(self._lib.pmg_level_info(self._ctx, level, C.byref(n), C.byref(nc), C.byref(mc)))
@@ -198,7 +202,12 @@ class NSEMultiGrid:
     def timers(self) -> dict:
         t = _abi.PmgTimers()
         self._lib.pmg_timers_get(self._ctx, C.byref(t))
-        return {k: getattr(t, k) for k, _ in t._fields_}
+        out = {}
+        for k in _abi.KSTATS:
+            ks = getattr(t, k)
+            out[k] = dict(ms=ks.ms, calls=ks.calls, bytes=ks.bytes, flops=ks.flops)
+        out["sweep_ms"], out["sweep_calls"] = t.sweep_ms, t.sweep_calls
+        return out
 
     def reset_timers(self):
         self._lib.pmg_timers_reset(self._ctx)
