@@ -911,7 +911,7 @@ int oracle_create(const pmg_params *params, const pmg_level_desc *levels, int n_
       L.maxCellsPerColor = std::max(L.maxCellsPerColor, L.ptrCellColors[c + 1] - L.ptrCellColors[c]);
     L.sol.assign(L.N_DOF, 0.0); L.rhs.assign(L.N_DOF, 0.0);
     L.aux0.assign(L.N_DOF, 0.0); L.aux1.assign(L.N_DOF, 0.0);
-    if (!refactor && l > 0) {
+    if (!refactor) {  // every level can be smoothed (the coarsest one too)
       L.lu.assign(size_t(L.d.n_cells) * L.N_LocalDOF * L.N_LocalDOF, 0.0);
       L.piv.assign(size_t(L.d.n_cells) * L.N_LocalDOF, 0);
       L.lu_ready.assign(L.d.n_cells, 0);
