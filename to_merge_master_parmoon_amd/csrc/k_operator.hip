@@ -10,6 +10,7 @@
 // deterministic last-block finish: every block writes its partial sums, the block that
 // arrives last sums them in a fixed order and applies the projection.
 #include "pmg_internal.hpp"
+#include "pmg_rows.hpp"
 
 namespace pmg {
 
@@ -81,47 +82,27 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
   __shared__ double sh[8];
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
-  const AEnt<T> *__restrict__ AE = op.ae;
-  const BEnt<T> *__restrict__ BE = op.be;
-  const BEnt<T> *__restrict__ BTE = op.bte;
   const T *__restrict__ x1 = x;
   const T *__restrict__ x2 = x + N;
   const T *__restrict__ x3 = x + 2 * N;
   const T *__restrict__ xp = x + 3 * N;
   double pc = 0.0, sq = 0.0;
-  // this block's contiguous row range (XCD-aware logical block id)
-  const int nrows = N + NP;
-  const int per = (nrows + gridDim.x - 1) / gridDim.x;
-  const int lb = xcd_block();
-  const int r0 = lb * per, r1 = min(nrows, r0 + per);
+  // this block's chunk of rows
+  int r0, r1;
+  chunk_of_block(long(N) + NP, r0, r1);
   for (int r = r0 + grp; r < r1; r += 16) {
     if (r < N) {
       T s1 = 0, s2 = 0, s3 = 0;
-      const int k0 = op.a_rowptr[r], k1 = op.a_rowptr[r + 1];
+      const int k0 = op.a_rowptr[r], n = op.a_rowptr[r + 1] - k0;
+      const int *__restrict__ kc = op.a_kcol + k0;
+      const T *__restrict__ v = op.av + 9L * k0;
       if (r < NA) {
-        for (int k = k0 + lane; k < k1; k += 16) {
-          const AEnt<T> e = AE[k];
-          const T u1 = x1[e.col], u2 = x2[e.col], u3 = x3[e.col];
-          s1 += e.v[0] * u1 + e.v[1] * u2 + e.v[2] * u3;
-          s2 += e.v[3] * u1 + e.v[4] * u2 + e.v[5] * u3;
-          s3 += e.v[6] * u1 + e.v[7] * u2 + e.v[8] * u3;
-        }
-        const int q0 = op.bt_rowptr[r], q1 = op.bt_rowptr[r + 1];
-        for (int k = q0 + lane; k < q1; k += 16) {
-          const BEnt<T> e = BTE[k];
-          const T pv = xp[e.col];
-          s1 += e.v[0] * pv;
-          s2 += e.v[1] * pv;
-          s3 += e.v[2] * pv;
-        }
+        row_a9<T, 4>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+        const int q0 = op.bt_rowptr[r], m = op.bt_rowptr[r + 1] - q0;
+        row_bt<T, 2>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, xp, s1, s2, s3);
       } else {
         // rows N_Active..N_UDOF: diagonal blocks only (MGComponents3D.C:3136-3156)
-        for (int k = k0 + lane; k < k1; k += 16) {
-          const AEnt<T> e = AE[k];
-          s1 += e.v[0] * x1[e.col];
-          s2 += e.v[4] * x2[e.col];
-          s3 += e.v[8] * x3[e.col];
-        }
+        row_a_diag<T, 1>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
       }
       s1 = red16(s1);
       s2 = red16(s2);
@@ -140,11 +121,8 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
     } else {
       const int pr = r - N;
       T s = 0;
-      const int k0 = op.b_rowptr[pr], k1 = op.b_rowptr[pr + 1];
-      for (int k = k0 + lane; k < k1; k += 16) {
-        const BEnt<T> e = BE[k];
-        s += e.v[0] * x1[e.col] + e.v[1] * x2[e.col] + e.v[2] * x3[e.col];
-      }
+      const int k0 = op.b_rowptr[pr], n = op.b_rowptr[pr + 1] - k0;
+      row_b<T, 2>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, x1, x2, x3, s);
       s = red16(s);
       if (lane == 0) {
         const T v = (MODE == 0) ? s : T(bv[3 * N + pr] - s);
@@ -287,8 +265,7 @@ inline int grid_for(long n, int bs = 256, int cap = 8192) {
 template <typename T>
 void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project, double *res,
                   RedScratch &red, hipStream_t s) {
-  const long rows = long(op.n_udof) + op.n_pdof;
-  const int grid = xcd_grid(rows, 64, RED_BLOCKS);
+  const int grid = chunk_grid(long(op.n_udof) + op.n_pdof, 64, RED_BLOCKS);
   const int want_norm = res != nullptr;
   const int proj = (mode != 0 && project) ? 1 : 0;
   double *slots = red.result + 8;
@@ -313,45 +290,35 @@ template void launch_type4<float>(const OpView<float> &, int, const float *, con
                                   double *, RedScratch &, hipStream_t);
 
 namespace {
-template <typename T>
-__global__ void k_records_a(const int *__restrict__ col, const double *__restrict__ a, long nnz,
-                            AEnt<T> *__restrict__ out) {
-  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < nnz; k += long(gridDim.x) * blockDim.x) {
-    AEnt<T> e;
-    e.col = col[k];
-    if constexpr (sizeof(T) == 8) e.pad = 0;
+// one 16-lane group per row: row r of an nb-block CSR matrix, SoA values src[b*nnz + k],
+// goes to dst[nb*k0 + b*n + (k-k0)]
+template <typename T, int NB>
+__global__ void k_row_blocked(const int *__restrict__ rowptr, int rows, const double *__restrict__ src, long nnz,
+                              T *__restrict__ dst) {
+  const int lane = threadIdx.x & 15;
+  const long grp = (long(blockIdx.x) * blockDim.x + threadIdx.x) >> 4;
+  const long ngrp = (long(gridDim.x) * blockDim.x) >> 4;
+  for (long r = grp; r < rows; r += ngrp) {
+    const int k0 = rowptr[r], n = rowptr[r + 1] - k0;
+    T *out = dst + long(NB) * k0;
+    for (int j = lane; j < n; j += 16)
 #pragma unroll
-    for (int b = 0; b < 9; ++b) e.v[b] = T(a[b * nnz + k]);
-    out[k] = e;
-  }
-}
-template <typename T>
-__global__ void k_records_b(const int *__restrict__ col, const double *__restrict__ a, long nnz,
-                            BEnt<T> *__restrict__ out) {
-  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < nnz; k += long(gridDim.x) * blockDim.x) {
-    BEnt<T> e;
-    e.col = col[k];
-    if constexpr (sizeof(T) == 8) e.pad = 0;
-#pragma unroll
-    for (int b = 0; b < 3; ++b) e.v[b] = T(a[b * nnz + k]);
-    out[k] = e;
+      for (int b = 0; b < NB; ++b) out[long(b) * n + j] = T(src[long(b) * nnz + k0 + j]);
   }
 }
 }  // namespace
 
 template <typename T>
-void launch_build_records(const DevLevel &L, AEnt<T> *ae, BEnt<T> *be, BEnt<T> *bte, hipStream_t s) {
-  k_records_a<T><<<grid_for(L.a_nnz), 256, 0, s>>>(L.a_kcol, L.a, L.a_nnz, ae);
+void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s) {
+  k_row_blocked<T, 9><<<grid_for(16L * L.n_udof), 256, 0, s>>>(L.a_rowptr, L.n_udof, L.a, L.a_nnz, av);
   PMG_HIP_CHECK(hipGetLastError());
-  k_records_b<T><<<grid_for(L.b_nnz), 256, 0, s>>>(L.b_kcol, L.b, L.b_nnz, be);
+  k_row_blocked<T, 3><<<grid_for(16L * L.n_pdof), 256, 0, s>>>(L.b_rowptr, L.n_pdof, L.b, L.b_nnz, bv);
   PMG_HIP_CHECK(hipGetLastError());
-  k_records_b<T><<<grid_for(L.bt_nnz), 256, 0, s>>>(L.bt_kcol, L.bt, L.bt_nnz, bte);
+  k_row_blocked<T, 3><<<grid_for(16L * L.n_udof), 256, 0, s>>>(L.bt_rowptr, L.n_udof, L.bt, L.bt_nnz, btv);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template void launch_build_records<double>(const DevLevel &, AEnt<double> *, BEnt<double> *, BEnt<double> *,
-                                           hipStream_t);
-template void launch_build_records<float>(const DevLevel &, AEnt<float> *, BEnt<float> *, BEnt<float> *,
-                                          hipStream_t);
+template void launch_row_blocked<double>(const DevLevel &, double *, double *, double *, hipStream_t);
+template void launch_row_blocked<float>(const DevLevel &, float *, float *, float *, hipStream_t);
 
 template <typename T>
 void launch_fill(T *x, T v, long n, hipStream_t s) {

@@ -14,6 +14,7 @@
 // (dgetf2 pivoting, reciprocal scaling, column-oriented dtrsm), so FP64 factors and
 // local solves agree with it bit for bit.
 #include "pmg_internal.hpp"
+#include "pmg_rows.hpp"
 
 namespace pmg {
 
@@ -189,8 +190,8 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *
 // ------------------------------------------------------------------ per-color residual
 // (slot, local row) items, 16 lanes each: rows 0..26 produce the three velocity
 // components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061).
-// items[] lists slot*32 + row sorted by the global row, and each block takes one
-// contiguous run of it (XCD-aware), so neighbouring rows share cache lines and L2.
+// items[] lists slot*32 + row sorted by the global row and each block takes one contiguous
+// run of it, so neighbouring rows share cache lines.
 template <typename T>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__restrict__ cells, int ncells,
                                                    int pos0, const int *__restrict__ u_global,
@@ -201,16 +202,12 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
                                                    T *__restrict__ rloc) {
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active;
-  const AEnt<T> *__restrict__ AE = op.ae;
-  const BEnt<T> *__restrict__ BE = op.be;
-  const BEnt<T> *__restrict__ BTE = op.bte;
   const T *__restrict__ u1 = u;
   const T *__restrict__ u2 = u + N;
   const T *__restrict__ u3 = u + 2 * N;
   const T *__restrict__ up = u + 3 * N;
-  const int nitems = ncells * (NU_LOC + NP_LOC);
-  const int per = (nitems + gridDim.x - 1) / gridDim.x;
-  const int i0 = xcd_block() * per, i1 = min(nitems, i0 + per);
+  int i0, i1;
+  chunk_of_block(long(ncells) * (NU_LOC + NP_LOC), i0, i1);
   for (int i = i0 + grp; i < i1; i += 16) {
     const int it = items[i];
     const int slot = it >> 5, r = it & 31;
@@ -221,30 +218,15 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
     if (r < NU_LOC) {
       const int UDOF = u_global[long(cell) * NU_LOC + r];
       T s1 = 0, s2 = 0, s3 = 0;
-      const int k0 = op.a_rowptr[UDOF], k1 = op.a_rowptr[UDOF + 1];
+      const int k0 = op.a_rowptr[UDOF], n = op.a_rowptr[UDOF + 1] - k0;
+      const int *__restrict__ kc = op.a_kcol + k0;
+      const T *__restrict__ v = op.av + 9L * k0;
       if (UDOF < NA) {
-        for (int k = k0 + lane; k < k1; k += 16) {
-          const AEnt<T> e = AE[k];
-          const T x1 = u1[e.col], x2 = u2[e.col], x3 = u3[e.col];
-          s1 += e.v[0] * x1 + e.v[1] * x2 + e.v[2] * x3;
-          s2 += e.v[3] * x1 + e.v[4] * x2 + e.v[5] * x3;
-          s3 += e.v[6] * x1 + e.v[7] * x2 + e.v[8] * x3;
-        }
-        const int q0 = op.bt_rowptr[UDOF], q1 = op.bt_rowptr[UDOF + 1];
-        for (int k = q0 + lane; k < q1; k += 16) {
-          const BEnt<T> e = BTE[k];
-          const T pv = up[e.col];
-          s1 += e.v[0] * pv;
-          s2 += e.v[1] * pv;
-          s3 += e.v[2] * pv;
-        }
+        row_a9<T, 4>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
+        const int q0 = op.bt_rowptr[UDOF], m = op.bt_rowptr[UDOF + 1] - q0;
+        row_bt<T, 2>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
       } else {
-        for (int k = k0 + lane; k < k1; k += 16) {
-          const AEnt<T> e = AE[k];
-          s1 += e.v[0] * u1[e.col];
-          s2 += e.v[4] * u2[e.col];
-          s3 += e.v[8] * u3[e.col];
-        }
+        row_a_diag<T, 1>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
       }
       s1 = red16(s1);
       s2 = red16(s2);
@@ -258,10 +240,8 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
       const int m = r - NU_LOC;
       const int PDOF = p_global[long(cell) * NP_LOC + m];
       T s = 0;
-      for (int k = op.b_rowptr[PDOF] + lane; k < op.b_rowptr[PDOF + 1]; k += 16) {
-        const BEnt<T> e = BE[k];
-        s += e.v[0] * u1[e.col] + e.v[1] * u2[e.col] + e.v[2] * u3[e.col];
-      }
+      const int k0 = op.b_rowptr[PDOF], n = op.b_rowptr[PDOF + 1] - k0;
+      row_b<T, 2>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
       s = red16(s);
       if (lane == 0) out[ip[3 * NU_LOC + m]] = f[3 * N + PDOF] - s;
     }
@@ -411,10 +391,10 @@ void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s) {
 
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0, const int *u_global,
-                           const int *p_global, const uint8_t *invperm, const int *items, const T *u, const T *f,
-                           T *rloc, hipStream_t s) {
+                           const int *p_global, const uint8_t *invperm, const int *items,
+                           const T *u, const T *f, T *rloc, hipStream_t s) {
   if (ncells <= 0) return;
-  const int grid = xcd_grid(long(ncells) * (NU_LOC + NP_LOC), 32, 8192);
+  const int grid = chunk_grid(long(ncells) * (NU_LOC + NP_LOC), 32, 8192);
   k_vanka_res<T><<<grid, 256, 0, s>>>(op, cells, ncells, pos0, u_global, p_global, invperm, items, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }

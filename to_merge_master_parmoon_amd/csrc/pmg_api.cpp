@@ -188,14 +188,18 @@ void build_transfer(pmg_ctx *c, int nloc_f, int nloc_c, int n_fine_cells, const 
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+}  // namespace
+
+namespace {
+
 void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
   for (int b = 0; b < 9; ++b) h2d(c, D.a + size_t(b) * D.a_nnz, d->a[b], D.a_nnz);
   for (int b = 0; b < 3; ++b) {
     h2d(c, D.b + size_t(b) * D.b_nnz, d->b[b], D.b_nnz);
     h2d(c, D.bt + size_t(b) * D.bt_nnz, d->bt[b], D.bt_nnz);
   }
-  launch_build_records<double>(D, D.ae64, D.be64, D.bte64, c->stream);
-  if (D.ae32) launch_build_records<float>(D, D.ae32, D.be32, D.bte32, c->stream);
+  launch_row_blocked<double>(D, D.av64, D.bv64, D.btv64, c->stream);
+  if (D.av32) launch_row_blocked<float>(D, D.av32, D.bv32, D.btv32, c->stream);
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
@@ -265,7 +269,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
   for (int col = 0; col < D.n_colors; ++col) {
     const int p0 = D.ptr_colors[col], nc = D.ptr_colors[col + 1] - p0;
     {
-      const double rb = D.res_bytes[col] * (vs == 8 ? 1.0 : 0.5);
+      const double rb = D.res_bytes[vs == 8 ? 0 : 1][col];
       Timed tm(c, &c->tm.vanka_res, rb, 0.0);
       launch_vanka_residual<T>(op, D.cell_reorder, nc, p0, D.u_global, D.p_global, D.invperm,
                                D.res_items + size_t(p0) * (NU_LOC + NP_LOC), u, f, rloc, c->stream);
@@ -293,7 +297,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
 
 template <typename T>
 void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, double *res_dev) {
-  Timed tm(c, &c->tm.defect, D.defect_bytes * (sizeof(T) == 8 ? 1.0 : 0.5), 0.0);
+  Timed tm(c, &c->tm.defect, D.defect_bytes[sizeof(T) == 8 ? 0 : 1], 0.0);
   launch_type4<T>(opview<T>(D), 2, u, f, d, c->prm.internal_project_pressure != 0, res_dev, c->red, c->stream);
 }
 
@@ -443,7 +447,7 @@ void setup_factors(pmg_ctx *c) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
     c->tm.patch_lu.calls += 1;
-    c->tm.patch_lu.bytes += double(D.n_cells) * (NLOC * NLOC * 8.0) + D.defect_bytes;
+    c->tm.patch_lu.bytes += double(D.n_cells) * (NLOC * NLOC * 8.0) + D.defect_bytes[0];
     c->tm.patch_lu.flops += double(D.n_cells) * (2.0 / 3.0) * NLOC * NLOC * NLOC;
     for (int p = 0; p < D.n_cells; ++p)
       if (infos[l][p])
@@ -497,7 +501,7 @@ double dev_norm2(pmg_ctx *c, const double *x, long n) {
 
 void defect_nse4_dev(pmg_ctx *c, const double *x, const double *b, double *r) {
   DevLevel &F = c->L.back();
-  Timed tm(c, &c->tm.matvec, F.defect_bytes, 0.0);
+  Timed tm(c, &c->tm.matvec, F.defect_bytes[0], 0.0);
   launch_type4<double>(F.op64(), 1, x, b, r, c->prm.internal_project_pressure != 0, nullptr, c->red, c->stream);
 }
 
@@ -824,13 +828,13 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     D.a = dalloc<double>(c, 9 * size_t(D.a_nnz));
     D.b = dalloc<double>(c, 3 * size_t(D.b_nnz));
     D.bt = dalloc<double>(c, 3 * size_t(D.bt_nnz));
-    D.ae64 = dalloc<AEnt<double>>(c, size_t(D.a_nnz));
-    D.be64 = dalloc<BEnt<double>>(c, size_t(D.b_nnz));
-    D.bte64 = dalloc<BEnt<double>>(c, size_t(D.bt_nnz));
+    D.av64 = dalloc<double>(c, 9 * size_t(D.a_nnz));
+    D.bv64 = dalloc<double>(c, 3 * size_t(D.b_nnz));
+    D.btv64 = dalloc<double>(c, 3 * size_t(D.bt_nnz));
     if (mixed(c)) {
-      D.ae32 = dalloc<AEnt<float>>(c, size_t(D.a_nnz));
-      D.be32 = dalloc<BEnt<float>>(c, size_t(D.b_nnz));
-      D.bte32 = dalloc<BEnt<float>>(c, size_t(D.bt_nnz));
+      D.av32 = dalloc<float>(c, 9 * size_t(D.a_nnz));
+      D.bv32 = dalloc<float>(c, 3 * size_t(D.b_nnz));
+      D.btv32 = dalloc<float>(c, 3 * size_t(D.bt_nnz));
     }
     upload_values(c, D, d);
     // dof maps (UGlobalNumbers + UBeginIndex[cell])
@@ -887,26 +891,28 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     }
     {
       RowBytes rb;
-      double vb = 0;
-      for (int i = 0; i < D.n_udof; ++i) vb += rb.vel(d, i, 8);
-      double pb = 0;
-      for (int i = 0; i < D.n_pdof; ++i) pb += rb.pre(d, i, 8);
-      // defect: rows + x read + f read + d write; matvec: no f
-      D.defect_bytes = vb + pb + 3.0 * 8.0 * D.n_dof;
-      D.matvec_bytes = vb + pb + 2.0 * 8.0 * D.n_dof;
-      D.res_bytes.assign(nc, 0.0);
-      for (int k = 0; k < nc; ++k) {
-        double b = 0;
-        for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
-          const int cell = reo[q];
-          b += 4;  // cell index
-          for (int j = 0; j < NU_LOC; ++j) {
-            const int g = ug[size_t(cell) * NU_LOC + j];
-            b += rb.vel(d, g, 8) + 3 * 8 + 3 * 8 + 4 + 3;   // row, f, local residual, dof, perm
+      for (int pi = 0; pi < 2; ++pi) {
+        const int vs = pi == 0 ? 8 : 4;
+        double vb = 0;
+        for (int i = 0; i < D.n_udof; ++i) vb += rb.vel(d, i, vs);
+        double pb = 0;
+        for (int i = 0; i < D.n_pdof; ++i) pb += rb.pre(d, i, vs);
+        // defect: rows + x read + f read + d write; matvec (FP64 only): no f
+        D.defect_bytes[pi] = vb + pb + 3.0 * vs * D.n_dof;
+        if (pi == 0) D.matvec_bytes = vb + pb + 2.0 * vs * D.n_dof;
+        D.res_bytes[pi].assign(nc, 0.0);
+        for (int k = 0; k < nc; ++k) {
+          double b = 0;
+          for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
+            const int cell = reo[q];
+            b += 4 + (NU_LOC + NP_LOC) * 4.0;  // cell index, work items
+            // row, f, local residual, dof index, inverse-permutation byte
+            for (int j = 0; j < NU_LOC; ++j)
+              b += rb.vel(d, ug[size_t(cell) * NU_LOC + j], vs) + 3 * vs + 3 * vs + 4 + 3;
+            for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], vs) + vs + vs + 4 + 1;
           }
-          for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], 8) + 8 + 8 + 4 + 1;
+          D.res_bytes[pi][k] = b;
         }
-        D.res_bytes[k] = b;
       }
     }
     // vectors

@@ -27,28 +27,14 @@ constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
 
-// Operator entries interleaved per nonzero (the streaming kernels read these): column
-// index and the nine A blocks A11 A12 A13 A21 A22 A23 A31 A32 A33 of one CSR entry, or the
-// column and the three B (BT) blocks.  A row is one contiguous run of records, so a row
-// touches only whole cache lines except at its two ends.
-template <typename T>
-struct AEnt;
-template <>
-struct alignas(16) AEnt<double> { int col; int pad; double v[9]; };   // 80 B
-template <>
-struct alignas(8) AEnt<float> { int col; float v[9]; };               // 40 B
-template <typename T>
-struct BEnt;
-template <>
-struct alignas(16) BEnt<double> { int col; int pad; double v[3]; };   // 32 B
-template <>
-struct alignas(16) BEnt<float> { int col; float v[3]; };              // 16 B
-static_assert(sizeof(AEnt<double>) == 80 && sizeof(AEnt<float>) == 40, "record size");
-static_assert(sizeof(BEnt<double>) == 32 && sizeof(BEnt<float>) == 16, "record size");
-
-// Type-4 operator.  Structure-of-arrays values (FP64 only, the reference layout: A values
-// a[b*a_nnz + k] for the 9 blocks, B/BT values b[c*b_nnz + k]) feed the set-up kernels;
-// the interleaved records feed the streaming kernels.
+// Type-4 operator.  Two value layouts:
+//  * structure of arrays, FP64 only, the reference layout (A block b of entry k at
+//    a[b*a_nnz + k], B/BT block c at b[c*b_nnz + k]); read by the set-up kernels;
+//  * row-blocked, FP64 and FP32: the values of one CSR row are stored together, block by
+//    block: A row r (entries k0..k1-1, n = k1-k0) at av[9*k0 + b*n + (k-k0)], B/BT rows at
+//    bv[3*k0 + c*n + (k-k0)].  A row is one contiguous region (whole cache lines but at its
+//    two ends) and 16 lanes reading 16 consecutive entries of one block coalesce.  Read by
+//    the streaming kernels (defect, matvec, Vanka residual).
 template <typename T>
 struct OpView {
   int n_udof, n_active, n_pdof;
@@ -61,23 +47,22 @@ struct OpView {
   const int *bt_rowptr, *bt_kcol;
   const T *bt;
   long bt_nnz;
-  const AEnt<T> *ae;
-  const BEnt<T> *be, *bte;
+  const T *av, *bv, *btv;   // row-blocked values
 };
 
-// hardware dispatches workgroup b to XCD b % 8: logical block ids that make the blocks of
-// one XCD cover one contiguous range of rows (one L2 sees neighbouring rows and columns)
-__device__ inline int xcd_block() {
-  const int G = gridDim.x, b = blockIdx.x;
-  const int per = G >> 3, rem = G & 7, x = b & 7, i = b >> 3;
-  return x * per + (x < rem ? x : rem) + i;
-}
-// grid size for n_items processed in chunks by 'per_block' workers: a multiple of 8
-inline int xcd_grid(long n_items, long per_block, int cap) {
+// Streaming kernels split their n items (rows, residual work items) into one contiguous
+// chunk per workgroup, chunk g = [n*g/G, n*(g+1)/G).  The hardware spreads consecutive
+// workgroups round-robin over the 8 XCDs, so every XCD gets an even mix of the cheap and
+// expensive rows (Dirichlet, pressure and full velocity rows differ up to ~50x in bytes).
+inline int chunk_grid(long n_items, long per_block, int cap) {
   long g = (n_items + per_block - 1) / per_block;
   if (g < 8) g = 8;
   if (g > cap) g = cap;
-  return int((g + 7) / 8 * 8);
+  return int(g);
+}
+__device__ inline void chunk_of_block(long n, int &i0, int &i1) {
+  i0 = int(n * blockIdx.x / gridDim.x);
+  i1 = int(n * (blockIdx.x + 1) / gridDim.x);
 }
 
 struct Csr {
@@ -95,10 +80,8 @@ struct DevLevel {
   int *a_rowptr = nullptr, *a_kcol = nullptr, *b_rowptr = nullptr, *b_kcol = nullptr;
   int *bt_rowptr = nullptr, *bt_kcol = nullptr;
   double *a = nullptr, *b = nullptr, *bt = nullptr;
-  AEnt<double> *ae64 = nullptr;
-  BEnt<double> *be64 = nullptr, *bte64 = nullptr;
-  AEnt<float> *ae32 = nullptr;
-  BEnt<float> *be32 = nullptr, *bte32 = nullptr;
+  double *av64 = nullptr, *bv64 = nullptr, *btv64 = nullptr;   // row-blocked values
+  float *av32 = nullptr, *bv32 = nullptr, *btv32 = nullptr;
   // per color, the (slot * 32 + local row) work items of the residual kernel sorted by the
   // global row they read (velocity rows, then pressure rows)
   int *res_items = nullptr;
@@ -126,19 +109,20 @@ struct DevLevel {
   // host copies kept for value updates / coarse assembly
   std::vector<int> h_a_rowptr, h_a_kcol, h_b_rowptr, h_b_kcol, h_bt_rowptr, h_bt_kcol;
   // algorithmic bytes (FP64 operator) of the per-color residual launches and of one defect
-  std::vector<double> res_bytes;
-  double defect_bytes = 0, matvec_bytes = 0;
+  // [0]: FP64 values and vectors, [1]: FP32
+  std::vector<double> res_bytes[2];
+  double defect_bytes[2] = {0, 0}, matvec_bytes = 0;
   double transfer_bytes = 0;
 
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
                           b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz,
-                          ae64, be64, bte64};
+                          av64, bv64, btv64};
   }
   OpView<float> op32() const {
     return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, nullptr, a_nnz,
                          b_rowptr, b_kcol, nullptr, b_nnz, bt_rowptr, bt_kcol, nullptr, bt_nnz,
-                         ae32, be32, bte32};
+                         av32, bv32, btv32};
   }
 };
 
@@ -177,9 +161,9 @@ template <typename T>
 void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0,
                            const int *u_global, const int *p_global, const uint8_t *invperm,
                            const int *items, const T *u, const T *f, T *rloc, hipStream_t s);
-// interleaved operator records from the structure-of-arrays FP64 values
+// row-blocked operator values from the structure-of-arrays FP64 values
 template <typename T>
-void launch_build_records(const DevLevel &L, AEnt<T> *ae, BEnt<T> *be, BEnt<T> *bte, hipStream_t s);
+void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s);
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc,
                         const int *u_global, const int *p_global, int n_udof, T *u, T damp,
