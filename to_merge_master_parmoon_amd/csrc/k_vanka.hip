@@ -58,131 +58,198 @@ struct Ld<float, uint16_t> {
   }
 };
 
-// ------------------------------------------------------------------ patch matrix + LU
-// One workgroup per patch position (cells in color order).  The local matrix is built in
-// LDS exactly as NSE_MGLevel4.C:1066-1290 builds System[col*85+row] (a CSR entry lands in
-// the column of the first local dof with the same global number; Dirichlet rows drop the
-// off-diagonal blocks and the BT columns), then factored with dgetf2 semantics.
-__global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *__restrict__ cells,
-                                                  const int *__restrict__ u_global,
-                                                  const int *__restrict__ p_global, int ncells,
-                                                  double *__restrict__ lu, uint8_t *__restrict__ invperm,
-                                                  int *__restrict__ info) {
-  extern __shared__ double S[];  // NLOC * NLOC, column-major
-  __shared__ int ud[NU_LOC], pd[NP_LOC], sigma[NLOC];
-  __shared__ int piv_s, info_s, nz_s;
-  const int pos = blockIdx.x;
-  if (pos >= ncells) return;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int cell = cells[pos];
-  const long nnzA = op.a_nnz, nnzB = op.b_nnz, nnzBT = op.bt_nnz;
-  if (t < NU_LOC) ud[t] = u_global[long(cell) * NU_LOC + t];
-  if (t < NP_LOC) pd[t] = p_global[long(cell) * NP_LOC + t];
-  if (t < NLOC) sigma[t] = t;
-  if (t == 0) info_s = 0;
-  for (int i = t; i < NLOC * NLOC; i += 256) S[i] = 0.0;
-  __syncthreads();
-  // velocity rows j, j+27, j+54
-  for (int j = wave; j < NU_LOC; j += 4) {
-    const int UDOF = ud[j];
-    const bool dir = UDOF >= op.n_active;
-    for (int k = op.a_rowptr[UDOF] + lane; k < op.a_rowptr[UDOF + 1]; k += 64) {
-      const int l = op.a_kcol[k];
-      int m = -1;
-      for (int q = 0; q < NU_LOC; ++q)
-        if (ud[q] == l) { m = q; break; }
-      if (m < 0) continue;
-      for (int r = 0; r < 3; ++r)
-        for (int c = 0; c < 3; ++c) {
-          double v = op.a[(3 * r + c) * nnzA + k];
-          if (dir && r != c) v = 0.0;
-          S[(m + c * NU_LOC) * NLOC + j + r * NU_LOC] = v;
+// One workgroup per patch position (cells in color order).  The local matrix is the
+// System[col*85+row] of NSE_MGLevel4.C:1066-1290 (velocity rows j, j+27, j+54, pressure
+// rows 81..84; Dirichlet rows keep only the diagonal blocks and no BT columns).  Its
+// entries are gathered straight into the register tiles through the per-cell map of
+// offsets into the row-blocked operator values (built once per level from the
+// structure), then factored with dgetf2 semantics.
+__device__ __forceinline__ double patch_entry(const OpView<double> &op, const int *__restrict__ mp, int r, int c) {
+  int mi = -1, li = 0, blk = 0;
+  const double *base = op.av;
+  bool dir_off = false;
+  if (r < 3 * NU_LOC) {
+    const int rr = r / NU_LOC, j = r - NU_LOC * rr;
+    if (c < 3 * NU_LOC) {
+      const int cc = c / NU_LOC, m = c - NU_LOC * cc;
+      mi = PM_A + j * NU_LOC + m;
+      li = PM_NA + j;
+      blk = 3 * rr + cc;
+      dir_off = rr != cc;
+    } else if (c < NLOC) {
+      mi = PM_BT + j * NP_LOC + (c - 3 * NU_LOC);
+      li = PM_MT + j;
+      blk = rr;
+      base = op.btv;
+    }
+  } else if (r < NLOC && c < 3 * NU_LOC) {
+    const int pj = r - 3 * NU_LOC, cc = c / NU_LOC, m = c - NU_LOC * cc;
+    mi = PM_B + pj * NU_LOC + m;
+    li = PM_NB + pj;
+    blk = cc;
+    base = op.bv;
+  }
+  const int off = mi >= 0 ? mp[mi] : -1;
+  int len = mp[li];
+  bool valid = off >= 0;
+  if (len < 0) {  // Dirichlet velocity row: A blocks r != c are dropped
+    len = -len;
+    valid = valid && !dir_off;
+  }
+  const double v = base[valid ? off + blk * len : 0];
+  return valid ? v : 0.0;
+}
+
+// one block of 16 columns (jb = JB) of the register-tiled dgetf2 in k_patch_lu
+template <int JB>
+__device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int ty, int *sigma, double *lbuf,
+                                           double *ubuf, int &piv_s, int &nz_s, int &info_s) {
+  constexpr int jb = JB;
+  const int jl_end = jb == 5 ? NLOC - 80 : 16;
+  for (int jl = 0; jl < jl_end; ++jl) {
+    const int j = 16 * jb + jl;
+    // pivot: first row of max |a(i, j)|, i >= j
+    if (tx == jl) {
+      double bv = -1.0;
+      int bi = NLOC;
+#pragma unroll
+      for (int a = jb; a < 6; ++a) {
+        const int r = ty + 16 * a;
+        const double v = fabs(A[a][jb]);
+        if (r >= j && r < NLOC && v > bv) {
+          bv = v;
+          bi = r;
         }
-    }
-    if (!dir) {
-      for (int k = op.bt_rowptr[UDOF] + lane; k < op.bt_rowptr[UDOF + 1]; k += 64) {
-        const int l = op.bt_kcol[k];
-        int m = -1;
-        for (int q = 0; q < NP_LOC; ++q)
-          if (pd[q] == l) { m = q; break; }
-        if (m < 0) continue;
-        for (int r = 0; r < 3; ++r) S[(3 * NU_LOC + m) * NLOC + j + r * NU_LOC] = op.bt[r * nnzBT + k];
       }
-    }
-  }
-  // pressure rows 81+j
-  for (int j = wave; j < NP_LOC; j += 4) {
-    const int PDOF = pd[j];
-    for (int k = op.b_rowptr[PDOF] + lane; k < op.b_rowptr[PDOF + 1]; k += 64) {
-      const int l = op.b_kcol[k];
-      int m = -1;
-      for (int q = 0; q < NU_LOC; ++q)
-        if (ud[q] == l) { m = q; break; }
-      if (m < 0) continue;
-      for (int c = 0; c < 3; ++c) S[(m + c * NU_LOC) * NLOC + 3 * NU_LOC + j] = op.b[c * nnzB + k];
-    }
-  }
-  __syncthreads();
-  // dgetf2
-  for (int j = 0; j < NLOC; ++j) {
-    double *colj = S + j * NLOC;
-    if (wave == 0) {
-      const int i0 = j + lane, i1 = j + lane + 64;
-      double v0 = i0 < NLOC ? fabs(colj[i0]) : -1.0;
-      double v1 = i1 < NLOC ? fabs(colj[i1]) : -1.0;
-      double bv = v0;
-      int bi = i0;
-      if (v1 > bv) { bv = v1; bi = i1; }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 16);
+        const int oi = __shfl_xor(bi, o, 16);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
       }
-      if (lane == 0) {
+      if (ty == 0) {
         piv_s = bi;
         nz_s = bv != 0.0;
+        if (bv == 0.0 && info_s == 0) info_s = j + 1;
       }
     }
     __syncthreads();
     const int p = piv_s;
     const bool nonzero = nz_s != 0;
-    if (nonzero) {
-      if (p != j) {
-        if (t < NLOC) {
-          const double a = S[t * NLOC + j];
-          S[t * NLOC + j] = S[t * NLOC + p];
-          S[t * NLOC + p] = a;
-        }
-        if (t == 0) {
-          const int s0 = sigma[j];
-          sigma[j] = sigma[p];
-          sigma[p] = s0;
-        }
-        __syncthreads();
+    if (nonzero && p != j) {
+      // swap rows j and p over all columns.  p >= j, so row p sits in slot pb >= jb of the
+      // lanes ty == p % 16.  Each slot is handled in its own branch (pb is uniform) so
+      // that every register access stays static.
+      const int pl = p & 15, pb = p >> 4;
+      if (t == 0) {
+        const int s0 = sigma[j];
+        sigma[j] = sigma[p];
+        sigma[p] = s0;
       }
-      const double r = 1.0 / colj[j];
-      if (t > j && t < NLOC) colj[t] *= r;
-    } else if (t == 0 && info_s == 0) {
-      info_s = j + 1;
+      const int src = ty == jl ? pl : (ty == pl ? jl : ty);
+#pragma unroll
+      for (int a = jb; a < 6; ++a) {
+        if (a == pb) {
+          if (pl != jl) {
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+              const double send = ty == jl ? A[jb][b] : A[a][b];
+              const double recv = __shfl(send, src, 16);
+              if (ty == jl) A[jb][b] = recv;
+              if (ty == pl) A[a][b] = recv;
+            }
+          } else if (ty == jl) {
+#pragma unroll
+            for (int b = 0; b < 6; ++b) {
+              const double rowj = A[jb][b];
+              A[jb][b] = A[a][b];
+              A[a][b] = rowj;
+            }
+          }
+        }
+      }
+    }
+    // scale column j below the diagonal by the reciprocal of the pivot, publish l and u
+    if (tx == jl) {
+      if (nonzero) {
+        const double rc = 1.0 / __shfl(A[jb][jb], jl, 16);
+#pragma unroll
+        for (int a = jb; a < 6; ++a)
+          if (ty + 16 * a > j) A[a][jb] *= rc;
+      }
+#pragma unroll
+      for (int a = jb; a < 6; ++a) lbuf[ty + 16 * a] = A[a][jb];
+    }
+    if (ty == jl) {
+#pragma unroll
+      for (int b = jb; b < 6; ++b) ubuf[tx + 16 * b] = A[jb][b];
     }
     __syncthreads();
-    const int n = NLOC - 1 - j;
-    for (int idx = t; idx < n * n; idx += 256) {
-      const int c = j + 1 + idx / n, i = j + 1 + idx % n;
-      S[c * NLOC + i] = fma(-colj[i], S[c * NLOC + j], S[c * NLOC + i]);
-    }
-    __syncthreads();
+    double lv[6], uv[6];
+#pragma unroll
+    for (int a = jb; a < 6; ++a) lv[a] = lbuf[ty + 16 * a];
+#pragma unroll
+    for (int b = jb; b < 6; ++b) uv[b] = ubuf[tx + 16 * b];
+#pragma unroll
+    for (int a = jb; a < 6; ++a)
+#pragma unroll
+      for (int b = jb; b < 6; ++b)
+        if (ty + 16 * a > j && tx + 16 * b > j) A[a][b] = fma(-lv[a], uv[b], A[a][b]);
   }
-  // packed store: strictly lower part column by column at [0, LU_L), upper part (with the
-  // diagonal) column by column at [LU_UOFF, LU_UOFF + LU_U)
+}
+
+// ------------------------------------------------------------------ patch matrix + LU
+// dgetf2 on a register-tiled copy: thread (tx, ty) = (t / 16, t % 16) holds rows ty + 16a
+// and columns tx + 16b (a, b < 6; indices >= 85 are zero padding).  Every element receives
+// the same operation sequence as in the column-sweep dgetf2 (swap, reciprocal scaling,
+// fma(-l, u, a) in step order), so the factors are bit-identical.  Column j lives in the
+// 16 lanes tx = j % 16 of one wave (pivot search and scaling by shuffles), row j in the
+// lanes ty = j % 16 of every 16-lane group (row swaps by shuffles); l and u are published
+// through LDS, two barriers per column.
+__global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *__restrict__ cells,
+                                                  const int *__restrict__ pmap, int ncells,
+                                                  double *__restrict__ lu, uint8_t *__restrict__ invperm,
+                                                  int *__restrict__ info) {
+  __shared__ int sigma[NLOC];
+  __shared__ double lbuf[96], ubuf[96];
+  __shared__ int piv_s, info_s, nz_s;
+  const int pos = blockIdx.x;
+  if (pos >= ncells) return;
+  const int t = threadIdx.x;
+  const int tx = t >> 4, ty = t & 15;
+  const int *__restrict__ mp = pmap + long(cells[pos]) * PM_STRIDE;
+  if (t < NLOC) sigma[t] = t;
+  if (t == 0) info_s = 0;
+  double A[6][6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) A[a][b] = patch_entry(op, mp, ty + 16 * a, tx + 16 * b);
+  __syncthreads();
+  lu_columns<0>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  lu_columns<1>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  lu_columns<2>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  lu_columns<3>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  lu_columns<4>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  lu_columns<5>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  // packed store straight from the tiles: strictly lower part column by column at
+  // [0, LU_L), upper part (with the diagonal) column by column at [LU_UOFF, LU_UOFF + LU_U)
   double *out = lu + long(pos) * LU_STRIDE;
-  for (int i = t; i < NLOC * NLOC; i += 256) {
-    const int c = i / NLOC, r = i - c * NLOC;
-    if (r > c)
-      out[lu_loff(c) + r - c - 1] = S[i];
-    else
-      out[LU_UOFF + lu_uoff(c) + r] = S[i];
-  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const int r = ty + 16 * a, c = tx + 16 * b;
+      if (r < NLOC && c < NLOC) {
+        if (r > c)
+          out[lu_loff(c) + r - c - 1] = A[a][b];
+        else
+          out[LU_UOFF + lu_uoff(c) + r] = A[a][b];
+      }
+    }
   if (t < NLOC) invperm[long(pos) * PERM_STRIDE + sigma[t]] = uint8_t(t);
   if (t == 0) info[pos] = info_s;
 }
@@ -383,9 +450,7 @@ __global__ void k_lu_bf16(const double *__restrict__ a, uint16_t *__restrict__ b
 
 void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s) {
   if (L.n_cells <= 0) return;
-  const size_t shm = size_t(NLOC) * NLOC * sizeof(double);
-  k_patch_lu<<<L.n_cells, 256, shm, s>>>(L.op64(), L.cell_reorder, L.u_global, L.p_global, L.n_cells, lu,
-                                         L.invperm, info);
+  k_patch_lu<<<L.n_cells, 256, 0, s>>>(L.op64(), L.cell_reorder, L.pmap, L.n_cells, lu, L.invperm, info);
   PMG_HIP_CHECK(hipGetLastError());
 }
 

@@ -192,6 +192,51 @@ void build_transfer(pmg_ctx *c, int nloc_f, int nloc_c, int n_fine_cells, const 
 
 namespace {
 
+// Per-cell gather map of the local Vanka matrix (layout: PM_* constants).  The entry of
+// local velocity row j and local column m is the CSR entry of row UDOF_j whose column is
+// UDOF_m (NSE_MGLevel4.C:1149-1287 finds it by linear search; the map stores where it is
+// in the row-blocked values).  Dirichlet rows keep only their diagonal entry.
+std::vector<int> build_patch_map(const pmg_level_desc *d, const std::vector<int> &ug, const std::vector<int> &pg) {
+  const int nc = d->n_cells;
+  std::vector<int> map(size_t(nc) * PM_STRIDE, -1);
+  // global dof -> local index in the current cell (marks reset per cell)
+  std::vector<int> uloc(d->n_udof, -1), ploc(d->n_pdof, -1);
+  for (int cell = 0; cell < nc; ++cell) {
+    int *mp = map.data() + size_t(cell) * PM_STRIDE;
+    const int *U = ug.data() + size_t(cell) * NU_LOC;
+    const int *P = pg.data() + size_t(cell) * NP_LOC;
+    for (int j = NU_LOC - 1; j >= 0; --j) uloc[U[j]] = j;  // first local dof wins
+    for (int j = NP_LOC - 1; j >= 0; --j) ploc[P[j]] = j;
+    for (int j = 0; j < NU_LOC; ++j) {
+      const int row = U[j], k0 = d->a_rowptr[row], k1 = d->a_rowptr[row + 1];
+      const bool dir = row >= d->n_active;
+      mp[PM_NA + j] = dir ? -(k1 - k0) : (k1 - k0);
+      for (int k = k0; k < k1; ++k) {
+        const int m = uloc[d->a_kcol[k]];
+        if (m >= 0 && mp[PM_A + j * NU_LOC + m] < 0) mp[PM_A + j * NU_LOC + m] = 9 * k0 + (k - k0);
+      }
+      const int q0 = d->bt_rowptr[row], q1 = d->bt_rowptr[row + 1];
+      mp[PM_MT + j] = q1 - q0;
+      if (!dir)
+        for (int q = q0; q < q1; ++q) {
+          const int m = ploc[d->bt_kcol[q]];
+          if (m >= 0 && mp[PM_BT + j * NP_LOC + m] < 0) mp[PM_BT + j * NP_LOC + m] = 3 * q0 + (q - q0);
+        }
+    }
+    for (int j = 0; j < NP_LOC; ++j) {
+      const int row = P[j], k0 = d->b_rowptr[row], k1 = d->b_rowptr[row + 1];
+      mp[PM_NB + j] = k1 - k0;
+      for (int k = k0; k < k1; ++k) {
+        const int m = uloc[d->b_kcol[k]];
+        if (m >= 0 && mp[PM_B + j * NU_LOC + m] < 0) mp[PM_B + j * NU_LOC + m] = 3 * k0 + (k - k0);
+      }
+    }
+    for (int j = 0; j < NU_LOC; ++j) uloc[U[j]] = -1;
+    for (int j = 0; j < NP_LOC; ++j) ploc[P[j]] = -1;
+  }
+  return map;
+}
+
 void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
   for (int b = 0; b < 9; ++b) h2d(c, D.a + size_t(b) * D.a_nnz, d->a[b], D.a_nnz);
   for (int b = 0; b < 3; ++b) {
@@ -931,6 +976,11 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
       if (c->prm.precision == PMG_PREC_FP32) D.lu32 = dalloc<float>(c, lun);
       if (c->prm.precision == PMG_PREC_FP32_BF16) D.lu16 = dalloc<uint16_t>(c, lun);
       D.invperm = dalloc<uint8_t>(c, size_t(D.n_cells) * PERM_STRIDE);
+      {
+        const std::vector<int> pm = build_patch_map(d, ug, pg);
+        D.pmap = dalloc<int>(c, pm.size());
+        h2d(c, D.pmap, pm.data(), pm.size());
+      }
       D.patch_info = dalloc<int>(c, D.n_cells);
       D.rloc = dalloc<double>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
       if (mixed(c)) D.rloc32 = dalloc<float>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);

@@ -26,6 +26,18 @@ static_assert(LU_UOFF + LU_U <= LU_STRIDE, "packed factors exceed the patch stri
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
+// per-cell gather map of the patch matrix (ints): offsets of the A entries (j, m) into the
+// row-blocked A values (block 0; -1: no entry), row lengths of the 27 velocity rows
+// (negative: Dirichlet row), BT offsets (j, pressure m), BT row lengths, B offsets
+// (pressure j, m), B row lengths
+constexpr int PM_A = 0;
+constexpr int PM_NA = PM_A + NU_LOC * NU_LOC;    // 729
+constexpr int PM_BT = PM_NA + NU_LOC;            // 756
+constexpr int PM_MT = PM_BT + NU_LOC * NP_LOC;   // 864
+constexpr int PM_B = PM_MT + NU_LOC;             // 891
+constexpr int PM_NB = PM_B + NP_LOC * NU_LOC;    // 999
+constexpr int PM_STRIDE = 1008;
+static_assert(PM_NB + NP_LOC <= PM_STRIDE, "patch map stride");
 
 // Type-4 operator.  Two value layouts:
 //  * structure of arrays, FP64 only, the reference layout (A block b of entry k at
@@ -97,6 +109,7 @@ struct DevLevel {
   float *lu32 = nullptr;
   uint16_t *lu16 = nullptr;          // bf16 factors
   uint8_t *invperm = nullptr;        // n_cells * PERM_STRIDE
+  int *pmap = nullptr;               // n_cells * PM_STRIDE patch gather map
   int *patch_info = nullptr;         // n_cells
   bool factors_ready = false;
   // transfers to the next coarser level (this level = fine)
