@@ -18,7 +18,6 @@ namespace pmg {
 
 namespace {
 
-constexpr int NB = 32;     // max LU panel width (trsm / gemm tiles)
 constexpr int NBR = 16;    // panel width of the register-resident panel factorization
 constexpr int PNT = 512;   // threads of the register-resident panel factorization
 constexpr int TB = 64;     // triangular-solve block (one wavefront)
@@ -35,7 +34,8 @@ __device__ inline long cvel(int b, int r, int na) { return long(b) * na + r; }
 __device__ inline long cpre(int q, int na, int pin) { return 3L * na + q - pin; }
 
 // :5739-5833 restricted to the kept rows and columns: A blocks of active rows/columns,
-// BT columns except the pinned one, B rows except the pinned one
+// BT columns except the pinned one, B rows except the pinned one.  Row-major a[row*n+col]
+// while factoring (row interchanges and panel rows are contiguous).
 __global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long n, int pin) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
@@ -44,12 +44,12 @@ __global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long
       const int c = op.a_kcol[k];
       if (c >= NA) continue;
       for (int br = 0; br < 3; ++br)
-        for (int bc = 0; bc < 3; ++bc) a[cvel(bc, c, NA) * n + cvel(br, r, NA)] = op.a[(3 * br + bc) * op.a_nnz + k];
+        for (int bc = 0; bc < 3; ++bc) a[cvel(br, r, NA) * n + cvel(bc, c, NA)] = op.a[(3 * br + bc) * op.a_nnz + k];
     }
     for (int k = op.bt_rowptr[r]; k < op.bt_rowptr[r + 1]; ++k) {
       const int q = op.bt_kcol[k];
       if (pin && q == 0) continue;
-      for (int b = 0; b < 3; ++b) a[cpre(q, NA, pin) * n + cvel(b, r, NA)] = op.bt[b * op.bt_nnz + k];
+      for (int b = 0; b < 3; ++b) a[cvel(b, r, NA) * n + cpre(q, NA, pin)] = op.bt[b * op.bt_nnz + k];
     }
   } else if (r >= N && r < N + NP) {
     const int i = r - N;
@@ -57,7 +57,7 @@ __global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long
     for (int k = op.b_rowptr[i]; k < op.b_rowptr[i + 1]; ++k) {
       const int c = op.b_kcol[k];
       if (c >= NA) continue;
-      for (int b = 0; b < 3; ++b) a[cvel(b, c, NA) * n + cpre(i, NA, pin)] = op.b[b * op.b_nnz + k];
+      for (int b = 0; b < 3; ++b) a[cpre(i, NA, pin) * n + cvel(b, c, NA)] = op.b[b * op.b_nnz + k];
     }
   }
 }
@@ -66,158 +66,120 @@ __global__ void k_fill_d(double *x, double v, long n) {
   for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) x[i] = v;
 }
 
-// ------------------------------------------------------------------ blocked LU
-// panel [j0, j1): dgetf2 restricted to the panel columns (single workgroup)
-__global__ __launch_bounds__(1024) void k_lu_panel(double *__restrict__ a, int n, int j0, int j1,
-                                                   int *__restrict__ ipiv, int *__restrict__ info) {
-  __shared__ double sv[16];
-  __shared__ int si[16];
-  __shared__ int piv_s, nz_s;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int j = j0; j < j1; ++j) {
-    double *colj = a + long(j) * n;
-    // first index of max |a(i,j)|, i >= j: per thread in increasing row order, then
-    // (value, -index) maxima across lanes and waves
-    double bv = -1.0;
-    int bi = n;
-    for (int i = j + t; i < n; i += 1024) {
-      const double v = fabs(colj[i]);
-      if (v > bv) { bv = v; bi = i; }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(bv, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { sv[wave] = bv; si[wave] = bi; }
-    __syncthreads();
-    if (wave == 0) {
-      bv = lane < 16 ? sv[lane] : -1.0;
-      bi = lane < 16 ? si[lane] : n;
-      for (int o = 8; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-      }
-      if (lane == 0) { piv_s = bi; nz_s = bv != 0.0; }
-    }
-    __syncthreads();
-    const int p = piv_s;
-    const bool nz = nz_s != 0;
-    if (t == 0) ipiv[j] = p;
-    if (nz) {
-      if (p != j && t < j1 - j0) {
-        const long c = j0 + t;
-        const double x = a[c * n + j];
-        a[c * n + j] = a[c * n + p];
-        a[c * n + p] = x;
-      }
-      __syncthreads();
-      const double r = 1.0 / colj[j];
-      for (int i = j + 1 + t; i < n; i += 1024) colj[i] *= r;
-    } else if (t == 0 && info[0] == 0) {
-      info[0] = j + 1;
-    }
-    __syncthreads();
-    for (int c = j + 1; c < j1; ++c) {
-      double *colc = a + long(c) * n;
-      const double u = colc[j];
-      for (int i = j + 1 + t; i < n; i += 1024) colc[i] = fma(-colj[i], u, colc[i]);
-    }
-    __syncthreads();
-  }
-}
-// Same dgetf2 steps with the panel held in registers: thread t owns panel rows
-// j0 + t + 1024 q (q < PR), so the column loop touches global memory only to load and
-// store the panel.  The pivot row travels through LDS (it is also the U row of the step).
+// Panel factorization of columns [j0, j1) (width <= NBR) with dgetf2 steps, the panel held
+// in registers (thread t owns rows j0 + t + PNT q, q < PR); two barriers per column (pivot
+// reduction, pivot row exchange), LDS double-buffered by column parity.  Then the panel's
+// row interchanges are applied to every other column in one pass (the <= 2*NBR rows they
+// involve are permuted at once, as dlaswp's sequential swaps would leave them), and the
+// U block row of the previous step (rows pj0..pj1-1, kept in ubuf by k_lu_update) is
+// copied into place.  Row-major matrix a[row*n + col].
 template <int PR>
-__global__ __launch_bounds__(PNT) void k_lu_panel_reg(double *__restrict__ a, int n, int j0, int j1,
-                                                       int *__restrict__ ipiv, int *__restrict__ info) {
-  __shared__ double bufP[NBR], bufJ[NBR];
-  __shared__ double sv[16];
-  __shared__ int si[16];
-  __shared__ int piv_s, nz_s;
+__global__ __launch_bounds__(PNT) void k_lu_panel(double *__restrict__ a, int n, int j0, int j1,
+                                                  int *__restrict__ ipiv, int *__restrict__ info,
+                                                  const double *__restrict__ ubuf, int pj0, int pj1) {
+  __shared__ double sv[2][PNT / 64];
+  __shared__ int si[2][PNT / 64];
+  __shared__ double bufP[2][NBR], bufJ[2][NBR];
+  __shared__ int pivs[NBR];
+  __shared__ int rows_s[2 * NBR], src_s[2 * NBR];
+  __shared__ int nk_s;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int w = j1 - j0;
+  if (pj1 > pj0) {
+    const int pw = pj1 - pj0;
+    for (int kk = 0; kk < pw; ++kk)
+      for (int c = pj1 + t; c < n; c += PNT) a[long(pj0 + kk) * n + c] = ubuf[long(kk) * n + c];
+  }
   double v[PR][NBR];
 #pragma unroll
   for (int q = 0; q < PR; ++q) {
     const int r = j0 + t + PNT * q;
 #pragma unroll
-    for (int cc = 0; cc < NBR; ++cc) v[q][cc] = (r < n && cc < w) ? a[long(j0 + cc) * n + r] : 0.0;
+    for (int cc = 0; cc < NBR; ++cc) v[q][cc] = (r < n && cc < w) ? a[long(r) * n + j0 + cc] : 0.0;
   }
 #pragma unroll
   for (int jj = 0; jj < NBR; ++jj) {
-    const int j = j0 + jj;
     if (jj < w) {
-    double bv = -1.0;
-    int bi = n;
+      const int j = j0 + jj, par = jj & 1;
+      double bv = -1.0;
+      int bi = n;
 #pragma unroll
-    for (int q = 0; q < PR; ++q) {
-      const int r = j0 + t + PNT * q;
-      if (r >= j && r < n) {
-        const double x = fabs(v[q][jj]);
-        if (x > bv) { bv = x; bi = r; }
+      for (int q = 0; q < PR; ++q) {
+        const int r = j0 + t + PNT * q;
+        if (r >= j && r < n) {
+          const double x = fabs(v[q][jj]);
+          if (x > bv) {
+            bv = x;
+            bi = r;
+          }
+        }
       }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(bv, o, 64);
-      const int oi = __shfl_xor(bi, o, 64);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { sv[wave] = bv; si[wave] = bi; }
-    __syncthreads();
-    if (wave == 0) {
-      bv = lane < PNT / 64 ? sv[lane] : -1.0;
-      bi = lane < PNT / 64 ? si[lane] : n;
-      for (int o = 4; o > 0; o >>= 1) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
         const double ov = __shfl_xor(bv, o, 64);
         const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
       }
-      if (lane == 0) { piv_s = bi; nz_s = bv != 0.0; }
-    }
-    __syncthreads();
-    const int p = piv_s;
-    if (t == 0) ipiv[j] = p;
-    if (nz_s) {
-      const int tp = (p - j0) % PNT, qp = (p - j0) / PNT;
-      // pivot row -> bufP, row j -> bufJ (row j is owned by thread jj, q = 0)
-#pragma unroll
-      for (int q = 0; q < PR; ++q)
-        if (t == tp && q == qp)
-#pragma unroll
-          for (int cc = 0; cc < NBR; ++cc) bufP[cc] = v[q][cc];
-      if (t == jj)
-#pragma unroll
-        for (int cc = 0; cc < NBR; ++cc) bufJ[cc] = v[0][cc];
+      if (lane == 0) {
+        sv[par][wave] = bv;
+        si[par][wave] = bi;
+      }
       __syncthreads();
-      if (p != j) {
-        if (t == jj)
+      bv = sv[par][0];
+      bi = si[par][0];
 #pragma unroll
-          for (int cc = 0; cc < NBR; ++cc) v[0][cc] = bufP[cc];
+      for (int w2 = 1; w2 < PNT / 64; ++w2) {
+        const double ov = sv[par][w2];
+        const int oi = si[par][w2];
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      const int p = bi;
+      const bool nz = bv != 0.0;
+      if (t == 0) {
+        ipiv[j] = p;
+        pivs[jj] = p;
+        if (!nz && info[0] == 0) info[0] = j + 1;
+      }
+      if (nz) {
+        const int tp = (p - j0) % PNT, qp = (p - j0) / PNT;
 #pragma unroll
         for (int q = 0; q < PR; ++q)
           if (t == tp && q == qp)
 #pragma unroll
-            for (int cc = 0; cc < NBR; ++cc) v[q][cc] = bufJ[cc];
-      }
-      const double rinv = 1.0 / bufP[jj];
+            for (int cc = 0; cc < NBR; ++cc) bufP[par][cc] = v[q][cc];
+        if (t == jj)
 #pragma unroll
-      for (int q = 0; q < PR; ++q) {
-        const int r = j0 + t + PNT * q;
-        if (r > j && r < n) {
-          const double l = v[q][jj] * rinv;
-          v[q][jj] = l;
+          for (int cc = 0; cc < NBR; ++cc) bufJ[par][cc] = v[0][cc];
+        __syncthreads();
+        if (p != j) {
+          if (t == jj)
 #pragma unroll
-          for (int cc = 0; cc < NBR; ++cc)
-            if (cc > jj) v[q][cc] = fma(-l, bufP[cc], v[q][cc]);
+            for (int cc = 0; cc < NBR; ++cc) v[0][cc] = bufP[par][cc];
+#pragma unroll
+          for (int q = 0; q < PR; ++q)
+            if (t == tp && q == qp)
+#pragma unroll
+              for (int cc = 0; cc < NBR; ++cc) v[q][cc] = bufJ[par][cc];
+        }
+        const double rinv = 1.0 / bufP[par][jj];
+#pragma unroll
+        for (int q = 0; q < PR; ++q) {
+          const int r = j0 + t + PNT * q;
+          if (r > j && r < n) {
+            const double l = v[q][jj] * rinv;
+            v[q][jj] = l;
+#pragma unroll
+            for (int cc = 0; cc < NBR; ++cc)
+              if (cc > jj) v[q][cc] = fma(-l, bufP[par][cc], v[q][cc]);
+          }
         }
       }
-    } else if (t == 0 && info[0] == 0) {
-      info[0] = j + 1;
-    }
-    __syncthreads();
     }
   }
 #pragma unroll
@@ -226,64 +188,95 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_reg(double *__restrict__ a, in
     if (r < n)
 #pragma unroll
       for (int cc = 0; cc < NBR; ++cc)
-        if (cc < w) a[long(j0 + cc) * n + r] = v[q][cc];
+        if (cc < w) a[long(r) * n + j0 + cc] = v[q][cc];
+  }
+  __syncthreads();
+  if (wave == 0) {
+    // rows involved, one slot per lane: j0..j1-1 (slots 0..w-1), then the pivot rows below
+    // the panel in order of first use; src = row whose original value ends in the slot's
+    // row after the sequential interchanges
+    int row = lane < w ? j0 + lane : -1;
+    int nk = w;
+    for (int k = 0; k < w; ++k) {
+      const int p = pivs[k];
+      if (p >= j1 && __ballot(row == p) == 0) {
+        if (lane == nk) row = p;
+        ++nk;
+      }
+    }
+    int src = row;
+    for (int k = 0; k < w; ++k) {
+      const int p = pivs[k];
+      if (p == j0 + k) continue;
+      const unsigned long long m = __ballot(row == p);
+      const int f = __ffsll((long long)m) - 1;
+      const int sk = __shfl(src, k, 64), sf = __shfl(src, f, 64);
+      if (lane == k) src = sf;
+      if (lane == f) src = sk;
+    }
+    if (lane < 2 * NBR) {
+      rows_s[lane] = row;
+      src_s[lane] = src;
+    }
+    if (lane == 0) nk_s = nk;
+  }
+  __syncthreads();
+  const int nk = nk_s;
+  for (int c = t; c < n - w; c += PNT) {
+    const int col = c < j0 ? c : c + w;
+    double *cp = a + col;
+    double x[2 * NBR];
+#pragma unroll
+    for (int k = 0; k < 2 * NBR; ++k) x[k] = k < nk ? cp[long(src_s[k]) * n] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 2 * NBR; ++k)
+      if (k < nk && src_s[k] != rows_s[k]) cp[long(rows_s[k]) * n] = x[k];
   }
 }
 
-// apply the panel's row interchanges to the columns outside [j0, j1)
-__global__ void k_lu_swap(double *__restrict__ a, int n, int j0, int j1, const int *__restrict__ ipiv) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nout = n - (j1 - j0);
-  if (c >= nout) return;
-  if (c >= j0) c += j1 - j0;
-  double *col = a + long(c) * n;
-  for (int j = j0; j < j1; ++j) {
-    const int p = ipiv[j];
-    if (p != j) {
-      const double x = col[j];
-      col[j] = col[p];
-      col[p] = x;
-    }
-  }
-}
-// U block row: rows [j0, j1) of the columns >= j1, unit-lower forward sweep with L11
-__global__ __launch_bounds__(256) void k_lu_trsm(double *__restrict__ a, int n, int j0, int j1) {
-  __shared__ double L[NB * NB];
-  const int w = j1 - j0;
-  for (int idx = threadIdx.x; idx < w * w; idx += 256) {
-    const int cc = idx / w, rr = idx % w;
-    L[cc * NB + rr] = a[long(j0 + cc) * n + j0 + rr];
-  }
-  __syncthreads();
-  const int c = j1 + blockIdx.x * 256 + threadIdx.x;
-  if (c >= n) return;
-  double *col = a + long(c) * n + j0;
-  double v[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) v[i] = i < w ? col[i] : 0.0;
-#pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    if (k >= w) break;
-    const double uk = v[k];
-#pragma unroll
-    for (int i = k + 1; i < NB; ++i)
-      if (i < w) v[i] = fma(-L[k * NB + i], uk, v[i]);
-  }
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < w) col[i] = v[i];
-}
-// trailing update a[i][c] -= sum_k l[i][k] u[k][c], per-entry fma chain over k ascending
-__global__ __launch_bounds__(256) void k_lu_gemm(double *__restrict__ a, int n, int j0, int j1) {
-  __shared__ double Ls[NB][64 + 1];
-  __shared__ double Us[NB][64 + 1];
+// Trailing update of one 64x64 tile of A22 = rows/columns >= j1: the block first solves its
+// 16x64 tile of the U block row, U12 = L11^-1 A12 (unit lower, per-column fma sweep as
+// dtrsm), then A22 -= L21 U12 as a per-entry fma chain over the panel in order.  A12 itself
+// is not overwritten here (other blocks still read it); the blocks of the first row tile
+// keep U12 in ubuf and the next panel kernel copies it into place.
+__global__ __launch_bounds__(256) void k_lu_update(double *__restrict__ a, double *__restrict__ ubuf, int n,
+                                                   int j0, int j1) {
+  __shared__ double L11[NBR][NBR + 1];
+  __shared__ double Us[NBR][64 + 1];
+  __shared__ double Ls[NBR][64 + 1];
   const int w = j1 - j0;
   const int i0 = j1 + blockIdx.y * 64, c0 = j1 + blockIdx.x * 64;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  for (int idx = threadIdx.x; idx < NB * 64; idx += 256) {
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  {
+    const int i = t / NBR, k = t % NBR;  // 256 = NBR * NBR; L11[k][i] = l(j0+i, j0+k)
+    L11[k][i] = (k < w && i < w) ? a[long(j0 + i) * n + j0 + k] : 0.0;
+  }
+  for (int idx = t; idx < NBR * 64; idx += 256) {
     const int k = idx / 64, r = idx % 64;
-    Ls[k][r] = (k < w && i0 + r < n) ? a[long(j0 + k) * n + i0 + r] : 0.0;
-    Us[k][r] = (k < w && c0 + r < n) ? a[long(c0 + r) * n + j0 + k] : 0.0;
+    Us[k][r] = (k < w && c0 + r < n) ? a[long(j0 + k) * n + c0 + r] : 0.0;
+    const int r2 = idx / NBR, k2 = idx % NBR;
+    Ls[k2][r2] = (k2 < w && i0 + r2 < n) ? a[long(i0 + r2) * n + j0 + k2] : 0.0;
+  }
+  __syncthreads();
+  if (t < 64) {
+    double v[NBR];
+#pragma unroll
+    for (int i = 0; i < NBR; ++i) v[i] = Us[i][t];
+#pragma unroll
+    for (int k = 0; k < NBR; ++k) {
+      if (k < w) {
+        const double uk = v[k];
+#pragma unroll
+        for (int i = k + 1; i < NBR; ++i)
+          if (i < w) v[i] = fma(-L11[k][i], uk, v[i]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NBR; ++i) Us[i][t] = v[i];
+    if (blockIdx.y == 0 && c0 + t < n)
+#pragma unroll
+      for (int i = 0; i < NBR; ++i)
+        if (i < w) ubuf[long(i) * n + c0 + t] = v[i];
   }
   __syncthreads();
   double acc[4][4];
@@ -292,7 +285,7 @@ __global__ __launch_bounds__(256) void k_lu_gemm(double *__restrict__ a, int n, 
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
       const int i = i0 + ty * 4 + ii, c = c0 + tx * 4 + cc;
-      acc[ii][cc] = (i < n && c < n) ? a[long(c) * n + i] : 0.0;
+      acc[ii][cc] = (i < n && c < n) ? a[long(i) * n + c] : 0.0;
     }
   for (int k = 0; k < w; ++k) {
     double l[4], u[4];
@@ -310,8 +303,24 @@ __global__ __launch_bounds__(256) void k_lu_gemm(double *__restrict__ a, int n, 
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
       const int i = i0 + ty * 4 + ii, c = c0 + tx * 4 + cc;
-      if (i < n && c < n) a[long(c) * n + i] = acc[ii][cc];
+      if (i < n && c < n) a[long(i) * n + c] = acc[ii][cc];
     }
+}
+
+// the row-major factors into the column-major copy read by the triangular sweeps
+__global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ a, double *__restrict__ f, int n) {
+  __shared__ double tile[64][65];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < 64 * 64; idx += 256) {
+    const int r = idx / 64, c = idx % 64;
+    if (r0 + r < n && c0 + c < n) tile[r][c] = a[long(r0 + r) * n + c0 + c];
+  }
+  __syncthreads();
+  for (int idx = t; idx < 64 * 64; idx += 256) {
+    const int c = idx / 64, r = idx % 64;
+    if (r0 + r < n && c0 + c < n) f[long(c0 + c) * n + r0 + r] = tile[r][c];
+  }
 }
 // b'[i] = b[sigma[i]] of the sequential interchanges (permutation kept in LDS)
 __global__ void k_sigma(const int *ipiv, int *sigma, int n) {
@@ -472,33 +481,37 @@ void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s) {
 
 void coarse_lu_factor(Coarse &C, hipStream_t s) {
   const int n = C.n;
+  const int pr_max = 6;
+  if ((n + PNT - 1) / PNT > pr_max)
+    throw DeviceError("coarse system of " + std::to_string(n) + " unknowns exceeds the dense LU limit (" +
+                      std::to_string(pr_max * PNT) + ")");
   PMG_HIP_CHECK(hipMemsetAsync(C.info, 0, sizeof(int), s));
+  int pj0 = 0, pj1 = 0;
   for (int j0 = 0; j0 < n; j0 += NBR) {
     const int j1 = std::min(n, j0 + NBR);
     const int pr = (n - j0 + PNT - 1) / PNT;
     switch (pr) {
-      case 1: k_lu_panel_reg<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
-      case 2: k_lu_panel_reg<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
-      case 3: k_lu_panel_reg<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
-      case 4: k_lu_panel_reg<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
-      case 5: k_lu_panel_reg<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
-      case 6: k_lu_panel_reg<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
-      default: k_lu_panel<<<1, 1024, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info); break;
+      case 1: k_lu_panel<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
+      case 2: k_lu_panel<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
+      case 3: k_lu_panel<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
+      case 4: k_lu_panel<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
+      case 5: k_lu_panel<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
+      default: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
     }
     PMG_HIP_CHECK(hipGetLastError());
-    const int nout = n - (j1 - j0);
-    if (nout > 0) {
-      k_lu_swap<<<(nout + 255) / 256, 256, 0, s>>>(C.a, n, j0, j1, C.ipiv);
-      PMG_HIP_CHECK(hipGetLastError());
-    }
     if (j1 < n) {
-      k_lu_trsm<<<(n - j1 + 255) / 256, 256, 0, s>>>(C.a, n, j0, j1);
-      PMG_HIP_CHECK(hipGetLastError());
       dim3 g((n - j1 + 63) / 64, (n - j1 + 63) / 64);
-      k_lu_gemm<<<g, 256, 0, s>>>(C.a, n, j0, j1);
+      k_lu_update<<<g, 256, 0, s>>>(C.a, C.ubuf, n, j0, j1);
       PMG_HIP_CHECK(hipGetLastError());
+      pj0 = j0;
+      pj1 = j1;
+    } else {
+      pj0 = pj1 = 0;
     }
   }
+  dim3 gt((n + 63) / 64, (n + 63) / 64);
+  k_transpose<<<gt, 256, 0, s>>>(C.a, C.f, n);
+  PMG_HIP_CHECK(hipGetLastError());
   k_sigma<<<1, 64, size_t(n) * sizeof(int), s>>>(C.ipiv, C.sigma, n);
   PMG_HIP_CHECK(hipGetLastError());
 }
@@ -510,13 +523,13 @@ void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, doubl
   for (int K0 = 0; K0 < n; K0 += TB) {
     const int below = n - std::min(n, K0 + TB);
     const int g = std::max(1, (below + 255) / 256);
-    k_trsv_fwd<<<g, 256, 0, s>>>(C.a, C.b, C.y, n, K0);
+    k_trsv_fwd<<<g, 256, 0, s>>>(C.f, C.b, C.y, n, K0);
     PMG_HIP_CHECK(hipGetLastError());
   }
   const int last = ((n - 1) / TB) * TB;
   for (int K0 = last; K0 >= 0; K0 -= TB) {
     const int g = std::max(1, (K0 + 255) / 256);
-    k_trsv_bwd<<<g, 256, 0, s>>>(C.a, C.y, C.x, n, K0);
+    k_trsv_bwd<<<g, 256, 0, s>>>(C.f, C.y, C.x, n, K0);
     PMG_HIP_CHECK(hipGetLastError());
   }
   const long nf = 3L * op.n_udof + op.n_pdof;

@@ -449,18 +449,21 @@ void setup_factors(pmg_ctx *c) {
     const int pin = c->prm.internal_project_pressure ? 1 : 0;
     const int nr = 3 * D.n_active + D.n_pdof - pin;
     if (C.a && C.n != nr) {
-      for (void *p : {(void *)C.a, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y, (void *)C.x})
+      for (void *p : {(void *)C.a, (void *)C.f, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y,
+                      (void *)C.x, (void *)C.ubuf})
         dfree(c, p);
       C.a = nullptr;
     }
     if (!C.a) {
       C.n = nr;
       C.a = dalloc<double>(c, size_t(C.n) * C.n);
+      C.f = dalloc<double>(c, size_t(C.n) * C.n);
       C.ipiv = dalloc<int>(c, C.n);
       C.sigma = dalloc<int>(c, C.n);
       C.b = dalloc<double>(c, C.n);
       C.y = dalloc<double>(c, C.n);
       C.x = dalloc<double>(c, C.n);
+      C.ubuf = dalloc<double>(c, size_t(C.n) * 16);
       if (!C.info) C.info = dalloc<int>(c, 1);
     }
     C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;

@@ -145,9 +145,11 @@ struct Coarse {
   int n = 0;
   int N = 0, na = 0, np = 0, pin = 0;   // velocity dofs, active velocity dofs, pressure dofs, pinned?
   double *x = nullptr;     // reduced solution
-  double *a = nullptr;     // dense column-major LU
+  double *a = nullptr;     // dense row-major matrix, factored in place
+  double *f = nullptr;     // the LU factors column-major (read by the triangular sweeps)
   int *ipiv = nullptr;     // device
   int *sigma = nullptr;    // b'[i] = b[sigma[i]]
+  double *ubuf = nullptr;  // U block row of the last LU panel step (n * 16)
   double *b = nullptr;     // work vectors of the two sweeps
   double *y = nullptr;
   int *info = nullptr;
