@@ -13,6 +13,7 @@
 // triangular solve of the U block row, trailing update as an fma chain over the panel in
 // order); each visit runs blocked column sweeps in the dgetrs order.
 #include "pmg_internal.hpp"
+#include "pmg_rows.hpp"
 
 namespace pmg {
 
@@ -345,31 +346,35 @@ __device__ inline double dirichlet_value(const OpView<double> &op, const double 
   const int k = op.a_rowptr[c];   // Dirichlet rows hold only their diagonal
   return f[long(b) * op.n_udof + c] / op.a[(4 * b) * op.a_nnz + k];
 }
-__global__ void k_coarse_rhs(OpView<double> op, const double *__restrict__ f, const int *__restrict__ sigma,
-                             double *__restrict__ bout, int n, int pin) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_coarse_rhs(OpView<double> op, const double *__restrict__ f,
+                                                    const int *__restrict__ sigma, double *__restrict__ bout, int n,
+                                                    int pin) {
+  const int lane = threadIdx.x & 15;
+  const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;  // 16 lanes per reduced row
   if (i >= n) return;
   const int k = sigma[i];
   const int N = op.n_udof, NA = op.n_active;
-  double v;
+  double v = 0.0;
+  double f0;
   if (k < 3 * NA) {
     const int br = k / NA, r = k - br * NA;
-    v = f[long(br) * N + r];
-    for (int q = op.a_rowptr[r]; q < op.a_rowptr[r + 1]; ++q) {
+    f0 = f[long(br) * N + r];
+    for (int q = op.a_rowptr[r] + lane; q < op.a_rowptr[r + 1]; q += 16) {
       const int c = op.a_kcol[q];
       if (c < NA) continue;
-      for (int bc = 0; bc < 3; ++bc) v -= op.a[(3 * br + bc) * op.a_nnz + q] * dirichlet_value(op, f, bc, c);
+      for (int bc = 0; bc < 3; ++bc) v += op.a[(3 * br + bc) * op.a_nnz + q] * dirichlet_value(op, f, bc, c);
     }
   } else {
     const int pi = k - 3 * NA + pin;
-    v = f[3L * N + pi];
-    for (int q = op.b_rowptr[pi]; q < op.b_rowptr[pi + 1]; ++q) {
+    f0 = f[3L * N + pi];
+    for (int q = op.b_rowptr[pi] + lane; q < op.b_rowptr[pi + 1]; q += 16) {
       const int c = op.b_kcol[q];
       if (c < NA) continue;
-      for (int b = 0; b < 3; ++b) v -= op.b[b * op.b_nnz + q] * dirichlet_value(op, f, b, c);
+      for (int b = 0; b < 3; ++b) v += op.b[b * op.b_nnz + q] * dirichlet_value(op, f, b, c);
     }
   }
-  bout[i] = v;
+  v = red16(v);
+  if (lane == 0) bout[i] = f0 - v;
 }
 // scatter the reduced solution back: kept dofs, Dirichlet dofs f_D/diag, pinned pressure 0
 __global__ void k_coarse_out(OpView<double> op, const double *__restrict__ f, const double *__restrict__ xr,
@@ -385,86 +390,131 @@ __global__ void k_coarse_out(OpView<double> op, const double *__restrict__ f, co
   }
 }
 
-// forward sweep, block [K0, K0+TB): every workgroup solves the diagonal block in wave 0
-// (rows of the block in b are final), workgroup 0 stores it in y, all workgroups update
-// the rows >= K0+TB of b
-__global__ __launch_bounds__(256) void k_trsv_fwd(const double *__restrict__ a, double *__restrict__ b,
-                                                  double *__restrict__ y, int n, int K0) {
-  __shared__ double xs[TB];
-  const int t = threadIdx.x;
-  const int kb = min(TB, n - K0);
-  if (t < 64) {
-    // the lane's row of the diagonal block, loaded before the dependent chain starts
-    const int tr = K0 + min(t, kb - 1);
-    double row[TB];
-#pragma unroll
-    for (int k = 0; k < TB; ++k) row[k] = a[long(K0 + min(k, kb - 1)) * n + tr];
-    double x = t < kb ? b[K0 + t] : 0.0;
-#pragma unroll
-    for (int k = 0; k < TB; ++k) {
-      if (k < kb) {
-        const double xk = bcast(x, k);
-        if (t > k && t < kb) x = fma(-xk, row[k], x);
-      }
+// Triangular sweeps of the coarse solve, one launch each.  Workgroup g handles the row
+// block of 64 rows that is the g-th to be solved (forward: rows 64g..; backward: the g-th
+// block from the bottom).  It accumulates the products with the solution segments of the
+// blocks solved before it as each one is published (flag of that block = this launch's
+// generation, acquire; each wave polls for itself, no workgroup barrier per segment),
+// then solves its diagonal block in one wavefront and publishes its segment (release).  A workgroup only waits for workgroups with smaller ids, which are
+// dispatched before it, so the chain always drains; every wait is bounded and sets *err
+// when exceeded.  The generation lives in sync[0] and is advanced by the last workgroup to
+// finish (sync[1] counts them), so no host reset is needed between launches.
+constexpr long SPIN_LIMIT = 1L << 24;
+
+__device__ inline bool wait_flag(const unsigned *flag, unsigned gen, int *err) {
+  long spins = 0;
+  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
+    if (++spins > SPIN_LIMIT) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
     }
-    xs[t] = x;
-    if (blockIdx.x == 0 && t < kb) y[K0 + t] = x;
+    __builtin_amdgcn_s_sleep(2);
   }
-  __syncthreads();
-  const int i = K0 + kb + blockIdx.x * 256 + t;
-  if (i >= n) return;
-  double v = b[i];
-#pragma unroll 16
-  for (int k = 0; k < kb; ++k) v = fma(-xs[k], a[long(K0 + k) * n + i], v);
-  b[i] = v;
-}
-// backward sweep, block [K0, K0+kb): diagonal block of y (division) into x, then the
-// rows < K0 of y
-__global__ __launch_bounds__(256) void k_trsv_bwd(const double *__restrict__ a, double *__restrict__ y,
-                                                  double *__restrict__ xo, int n, int K0) {
-  __shared__ double xs[TB];
-  const int t = threadIdx.x;
-  const int kb = min(TB, n - K0);
-  if (t < 64) {
-    const int tr = K0 + min(t, kb - 1);
-    double row[TB];
-#pragma unroll
-    for (int k = 0; k < TB; ++k) row[k] = a[long(K0 + min(k, kb - 1)) * n + tr];
-    double x = t < kb ? y[K0 + t] : 0.0;
-#pragma unroll
-    for (int k = TB - 1; k >= 0; --k) {
-      if (k < kb) {
-        // U(k,k) sits in lane k's row[k]
-        const double xk = bcast(x, k) / bcast(row[k], k);
-        if (t == k) x = xk;
-        if (t < k) x = fma(-xk, row[k], x);
-      }
-    }
-    xs[t] = x;
-    if (blockIdx.x == 0 && t < kb) xo[K0 + t] = x;
-  }
-  __syncthreads();
-  const int i = blockIdx.x * 256 + t;
-  if (i >= K0) return;
-  double v = y[i];
-#pragma unroll 16
-  for (int k = kb - 1; k >= 0; --k) v = fma(-xs[k], a[long(K0 + k) * n + i], v);
-  y[i] = v;
+  return true;
 }
 
-// ------------------------------------------------------------------ transfers
-template <typename T>
-__global__ void k_csr(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
-                      const double *__restrict__ val, int ncomp, const T *__restrict__ x, long xs,
-                      T *__restrict__ y, long ys) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_rows) return;
-  const int k0 = rp[r], k1 = rp[r + 1];
-  for (int c = 0; c < ncomp; ++c) {
-    T s = 0;
-    const T *xc = x + c * xs;
-    for (int k = k0; k < k1; ++k) s += T(val[k]) * xc[col[k]];
-    y[c * ys + r] = s;
+template <bool UPPER>
+__global__ __launch_bounds__(256) void k_trsv_chain(const double *__restrict__ f, const double *__restrict__ rhs,
+                                                    double *__restrict__ x, int n, unsigned *__restrict__ sync,
+                                                    unsigned *__restrict__ flags, int *__restrict__ err) {
+  __shared__ double part[4][TB];
+  __shared__ unsigned gen_s;
+  const int t = threadIdx.x, r = t & 63, qd = t >> 6;
+  const int nblk = (n + TB - 1) / TB;
+  const int g = blockIdx.x;
+  const int rb = UPPER ? nblk - 1 - g : g;  // row block solved by this workgroup
+  const int R0 = rb * TB, kb = min(TB, n - R0);
+  if (t == 0) gen_s = __hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+  __syncthreads();
+  const unsigned gen = gen_s;
+  const int row = R0 + min(r, kb - 1);
+  // the diagonal block's row of this lane (wave 0), loaded before any wait
+  double row_v[TB];
+  if (qd == 0) {
+#pragma unroll
+    for (int k = 0; k < TB; ++k) row_v[k] = f[long(R0 + min(k, kb - 1)) * n + row];
+  }
+  // earlier segments in solve order: wave qd takes columns 16qd..16qd+15 of each; the
+  // factor values are loaded before the segment's flag is awaited
+  double acc = 0.0;
+  bool ok = true;
+  for (int h = 0; h < g; ++h) {
+    const int jb = UPPER ? nblk - 1 - h : h;
+    const int J0 = jb * TB, jw = min(TB, n - J0);
+    double fv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) fv[q] = f[long(J0 + min(qd * 16 + q, jw - 1)) * n + row];
+    if (r == 0 && ok) ok = wait_flag(&flags[jb], gen, err);
+    ok = __shfl(ok ? 1 : 0, 0, 64) != 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int kk = qd * 16 + q;
+      if (kk < jw) {
+        const double xk = __hip_atomic_load(&x[J0 + kk], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        acc = fma(xk, fv[q], acc);
+      }
+    }
+  }
+  part[qd][r] = acc;
+  __syncthreads();
+  if (qd == 0) {
+    double v = r < kb ? rhs[R0 + r] - ((part[0][r] + part[1][r]) + (part[2][r] + part[3][r])) : 0.0;
+    if (!UPPER) {
+#pragma unroll
+      for (int k = 0; k < TB; ++k) {
+        if (k < kb) {
+          const double xk = bcast(v, k);
+          if (r > k && r < kb) v = fma(-xk, row_v[k], v);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = TB - 1; k >= 0; --k) {
+        if (k < kb) {
+          // U(k,k) sits in lane k's row_v[k]
+          const double xk = bcast(v, k) / bcast(row_v[k], k);
+          if (r == k) v = xk;
+          if (r < k) v = fma(-xk, row_v[k], v);
+        }
+      }
+    }
+    if (r < kb) x[R0 + r] = v;
+    __threadfence();
+  }
+  __syncthreads();
+  if (t == 0) {
+    __hip_atomic_store(&flags[rb], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned done = __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (done == unsigned(nblk - 1)) {
+      __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&sync[0], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// y[c*ys + r] = sum_k val[k] x[c*xs + col[k]] for the NC components, 16 lanes per row
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_csr(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
+                                             const double *__restrict__ val, const T *__restrict__ x, long xs,
+                                             T *__restrict__ y, long ys) {
+  const int lane = threadIdx.x & 15;
+  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4, ngrp = gridDim.x * 16;
+  for (int r = grp; r < n_rows; r += ngrp) {
+    const int k0 = rp[r], k1 = rp[r + 1];
+    T s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] = 0;
+    for (int k = k0 + lane; k < k1; k += 16) {
+      const int j = col[k];
+      const T v = T(val[k]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) s[c] += v * x[c * xs + j];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] = red16(s[c]);
+    if (lane == 0)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) y[c * ys + r] = s[c];
   }
 }
 
@@ -518,20 +568,14 @@ void coarse_lu_factor(Coarse &C, hipStream_t s) {
 
 void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s) {
   const int n = C.n;
-  k_coarse_rhs<<<(n + 255) / 256, 256, 0, s>>>(op, f, C.sigma, C.b, n, C.pin);
+  const int nblk = (n + TB - 1) / TB;
+  k_coarse_rhs<<<(n + 15) / 16, 256, 0, s>>>(op, f, C.sigma, C.b, n, C.pin);
   PMG_HIP_CHECK(hipGetLastError());
-  for (int K0 = 0; K0 < n; K0 += TB) {
-    const int below = n - std::min(n, K0 + TB);
-    const int g = std::max(1, (below + 255) / 256);
-    k_trsv_fwd<<<g, 256, 0, s>>>(C.f, C.b, C.y, n, K0);
-    PMG_HIP_CHECK(hipGetLastError());
-  }
-  const int last = ((n - 1) / TB) * TB;
-  for (int K0 = last; K0 >= 0; K0 -= TB) {
-    const int g = std::max(1, (K0 + 255) / 256);
-    k_trsv_bwd<<<g, 256, 0, s>>>(C.f, C.y, C.x, n, K0);
-    PMG_HIP_CHECK(hipGetLastError());
-  }
+  // L y = b (unit lower), then U x = y
+  k_trsv_chain<false><<<nblk, 256, 0, s>>>(C.f, C.b, C.y, n, C.sync, C.flags, C.err);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_trsv_chain<true><<<nblk, 256, 0, s>>>(C.f, C.y, C.x, n, C.sync + 2, C.flags + nblk, C.err);
+  PMG_HIP_CHECK(hipGetLastError());
   const long nf = 3L * op.n_udof + op.n_pdof;
   k_coarse_out<<<int((nf + 255) / 256), 256, 0, s>>>(op, f, C.x, u, C.pin);
   PMG_HIP_CHECK(hipGetLastError());
@@ -540,7 +584,11 @@ void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, doubl
 template <typename T>
 void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long xs, T *y, long ys, hipStream_t s) {
   if (m.n_rows <= 0) return;
-  k_csr<T><<<(m.n_rows + 255) / 256, 256, 0, s>>>(m.n_rows, m.rowptr, m.col, m.val, ncomp, x, xs, y, ys);
+  const int grid = std::min(8192, (m.n_rows + 15) / 16);
+  if (ncomp == 3)
+    k_csr<T, 3><<<grid, 256, 0, s>>>(m.n_rows, m.rowptr, m.col, m.val, x, xs, y, ys);
+  else
+    k_csr<T, 1><<<grid, 256, 0, s>>>(m.n_rows, m.rowptr, m.col, m.val, x, xs, y, ys);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_csr_spmv<double>(const Csr &, int, const double *, long, double *, long, hipStream_t);

@@ -16,14 +16,6 @@ namespace pmg {
 
 namespace {
 
-template <typename T>
-__device__ inline T red16(T v) {
-  v += __shfl_xor(v, 8, 16);
-  v += __shfl_xor(v, 4, 16);
-  v += __shfl_xor(v, 2, 16);
-  v += __shfl_xor(v, 1, 16);
-  return v;
-}
 
 // block-wide sum of two doubles (blockDim.x == 256), fixed order: wave shuffles, then the
 // four wave sums in wave order; the result is valid in every thread

@@ -237,6 +237,15 @@ std::vector<int> build_patch_map(const pmg_level_desc *d, const std::vector<int>
   return map;
 }
 
+// the coarse triangular sweeps bound their waits and flag an exceeded bound on the device
+void check_sweep_error(pmg_ctx *c) {
+  if (!c->coarse.err) return;
+  int e = 0;
+  PMG_HIP_CHECK(hipMemcpyAsync(&e, c->coarse.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (e) throw Fail{PMG_ERR_DEVICE, "coarse triangular sweep: wait bound exceeded"};
+}
+
 void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
   for (int b = 0; b < 9; ++b) h2d(c, D.a + size_t(b) * D.a_nnz, d->a[b], D.a_nnz);
   for (int b = 0; b < 3; ++b) {
@@ -450,7 +459,7 @@ void setup_factors(pmg_ctx *c) {
     const int nr = 3 * D.n_active + D.n_pdof - pin;
     if (C.a && C.n != nr) {
       for (void *p : {(void *)C.a, (void *)C.f, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y,
-                      (void *)C.x, (void *)C.ubuf})
+                      (void *)C.x, (void *)C.ubuf, (void *)C.sync, (void *)C.flags})
         dfree(c, p);
       C.a = nullptr;
     }
@@ -464,6 +473,15 @@ void setup_factors(pmg_ctx *c) {
       C.y = dalloc<double>(c, C.n);
       C.x = dalloc<double>(c, C.n);
       C.ubuf = dalloc<double>(c, size_t(C.n) * 16);
+      const int nblk = (C.n + 63) / 64;
+      C.sync = dalloc<unsigned>(c, 4);
+      C.flags = dalloc<unsigned>(c, 2 * size_t(nblk));
+      PMG_HIP_CHECK(hipMemsetAsync(C.sync, 0, 4 * sizeof(unsigned), c->stream2));
+      PMG_HIP_CHECK(hipMemsetAsync(C.flags, 0, 2 * size_t(nblk) * sizeof(unsigned), c->stream2));
+      if (!C.err) {
+        C.err = dalloc<int>(c, 1);
+        PMG_HIP_CHECK(hipMemsetAsync(C.err, 0, sizeof(int), c->stream2));
+      }
       if (!C.info) C.info = dalloc<int>(c, 1);
     }
     C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;
@@ -1211,6 +1229,7 @@ int pmg_coarse_solve(pmg_ctx *c, double *u, const double *f, int mem) {
     coarse_solve_t<double>(c, D, du, df);
     io.out(0, u);
     io.sync();
+    check_sweep_error(c);
   });
 }
 
@@ -1224,6 +1243,7 @@ int pmg_vcycle(pmg_ctx *c, const double *v, double *z, int mem) {
     vcycle_dev(c, dv, dz);
     io.out(2, z);
     io.sync();
+    check_sweep_error(c);
   });
 }
 
@@ -1240,6 +1260,7 @@ int pmg_fgmres_solve(pmg_ctx *c, double *sol, const double *rhs, int *iters, dou
     st = fgmres_dev(c, ds, dr, &it, &fr);
     io.out(0, sol);
     io.sync();
+    check_sweep_error(c);
   });
   if (iters) *iters = it;
   if (final_res) *final_res = fr;

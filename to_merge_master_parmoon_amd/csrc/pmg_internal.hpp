@@ -153,6 +153,10 @@ struct Coarse {
   double *b = nullptr;     // work vectors of the two sweeps
   double *y = nullptr;
   int *info = nullptr;
+  // triangular sweeps: generation + finish counter per sweep (4), per-row-block flags of the
+  // forward then the backward sweep (2 * ceil(n/64)), wait-limit error flag
+  unsigned *sync = nullptr, *flags = nullptr;
+  int *err = nullptr;
   bool ready = false;
 };
 

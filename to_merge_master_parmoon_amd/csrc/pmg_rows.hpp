@@ -10,6 +10,16 @@
 
 namespace pmg {
 
+// sum over the 16 lanes of a row group (every lane gets the total)
+template <typename T>
+__device__ inline T red16(T v) {
+  v += __shfl_xor(v, 8, 16);
+  v += __shfl_xor(v, 4, 16);
+  v += __shfl_xor(v, 2, 16);
+  v += __shfl_xor(v, 1, 16);
+  return v;
+}
+
 // A row with all nine blocks: s_r += sum_c A_rc(row, col) x_c(col)
 template <typename T, int U>
 __device__ inline void row_a9(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
