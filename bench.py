@@ -81,7 +81,22 @@ def run_gpu(prob, precision: int, steps: int, warmup: int, log):
     return dict(times=times, iters=iters, x=x, hist=hist, timers=tm, mg=mg)
 
 
-def roofline_from(tm: dict):
+def measured_traffic(kernel: str, precision: str, n: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (calibrated against a copy kernel), for the workload it was measured on."""
+    if precision != "fp32" or n != 32:
+        return None, None
+    path = ROOT / "profiles" / "k_vanka_res_oseen32_fp32_hbm_traffic.md"
+    if not path.exists():
+        return None, None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
+            return float(parts[2]), str(path.relative_to(ROOT))
+    return None, None
+
+
+def roofline_from(tm: dict, precision: str, n: int):
     kern = {k: v for k, v in tm.items() if isinstance(v, dict) and v["calls"] > 0}
     solve_kernels = {k: v for k, v in kern.items() if k not in ("patch_lu", "coarse_lu")}
     dom = max(solve_kernels, key=lambda k: solve_kernels[k]["ms"])
@@ -95,9 +110,13 @@ def roofline_from(tm: dict):
                  for k, x in kern.items()}
     names = {"vanka_res": "k_vanka_res", "vanka_solve": "k_vanka_solve", "defect": "k_type4<2>",
              "matvec": "k_type4<0/1>", "transfer": "k_csr", "coarse_solve": "k_trsv_fwd/bwd"}
+    traffic, src = measured_traffic(names.get(dom, dom), precision, n)
     roof = {"bound": "hbm", "kernel": names.get(dom, dom), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else round(traffic),
             "bytes_per_launch": round(bytes_per_launch), "avg_launch_us": round(avg_ms * 1e3, 2)}
+    if src:
+        roof["traffic_source"] = src
     return roof, breakdown
 
 
@@ -168,7 +187,7 @@ def main():
     log(f"measured solve: {args.precision}")
     g = run_gpu(prob, prec, args.steps, args.warmup, log)
     ms = 1e3 * float(np.mean(g["times"]))
-    roof, breakdown = roofline_from(g["timers"])
+    roof, breakdown = roofline_from(g["timers"], args.precision, args.n)
     iters = int(g["iters"][-1])
     cfg = {"workload": f"oseen3d_{args.n}^3_q2p1disc_fgmres30_vcycle22", "levels": len(g['mg'].n_dof),
            "dofs": sz["n_dof"], "precision": args.precision, "nu": 1e-2,

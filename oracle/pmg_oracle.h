@@ -10,6 +10,10 @@
  * itself cannot be compiled here (its hot-path headers include cuda.h, magma_v2.h and
  * mkl.h unconditionally), so this restatement is pinned by the closed-form and brute-force
  * checks in tests/test_oracle_*.py.
+ *
+ * Parity unpinned: the step sequence of Cycle for V(nu1, nu2) with nu > 0 (smoothing inside
+ * the recursion) is checked only through its parts and the V(0,0) closed form; the greedy
+ * color order of oracle_color_cells is the reference's by reading, checked for validity.
  */
 #ifndef PMG_ORACLE_H
 #define PMG_ORACLE_H
