@@ -37,17 +37,18 @@ __device__ inline long cpre(int q, int na, int pin) { return 3L * na + q - pin; 
 // :5739-5833 restricted to the kept rows and columns: A blocks of active rows/columns,
 // BT columns except the pinned one, B rows except the pinned one.  Row-major a[row*n+col]
 // while factoring (row interchanges and panel rows are contiguous).
-__global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long n, int pin) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long n, int pin) {
+  const int lane = threadIdx.x & 15;
+  const int r = (blockIdx.x * 256 + threadIdx.x) >> 4;  // 16 lanes per operator row
   const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
   if (r < NA) {
-    for (int k = op.a_rowptr[r]; k < op.a_rowptr[r + 1]; ++k) {
+    for (int k = op.a_rowptr[r] + lane; k < op.a_rowptr[r + 1]; k += 16) {
       const int c = op.a_kcol[k];
       if (c >= NA) continue;
       for (int br = 0; br < 3; ++br)
         for (int bc = 0; bc < 3; ++bc) a[cvel(br, r, NA) * n + cvel(bc, c, NA)] = op.a[(3 * br + bc) * op.a_nnz + k];
     }
-    for (int k = op.bt_rowptr[r]; k < op.bt_rowptr[r + 1]; ++k) {
+    for (int k = op.bt_rowptr[r] + lane; k < op.bt_rowptr[r + 1]; k += 16) {
       const int q = op.bt_kcol[k];
       if (pin && q == 0) continue;
       for (int b = 0; b < 3; ++b) a[cvel(b, r, NA) * n + cpre(q, NA, pin)] = op.bt[b * op.bt_nnz + k];
@@ -55,12 +56,18 @@ __global__ void k_coarse_scatter(OpView<double> op, double *__restrict__ a, long
   } else if (r >= N && r < N + NP) {
     const int i = r - N;
     if (pin && i == 0) return;
-    for (int k = op.b_rowptr[i]; k < op.b_rowptr[i + 1]; ++k) {
+    for (int k = op.b_rowptr[i] + lane; k < op.b_rowptr[i + 1]; k += 16) {
       const int c = op.b_kcol[k];
       if (c >= NA) continue;
       for (int b = 0; b < 3; ++b) a[cpre(i, NA, pin) * n + cvel(b, c, NA)] = op.b[b * op.b_nnz + k];
     }
   }
+}
+
+// identity permutation over all n reduced unknowns (no interchanges yet)
+__global__ void k_iota(int *__restrict__ sigma, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) sigma[i] = i;
 }
 
 __global__ void k_fill_d(double *x, double v, long n) {
@@ -77,7 +84,8 @@ __global__ void k_fill_d(double *x, double v, long n) {
 template <int PR>
 __global__ __launch_bounds__(PNT) void k_lu_panel(double *__restrict__ a, int n, int j0, int j1,
                                                   int *__restrict__ ipiv, int *__restrict__ info,
-                                                  const double *__restrict__ ubuf, int pj0, int pj1) {
+                                                  const double *__restrict__ ubuf, int pj0, int pj1,
+                                                  int *__restrict__ sigma) {
   __shared__ double sv[2][PNT / 64];
   __shared__ int si[2][PNT / 64];
   __shared__ double bufP[2][NBR], bufJ[2][NBR];
@@ -220,6 +228,9 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(double *__restrict__ a, int n,
       src_s[lane] = src;
     }
     if (lane == 0) nk_s = nk;
+    // the permutation vector (sigma[i] = original row now at row i) moves like a column
+    const int sg = lane < nk ? sigma[src] : 0;
+    if (lane < nk && src != row) sigma[row] = sg;
   }
   __syncthreads();
   const int nk = nk_s;
@@ -323,22 +334,6 @@ __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ a,
     if (r0 + r < n && c0 + c < n) f[long(c0 + c) * n + r0 + r] = tile[r][c];
   }
 }
-// b'[i] = b[sigma[i]] of the sequential interchanges (permutation kept in LDS)
-__global__ void k_sigma(const int *ipiv, int *sigma, int n) {
-  extern __shared__ int sg[];
-  if (threadIdx.x != 0) return;
-  for (int i = 0; i < n; ++i) sg[i] = i;
-  for (int k = 0; k < n; ++k) {
-    const int p = ipiv[k];
-    if (p != k) {
-      const int x = sg[k];
-      sg[k] = sg[p];
-      sg[p] = x;
-    }
-  }
-  for (int i = 0; i < n; ++i) sigma[i] = sg[i];
-}
-
 // ------------------------------------------------------------------ per-visit solve
 // reduced right-hand side, already permuted: b[i] = (f_I - K_ID x_D)[sigma[i]] with
 // x_D = f_D / diag(A_cc) on the Dirichlet velocity dofs
@@ -525,7 +520,9 @@ void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s) {
   k_fill_d<<<2048, 256, 0, s>>>(C.a, 0.0, n * n);
   PMG_HIP_CHECK(hipGetLastError());
   const int rows = L.n_udof + L.n_pdof;
-  k_coarse_scatter<<<(rows + 127) / 128, 128, 0, s>>>(L.op64(), C.a, n, C.pin);
+  k_coarse_scatter<<<(rows * 16 + 255) / 256, 256, 0, s>>>(L.op64(), C.a, n, C.pin);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_iota<<<int((n + 255) / 256), 256, 0, s>>>(C.sigma, int(n));
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -541,12 +538,12 @@ void coarse_lu_factor(Coarse &C, hipStream_t s) {
     const int j1 = std::min(n, j0 + NBR);
     const int pr = (n - j0 + PNT - 1) / PNT;
     switch (pr) {
-      case 1: k_lu_panel<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
-      case 2: k_lu_panel<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
-      case 3: k_lu_panel<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
-      case 4: k_lu_panel<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
-      case 5: k_lu_panel<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
-      default: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1); break;
+      case 1: k_lu_panel<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      case 2: k_lu_panel<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      case 3: k_lu_panel<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      case 4: k_lu_panel<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      case 5: k_lu_panel<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      default: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
     }
     PMG_HIP_CHECK(hipGetLastError());
     if (j1 < n) {
@@ -561,8 +558,6 @@ void coarse_lu_factor(Coarse &C, hipStream_t s) {
   }
   dim3 gt((n + 63) / 64, (n + 63) / 64);
   k_transpose<<<gt, 256, 0, s>>>(C.a, C.f, n);
-  PMG_HIP_CHECK(hipGetLastError());
-  k_sigma<<<1, 64, size_t(n) * sizeof(int), s>>>(C.ipiv, C.sigma, n);
   PMG_HIP_CHECK(hipGetLastError());
 }
 

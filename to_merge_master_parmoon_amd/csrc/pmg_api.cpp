@@ -237,11 +237,29 @@ std::vector<int> build_patch_map(const pmg_level_desc *d, const std::vector<int>
   return map;
 }
 
-// the coarse triangular sweeps bound their waits and flag an exceeded bound on the device
-void check_sweep_error(pmg_ctx *c) {
-  if (!c->coarse.err) return;
+// Results reported by the device after the fact: the coarse LU (second stream) pivot check
+// and its time, and the wait-bound flag of the coarse triangular sweeps.
+void check_deferred(pmg_ctx *c) {
+  Coarse &C = c->coarse;
+  if (C.info_pending) {
+    C.info_pending = false;
+    int cinfo = 0;
+    PMG_HIP_CHECK(hipEventSynchronize(C.ev_done));
+    PMG_HIP_CHECK(hipMemcpy(&cinfo, C.info, sizeof(int), hipMemcpyDeviceToHost));
+    float ms = 0;
+    PMG_HIP_CHECK(hipEventElapsedTime(&ms, C.ev_start, C.ev_done));
+    c->tm.coarse_lu.ms += ms;
+    c->tm.coarse_lu.calls += 1;
+    c->tm.coarse_lu.bytes += double(C.n) * C.n * 16.0;
+    c->tm.coarse_lu.flops += (2.0 / 3.0) * double(C.n) * C.n * C.n;
+    if (cinfo) {
+      C.ready = false;
+      throw Fail{PMG_ERR_SINGULAR_COARSE, "zero pivot " + std::to_string(cinfo) + " in the coarse LU"};
+    }
+  }
+  if (!C.err) return;
   int e = 0;
-  PMG_HIP_CHECK(hipMemcpyAsync(&e, c->coarse.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PMG_HIP_CHECK(hipMemcpyAsync(&e, C.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
   if (e) throw Fail{PMG_ERR_DEVICE, "coarse triangular sweep: wait bound exceeded"};
 }
@@ -371,6 +389,10 @@ void prolongate(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *uc, T
 // coarse solve in FP64 (vectors of type T converted at the boundary)
 template <typename T>
 void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
+  if (c->coarse.pending) {
+    PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, c->coarse.ev_done, 0));
+    c->coarse.pending = false;
+  }
   const double n = c->coarse.n;
   Timed tm(c, &c->tm.coarse_solve, n * n * 8.0, 2.0 * n * n);
   if constexpr (sizeof(T) == 8) {
@@ -450,6 +472,7 @@ void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
 // value-dependent set-up: the dense coarse LU runs on a second stream while the batched
 // patch LUs of all finer levels occupy the main stream
 void setup_factors(pmg_ctx *c) {
+  check_deferred(c);
   auto t0 = std::chrono::steady_clock::now();
   const bool do_coarse = !c->coarse.ready;
   Coarse &C = c->coarse;
@@ -485,10 +508,18 @@ void setup_factors(pmg_ctx *c) {
       if (!C.info) C.info = dalloc<int>(c, 1);
     }
     C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;
+    if (!C.ev_start) {
+      PMG_HIP_CHECK(hipEventCreate(&C.ev_start));
+      PMG_HIP_CHECK(hipEventCreate(&C.ev_done));
+    }
     PMG_HIP_CHECK(hipEventRecord(c->evs, c->stream));
     PMG_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->evs, 0));
+    PMG_HIP_CHECK(hipEventRecord(C.ev_start, c->stream2));
     launch_coarse_build(D, C, c->stream2);
     coarse_lu_factor(C, c->stream2);
+    PMG_HIP_CHECK(hipEventRecord(C.ev_done, c->stream2));
+    C.pending = true;
+    C.info_pending = true;
   }
   std::vector<std::vector<int>> infos(c->L.size());
   for (size_t l = 1; l < c->L.size(); ++l) {
@@ -503,12 +534,6 @@ void setup_factors(pmg_ctx *c) {
   }
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
   const double t_patch = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  int cinfo = 0;
-  if (do_coarse) {
-    PMG_HIP_CHECK(hipMemcpyAsync(&cinfo, C.info, sizeof(int), hipMemcpyDeviceToHost, c->stream2));
-    PMG_HIP_CHECK(hipStreamSynchronize(c->stream2));
-  }
-  const double t_all = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (size_t l = 1; l < c->L.size(); ++l) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
@@ -523,14 +548,7 @@ void setup_factors(pmg_ctx *c) {
     D.factors_ready = true;
   }
   c->tm.patch_lu.ms += t_patch;
-  if (do_coarse) {
-    c->tm.coarse_lu.ms += t_all;   // wall time from the start of set-up (runs concurrently)
-    c->tm.coarse_lu.calls += 1;
-    c->tm.coarse_lu.bytes += double(C.n) * C.n * 16.0;
-    c->tm.coarse_lu.flops += (2.0 / 3.0) * double(C.n) * C.n * C.n;
-    if (cinfo) throw Fail{PMG_ERR_SINGULAR_COARSE, "zero pivot " + std::to_string(cinfo) + " in the coarse LU"};
-    C.ready = true;
-  }
+  if (do_coarse) C.ready = true;   // usable once ev_done completes; checked in check_deferred
   c->setup_done = true;
 }
 
@@ -830,6 +848,8 @@ int pmg_ctx_destroy(pmg_ctx *c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evs) (void)hipEventDestroy(c->evs);
+  if (c->coarse.ev_start) (void)hipEventDestroy(c->coarse.ev_start);
+  if (c->coarse.ev_done) (void)hipEventDestroy(c->coarse.ev_done);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1229,7 +1249,7 @@ int pmg_coarse_solve(pmg_ctx *c, double *u, const double *f, int mem) {
     coarse_solve_t<double>(c, D, du, df);
     io.out(0, u);
     io.sync();
-    check_sweep_error(c);
+    check_deferred(c);
   });
 }
 
@@ -1243,7 +1263,7 @@ int pmg_vcycle(pmg_ctx *c, const double *v, double *z, int mem) {
     vcycle_dev(c, dv, dz);
     io.out(2, z);
     io.sync();
-    check_sweep_error(c);
+    check_deferred(c);
   });
 }
 
@@ -1260,7 +1280,7 @@ int pmg_fgmres_solve(pmg_ctx *c, double *sol, const double *rhs, int *iters, dou
     st = fgmres_dev(c, ds, dr, &it, &fr);
     io.out(0, sol);
     io.sync();
-    check_sweep_error(c);
+    check_deferred(c);
   });
   if (iters) *iters = it;
   if (final_res) *final_res = fr;
@@ -1295,7 +1315,11 @@ int pmg_memcpy(pmg_ctx *c, void *dst, const void *src, size_t bytes, int kind) {
 }
 int pmg_synchronize(pmg_ctx *c) {
   if (!c) return PMG_ERR_INVALID_ARGUMENT;
-  return guard(c, [&] { PMG_HIP_CHECK(hipStreamSynchronize(c->stream)); });
+  return guard(c, [&] {
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream2));
+    check_deferred(c);
+  });
 }
 int pmg_timers_get(const pmg_ctx *c, pmg_timers *t) {
   if (!c || !t) return PMG_ERR_INVALID_ARGUMENT;

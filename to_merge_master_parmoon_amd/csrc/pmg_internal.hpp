@@ -157,6 +157,10 @@ struct Coarse {
   // forward then the backward sweep (2 * ceil(n/64)), wait-limit error flag
   unsigned *sync = nullptr, *flags = nullptr;
   int *err = nullptr;
+  // the factorization runs on the second stream; the main stream waits for ev_done only
+  // before the first coarse solve, and the pivot check happens at the next host sync
+  hipEvent_t ev_start = nullptr, ev_done = nullptr;
+  bool pending = false, info_pending = false;
   bool ready = false;
 };
 
