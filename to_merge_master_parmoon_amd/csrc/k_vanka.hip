@@ -56,9 +56,10 @@ struct Ld<float, uint16_t> {
 // entries are gathered straight into the register tiles through the per-cell map of
 // offsets into the row-blocked operator values (built once per level from the
 // structure), then factored with dgetf2 semantics.
-__device__ __forceinline__ double patch_entry(const OpView<double> &op, const int *__restrict__ mp, int r, int c) {
+template <typename T>
+__device__ __forceinline__ T patch_entry(const OpView<T> &op, const int *__restrict__ mp, int r, int c) {
   int mi = -1, li = 0, blk = 0;
-  const double *base = op.av;
+  const T *base = op.av;
   bool dir_off = false;
   if (r < 3 * NU_LOC) {
     const int rr = r / NU_LOC, j = r - NU_LOC * rr;
@@ -88,26 +89,40 @@ __device__ __forceinline__ double patch_entry(const OpView<double> &op, const in
     len = -len;
     valid = valid && !dir_off;
   }
-  const double v = base[valid ? off + blk * len : 0];
-  return valid ? v : 0.0;
+  const T v = base[valid ? off + blk * len : 0];
+  return valid ? v : T(0);
 }
 
+// factor entry in the storage precision (bf16: round to nearest even)
+template <typename F>
+struct StoreF {
+  template <typename T>
+  static __device__ inline F get(T v) { return F(v); }
+};
+template <>
+struct StoreF<uint16_t> {
+  static __device__ inline uint16_t get(float v) {
+    const uint32_t u = __float_as_uint(v);
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+  }
+};
+
 // one block of 16 columns (jb = JB) of the register-tiled dgetf2 in k_patch_lu
-template <int JB>
-__device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int ty, int *sigma, double *lbuf,
-                                           double *ubuf, int &piv_s, int &nz_s, int &info_s) {
+template <int JB, typename T>
+__device__ __forceinline__ void lu_columns(T (&A)[6][6], int t, int tx, int ty, int *sigma, T *lbuf, T *ubuf,
+                                           int &piv_s, int &nz_s, int &info_s) {
   constexpr int jb = JB;
   const int jl_end = jb == 5 ? NLOC - 80 : 16;
   for (int jl = 0; jl < jl_end; ++jl) {
     const int j = 16 * jb + jl;
     // pivot: first row of max |a(i, j)|, i >= j
     if (tx == jl) {
-      double bv = -1.0;
+      T bv = T(-1);
       int bi = NLOC;
 #pragma unroll
       for (int a = jb; a < 6; ++a) {
         const int r = ty + 16 * a;
-        const double v = fabs(A[a][jb]);
+        const T v = fabs(A[a][jb]);
         if (r >= j && r < NLOC && v > bv) {
           bv = v;
           bi = r;
@@ -115,7 +130,7 @@ __device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int
       }
 #pragma unroll
       for (int o = 8; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, 16);
+        const T ov = __shfl_xor(bv, o, 16);
         const int oi = __shfl_xor(bi, o, 16);
         if (ov > bv || (ov == bv && oi < bi)) {
           bv = ov;
@@ -148,15 +163,15 @@ __device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int
           if (pl != jl) {
 #pragma unroll
             for (int b = 0; b < 6; ++b) {
-              const double send = ty == jl ? A[jb][b] : A[a][b];
-              const double recv = __shfl(send, src, 16);
+              const T send = ty == jl ? A[jb][b] : A[a][b];
+              const T recv = __shfl(send, src, 16);
               if (ty == jl) A[jb][b] = recv;
               if (ty == pl) A[a][b] = recv;
             }
           } else if (ty == jl) {
 #pragma unroll
             for (int b = 0; b < 6; ++b) {
-              const double rowj = A[jb][b];
+              const T rowj = A[jb][b];
               A[jb][b] = A[a][b];
               A[a][b] = rowj;
             }
@@ -167,7 +182,7 @@ __device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int
     // scale column j below the diagonal by the reciprocal of the pivot, publish l and u
     if (tx == jl) {
       if (nonzero) {
-        const double rc = 1.0 / __shfl(A[jb][jb], jl, 16);
+        const T rc = T(1) / __shfl(A[jb][jb], jl, 16);
 #pragma unroll
         for (int a = jb; a < 6; ++a)
           if (ty + 16 * a > j) A[a][jb] *= rc;
@@ -180,7 +195,7 @@ __device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int
       for (int b = jb; b < 6; ++b) ubuf[tx + 16 * b] = A[jb][b];
     }
     __syncthreads();
-    double lv[6], uv[6];
+    T lv[6], uv[6];
 #pragma unroll
     for (int a = jb; a < 6; ++a) lv[a] = lbuf[ty + 16 * a];
 #pragma unroll
@@ -201,12 +216,14 @@ __device__ __forceinline__ void lu_columns(double (&A)[6][6], int t, int tx, int
 // 16 lanes tx = j % 16 of one wave (pivot search and scaling by shuffles), row j in the
 // lanes ty = j % 16 of every 16-lane group (row swaps by shuffles); l and u are published
 // through LDS, two barriers per column.
-__global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *__restrict__ cells,
-                                                  const int *__restrict__ pmap, int ncells,
-                                                  double *__restrict__ lu, uint8_t *__restrict__ invperm,
-                                                  int *__restrict__ info) {
+// T: arithmetic of the factorization (FP64 for the plain path, FP32 for the mixed path,
+// gathering from the FP32 operator copy), F: storage of the factors (T, or bf16).
+template <typename T, typename F>
+__global__ __launch_bounds__(256) void k_patch_lu(OpView<T> op, const int *__restrict__ cells,
+                                                  const int *__restrict__ pmap, int ncells, F *__restrict__ lu,
+                                                  uint8_t *__restrict__ invperm, int *__restrict__ info) {
   __shared__ int sigma[NLOC];
-  __shared__ double lbuf[96], ubuf[96];
+  __shared__ T lbuf[96], ubuf[96];
   __shared__ int piv_s, info_s, nz_s;
   const int pos = blockIdx.x;
   if (pos >= ncells) return;
@@ -215,11 +232,11 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *
   const int *__restrict__ mp = pmap + long(cells[pos]) * PM_STRIDE;
   if (t < NLOC) sigma[t] = t;
   if (t == 0) info_s = 0;
-  double A[6][6];
+  T A[6][6];
 #pragma unroll
   for (int a = 0; a < 6; ++a)
 #pragma unroll
-    for (int b = 0; b < 6; ++b) A[a][b] = patch_entry(op, mp, ty + 16 * a, tx + 16 * b);
+    for (int b = 0; b < 6; ++b) A[a][b] = patch_entry<T>(op, mp, ty + 16 * a, tx + 16 * b);
   __syncthreads();
   lu_columns<0>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
   lu_columns<1>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
@@ -229,7 +246,7 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *
   lu_columns<5>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
   // packed store straight from the tiles: strictly lower part column by column at
   // [0, LU_L), upper part (with the diagonal) column by column at [LU_UOFF, LU_UOFF + LU_U)
-  double *out = lu + long(pos) * LU_STRIDE;
+  F *out = lu + long(pos) * LU_STRIDE;
 #pragma unroll
   for (int a = 0; a < 6; ++a)
 #pragma unroll
@@ -237,9 +254,9 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<double> op, const int *
       const int r = ty + 16 * a, c = tx + 16 * b;
       if (r < NLOC && c < NLOC) {
         if (r > c)
-          out[lu_loff(c) + r - c - 1] = A[a][b];
+          out[lu_loff(c) + r - c - 1] = StoreF<F>::get(A[a][b]);
         else
-          out[LU_UOFF + lu_uoff(c) + r] = A[a][b];
+          out[LU_UOFF + lu_uoff(c) + r] = StoreF<F>::get(A[a][b]);
       }
     }
   if (t < NLOC) invperm[long(pos) * PERM_STRIDE + sigma[t]] = uint8_t(t);
@@ -359,13 +376,26 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1
   // branch-free: every lane reads an in-range LDS entry and keeps its old value where the
   // row is not below the pivot, so the reads carry no control dependence and issue early
+  // the L entries of step k are read PF steps ahead (independent of the chain), so the
+  // chain never waits for LDS
+  constexpr int PF = 6;
+  T pl0[PF], pl1[PF];
+#pragma unroll
+  for (int k = 0; k < PF; ++k) {
+    const int o = lu_loff(k) - k - 1;
+    pl0[k] = Ld<T, F>::get(sl, lane > k ? o + lane : 0);
+    pl1[k] = Ld<T, F>::get(sl, (two && lane + 64 > k) ? o + 64 + lane : 0);
+  }
 #pragma unroll
   for (int k = 0; k < NLOC - 1; ++k) {
+    const T l0 = pl0[k % PF], l1 = pl1[k % PF];
+    if (k + PF < NLOC - 1) {
+      const int kn = k + PF, o = lu_loff(kn) - kn - 1;
+      pl0[k % PF] = Ld<T, F>::get(sl, lane > kn ? o + lane : 0);
+      pl1[k % PF] = Ld<T, F>::get(sl, (two && lane + 64 > kn) ? o + 64 + lane : 0);
+    }
     const T bk = bcast(k < 64 ? x0 : x1, k & 63);
-    const int o = lu_loff(k) - k - 1;
     const bool a0 = lane > k, a1 = two && lane + 64 > k;
-    const T l0 = Ld<T, F>::get(sl, a0 ? o + lane : 0);
-    const T l1 = Ld<T, F>::get(sl, a1 ? o + 64 + lane : 0);
     const T y0 = tfma(-bk, l0, x0), y1 = tfma(-bk, l1, x1);
     x0 = a0 ? y0 : x0;
     x1 = a1 ? y1 : x1;
@@ -383,9 +413,22 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   const T dg0 = Ld<T, F>::get(sl, lu_uoff(lane) + lane);
   const T dg1 = two ? Ld<T, F>::get(sl, lu_uoff(64 + lane) + 64 + lane) : T(1);
   const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
+  T pu0[PF], pu1[PF];
+#pragma unroll
+  for (int i = 0; i < PF; ++i) {
+    const int k = NLOC - 1 - i, o = lu_uoff(k);
+    pu0[i] = Ld<T, F>::get(sl, lane < k ? o + lane : 0);
+    pu1[i] = Ld<T, F>::get(sl, (two && lane + 64 < k) ? o + 64 + lane : 0);
+  }
 #pragma unroll
   for (int k = NLOC - 1; k >= 0; --k) {
-    const int o = lu_uoff(k);
+    const int i = NLOC - 1 - k;
+    const T u0 = pu0[i % PF], u1 = pu1[i % PF];
+    if (k - PF >= 0) {
+      const int kn = k - PF, o = lu_uoff(kn);
+      pu0[i % PF] = Ld<T, F>::get(sl, lane < kn ? o + lane : 0);
+      pu1[i % PF] = Ld<T, F>::get(sl, (two && lane + 64 < kn) ? o + 64 + lane : 0);
+    }
     T q0 = x0 * rc0, q1 = x1 * rc1;
     if constexpr (sizeof(T) == 8) {
       q0 = tfma(tfma(-q0, dg0, x0), rc0, q0);
@@ -393,8 +436,6 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
     }
     const T xk = bcast(k < 64 ? q0 : q1, k & 63);
     const bool a0 = lane < k, a1 = two && lane + 64 < k;
-    const T u0 = Ld<T, F>::get(sl, a0 ? o + lane : 0);
-    const T u1 = Ld<T, F>::get(sl, a1 ? o + 64 + lane : 0);
     const T y0 = tfma(-xk, u0, x0), y1 = tfma(-xk, u1, x1);
     if (k < 64) {
       x0 = (lane == k) ? xk : (a0 ? y0 : x0);
@@ -425,24 +466,19 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   }
 }
 
-__global__ void k_lu_f32(const double *__restrict__ a, float *__restrict__ b, long n) {
-  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
-    b[i] = float(a[i]);
-}
-// round-to-nearest-even bf16
-__global__ void k_lu_bf16(const double *__restrict__ a, uint16_t *__restrict__ b, long n) {
-  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) {
-    const uint32_t u = __float_as_uint(float(a[i]));
-    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);
-    b[i] = uint16_t(r >> 16);
-  }
-}
-
 }  // namespace
 
-void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s) {
+void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s) {
   if (L.n_cells <= 0) return;
-  k_patch_lu<<<L.n_cells, 256, 0, s>>>(L.op64(), L.cell_reorder, L.pmap, L.n_cells, lu, L.invperm, info);
+  if (L.lu)
+    k_patch_lu<double, double><<<L.n_cells, 256, 0, s>>>(L.op64(), L.cell_reorder, L.pmap, L.n_cells, L.lu, L.invperm,
+                                                         info);
+  else if (L.lu32)
+    k_patch_lu<float, float><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.lu32, L.invperm,
+                                                       info);
+  else
+    k_patch_lu<float, uint16_t><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.lu16,
+                                                          L.invperm, info);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
@@ -477,17 +513,5 @@ template void launch_vanka_solve<float, float>(const float *, const int *, int, 
 template void launch_vanka_solve<float, uint16_t>(const uint16_t *, const int *, int, int, const float *,
                                                   const int *, const int *, int, float *, float, hipStream_t);
 
-void launch_convert_lu_f32(const double *lu, float *lu32, long n, hipStream_t s) {
-  long g = (n + 255) / 256;
-  if (g > 16384) g = 16384;
-  k_lu_f32<<<int(g), 256, 0, s>>>(lu, lu32, n);
-  PMG_HIP_CHECK(hipGetLastError());
-}
-void launch_convert_lu_bf16(const double *lu, uint16_t *lu16, long n, hipStream_t s) {
-  long g = (n + 255) / 256;
-  if (g > 16384) g = 16384;
-  k_lu_bf16<<<int(g), 256, 0, s>>>(lu, lu16, n);
-  PMG_HIP_CHECK(hipGetLastError());
-}
 
 }  // namespace pmg

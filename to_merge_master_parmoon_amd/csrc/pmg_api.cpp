@@ -525,9 +525,7 @@ void setup_factors(pmg_ctx *c) {
   for (size_t l = 1; l < c->L.size(); ++l) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
-    launch_patch_lu(D, D.lu, D.patch_info, c->stream);
-    if (D.lu32) launch_convert_lu_f32(D.lu, D.lu32, long(D.n_cells) * LU_STRIDE + LU_PAD, c->stream);
-    if (D.lu16) launch_convert_lu_bf16(D.lu, D.lu16, long(D.n_cells) * LU_STRIDE + LU_PAD, c->stream);
+    launch_patch_lu(D, D.patch_info, c->stream);
     infos[l].resize(D.n_cells);
     PMG_HIP_CHECK(hipMemcpyAsync(infos[l].data(), D.patch_info, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost,
                                  c->stream));
@@ -1012,10 +1010,16 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     }
     if (lvl > 0) {
       const size_t lun = size_t(D.n_cells) * LU_STRIDE + LU_PAD;
-      D.lu = dalloc<double>(c, lun);
-      launch_fill<double>(D.lu, 0.0, long(lun), c->stream);
-      if (c->prm.precision == PMG_PREC_FP32) D.lu32 = dalloc<float>(c, lun);
-      if (c->prm.precision == PMG_PREC_FP32_BF16) D.lu16 = dalloc<uint16_t>(c, lun);
+      if (c->prm.precision == PMG_PREC_FP64) {
+        D.lu = dalloc<double>(c, lun);
+        launch_fill<double>(D.lu, 0.0, long(lun), c->stream);
+      } else if (c->prm.precision == PMG_PREC_FP32) {
+        D.lu32 = dalloc<float>(c, lun);
+        launch_fill<float>(D.lu32, 0.0f, long(lun), c->stream);
+      } else {
+        D.lu16 = dalloc<uint16_t>(c, lun);
+        PMG_HIP_CHECK(hipMemsetAsync(D.lu16, 0, lun * sizeof(uint16_t), c->stream));
+      }
       D.invperm = dalloc<uint8_t>(c, size_t(D.n_cells) * PERM_STRIDE);
       {
         const std::vector<int> pm = build_patch_map(d, ug, pg);

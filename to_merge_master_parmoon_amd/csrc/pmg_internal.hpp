@@ -179,7 +179,9 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
                   double *res, RedScratch &red, hipStream_t s);
 
 // Vanka
-void launch_patch_lu(const DevLevel &L, double *lu, int *info, hipStream_t s);
+// patch LU of every cell of a level into the level's factor array: FP64 (lu), or, in the
+// mixed path, computed in FP32 from the FP32 operator copy into lu32 or bf16 lu16
+void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0,
                            const int *u_global, const int *p_global, const uint8_t *invperm,
@@ -191,8 +193,6 @@ template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc,
                         const int *u_global, const int *p_global, int n_udof, T *u, T damp,
                         hipStream_t s);
-void launch_convert_lu_f32(const double *lu, float *lu32, long n, hipStream_t s);
-void launch_convert_lu_bf16(const double *lu, uint16_t *lu16, long n, hipStream_t s);
 
 // vectors
 template <typename T>
