@@ -337,7 +337,12 @@ struct PackedSizes {
   static constexpr int RL = (NVL + 63) / 64, RU = (NVU + 63) / 64;
   // every lane loads RL (RU) vectors unconditionally; the overshoot past the part stays
   // inside the patch stride or the LU_PAD tail of the allocation
-  static constexpr int LDS_BYTES = 16 * 64 * (RU > RL ? RU : RL);
+  // the sweeps read at constant per-step offsets from each lane's base and discard the
+  // entries of rows outside the step's range; PADL front entries keep those indices >= 0
+  static constexpr int PADL = 64;
+  static constexpr int LDS_BYTES = PADL * int(sizeof(F)) + 16 * 64 * (RU > RL ? RU : RL);
+  // the highest entry a sweep reads, U(84,84) + 127 - 84 = lu_uoff(84) + 127, is inside
+  static_assert(lu_uoff(NLOC - 1) + 127 < 16 * 64 * RU / int(sizeof(F)), "sweep reads past the LDS copy");
   static_assert(LU_UOFF * sizeof(F) + 16 * 64 * RU <= (LU_STRIDE + LU_PAD) * sizeof(F), "pad too small");
 };
 
@@ -350,9 +355,12 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
                                                     const int *__restrict__ p_global, int N,
                                                     T *__restrict__ u, T damp) {
   using PS = PackedSizes<F>;
-  extern __shared__ v4u s4[];
-  const F *sl = reinterpret_cast<const F *>(s4);
+  extern __shared__ v4u s4raw[];
+  v4u *s4 = s4raw + PS::PADL * sizeof(F) / 16;  // factor part, after the front pad
   const int lane = threadIdx.x;
+  // each lane reads its rows at constant per-step offsets from these bases
+  const F *sl0 = reinterpret_cast<const F *>(s4) + lane;
+  const F *sl1 = sl0 + 64;
   const int slot = blockIdx.x;
   if (slot >= ncells) return;
   const int pos = pos0 + slot;
@@ -369,80 +377,61 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
 #pragma unroll
   for (int q = 0; q < PS::RU; ++q) rU[q] = gU[lane + 64 * q];
   const T *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
-  const bool two = lane < NLOC - 64;
+  const bool two = lane < NLOC - 64;  // lanes owning a second row 64 + lane
   T x0 = rs[lane];
   T x1 = two ? rs[64 + lane] : T(0);
   __syncthreads();
-  // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1
-  // branch-free: every lane reads an in-range LDS entry and keeps its old value where the
-  // row is not below the pivot, so the reads carry no control dependence and issue early
-  // the L entries of step k are read PF steps ahead (independent of the chain), so the
-  // chain never waits for LDS
-  constexpr int PF = 6;
-  T pl0[PF], pl1[PF];
-#pragma unroll
-  for (int k = 0; k < PF; ++k) {
-    const int o = lu_loff(k) - k - 1;
-    pl0[k] = Ld<T, F>::get(sl, lane > k ? o + lane : 0);
-    pl1[k] = Ld<T, F>::get(sl, (two && lane + 64 > k) ? o + 64 + lane : 0);
-  }
+  // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1.  Row
+  // 64+lane is below every k < 64, row lane is above every k >= 64, so per step only one
+  // of the two rows needs the "below the pivot" test.
 #pragma unroll
   for (int k = 0; k < NLOC - 1; ++k) {
-    const T l0 = pl0[k % PF], l1 = pl1[k % PF];
-    if (k + PF < NLOC - 1) {
-      const int kn = k + PF, o = lu_loff(kn) - kn - 1;
-      pl0[k % PF] = Ld<T, F>::get(sl, lane > kn ? o + lane : 0);
-      pl1[k % PF] = Ld<T, F>::get(sl, (two && lane + 64 > kn) ? o + 64 + lane : 0);
+    const int o = lu_loff(k) - k - 1;
+    if (k < 64) {
+      const T bk = bcast(x0, k);
+      const T l0 = Ld<T, F>::get(sl0, o), l1 = Ld<T, F>::get(sl1, o);
+      const T y0 = tfma(-bk, l0, x0);
+      x0 = lane > k ? y0 : x0;
+      x1 = tfma(-bk, l1, x1);  // lanes without a second row carry an unused value
+    } else {
+      const T bk = bcast(x1, k - 64);
+      const T l1 = Ld<T, F>::get(sl1, o);
+      const T y1 = tfma(-bk, l1, x1);
+      x1 = lane + 64 > k ? y1 : x1;
     }
-    const T bk = bcast(k < 64 ? x0 : x1, k & 63);
-    const bool a0 = lane > k, a1 = two && lane + 64 > k;
-    const T y0 = tfma(-bk, l0, x0), y1 = tfma(-bk, l1, x1);
-    x0 = a0 ? y0 : x0;
-    x1 = a1 ? y1 : x1;
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < PS::RU; ++q) s4[lane + 64 * q] = rU[q];
   __syncthreads();
   // upper sweep (dtrsm U): x_k /= U(k,k), then eliminate column k above the diagonal;
-  // U(r,k) at lu_uoff(k) + r
+  // U(r,k) at lu_uoff(k) + r.  Row lane is above every k >= 64, row 64+lane below every
+  // k < 64.
   // Diagonal of U and its reciprocal per lane, computed in parallel before the chain.  The
   // quotient x_k / U(k,k) is then q = x r, e = fma(-q, U, x), q' = fma(e, r, q): with r the
   // correctly rounded reciprocal this is the correctly rounded quotient (Markstein), i.e.
   // the same value as the division of the column-oriented dtrsm, in three dependent steps.
-  const T dg0 = Ld<T, F>::get(sl, lu_uoff(lane) + lane);
-  const T dg1 = two ? Ld<T, F>::get(sl, lu_uoff(64 + lane) + 64 + lane) : T(1);
+  const T dg0 = Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(lane) + lane);
+  const T dg1 = two ? Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(64 + lane) + 64 + lane) : T(1);
   const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
-  T pu0[PF], pu1[PF];
-#pragma unroll
-  for (int i = 0; i < PF; ++i) {
-    const int k = NLOC - 1 - i, o = lu_uoff(k);
-    pu0[i] = Ld<T, F>::get(sl, lane < k ? o + lane : 0);
-    pu1[i] = Ld<T, F>::get(sl, (two && lane + 64 < k) ? o + 64 + lane : 0);
-  }
 #pragma unroll
   for (int k = NLOC - 1; k >= 0; --k) {
-    const int i = NLOC - 1 - k;
-    const T u0 = pu0[i % PF], u1 = pu1[i % PF];
-    if (k - PF >= 0) {
-      const int kn = k - PF, o = lu_uoff(kn);
-      pu0[i % PF] = Ld<T, F>::get(sl, lane < kn ? o + lane : 0);
-      pu1[i % PF] = Ld<T, F>::get(sl, (two && lane + 64 < kn) ? o + 64 + lane : 0);
-    }
-    T q0 = x0 * rc0, q1 = x1 * rc1;
-    if constexpr (sizeof(T) == 8) {
-      q0 = tfma(tfma(-q0, dg0, x0), rc0, q0);
-      q1 = tfma(tfma(-q1, dg1, x1), rc1, q1);
-    }
-    const T xk = bcast(k < 64 ? q0 : q1, k & 63);
-    const bool a0 = lane < k, a1 = two && lane + 64 < k;
-    const T y0 = tfma(-xk, u0, x0), y1 = tfma(-xk, u1, x1);
-    if (k < 64) {
-      x0 = (lane == k) ? xk : (a0 ? y0 : x0);
-      x1 = a1 ? y1 : x1;
+    const int o = lu_uoff(k);
+    if (k >= 64) {
+      T q1 = x1 * rc1;
+      if constexpr (sizeof(T) == 8) q1 = tfma(tfma(-q1, dg1, x1), rc1, q1);
+      const T xk = bcast(q1, k - 64);
+      const T u0 = Ld<T, F>::get(sl0, o), u1 = Ld<T, F>::get(sl1, o);
+      x0 = tfma(-xk, u0, x0);  // every row lane < 64 <= k lies above the pivot
+      const T y1 = tfma(-xk, u1, x1);
+      x1 = (lane + 64 == k) ? xk : (lane + 64 < k ? y1 : x1);
     } else {
-      x0 = a0 ? y0 : x0;
-      x1 = (lane == k - 64) ? xk : (a1 ? y1 : x1);
+      T q0 = x0 * rc0;
+      if constexpr (sizeof(T) == 8) q0 = tfma(tfma(-q0, dg0, x0), rc0, q0);
+      const T xk = bcast(q0, k);
+      const T u0 = Ld<T, F>::get(sl0, o);
+      const T y0 = tfma(-xk, u0, x0);
+      x0 = (lane == k) ? xk : (lane < k ? y0 : x0);
     }
   }
   // u[UDOF] += damp * delta, p[PDOF] += damp * delta
