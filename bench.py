@@ -5,7 +5,9 @@ One step = one complete solve of the Oseen system with the operator resident in 
 the value-dependent set-up (batched LU of every 85x85 Vanka patch on levels 8^3..32^3 and
 the dense LU of the 4^3 level) followed by FGMRES from x = 0 until the restart-cycle
 relative residual drops below 1e-10, each FGMRES iteration preconditioned by one V(2,2)
-cycle.  ``value`` is seconds per solve (lower is better).
+cycle.  ``value`` is seconds per solve (lower is better) of the mixed-precision form (by
+default bf16 patch factors, FP32 smoother and level operators, FP64 FGMRES and coarse
+solve); the plain FP64 form is timed beside it.
 
 A separate profiling pass then times every hot-kernel launch with HIP events on the
 library's stream and divides the kernel's algorithmic bytes by its time (roofline), and
@@ -84,9 +86,9 @@ def run_gpu(prob, precision: int, steps: int, warmup: int, log):
 def measured_traffic(kernel: str, precision: str, n: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (calibrated against a copy kernel), for the workload it was measured on."""
-    if precision != "fp32" or n != 32:
+    if precision not in ("fp32", "bf16") or n != 32:   # the mixed paths share k_vanka_res<float>
         return None, None
-    path = ROOT / "profiles" / "k_vanka_res_oseen32_fp32_hbm_traffic.md"
+    path = ROOT / "profiles" / f"k_vanka_res_oseen32_{precision}_hbm_traffic.md"
     if not path.exists():
         return None, None
     for line in path.read_text().splitlines():
@@ -160,7 +162,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=32, help="cells per edge of the finest level")
-    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="fp32",
+    ap.add_argument("--precision", choices=["fp64", "fp32", "bf16"], default="bf16",
                     help="smoother/level-operator arithmetic of the measured solve (FGMRES is FP64)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")

@@ -72,7 +72,7 @@ typedef struct {
   int    sc_minit;
   int    sc_amg_prec_it;
   int    precision;                            /* pmg_precision                           */
-  int    bf16_refine_steps;                    /* FP32 refinement steps per BF16 local solve */
+  int    bf16_refine_steps;                    /* must be 0: no local refinement; FGMRES refines */
 } pmg_params;
 
 /*

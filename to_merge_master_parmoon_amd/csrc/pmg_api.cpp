@@ -802,6 +802,11 @@ static int check_params(const pmg_params *p, std::string &why) {
   if (p->sc_coarse_smoother_saddle != 17) { why = "only the dense coarse solve (17)"; return PMG_ERR_UNSUPPORTED; }
   if (p->precision < 0 || p->precision > 2) { why = "bad precision"; return PMG_ERR_INVALID_ARGUMENT; }
   if (p->sc_pre_smooth_saddle < 0 || p->sc_post_smooth_saddle < 0) { why = "negative smoothing steps"; return PMG_ERR_INVALID_ARGUMENT; }
+  if (p->bf16_refine_steps != 0) {
+    why = "local refinement steps are not provided (the outer FP64 FGMRES refines the solution)";
+    return PMG_ERR_UNSUPPORTED;
+  }
+  if (p->sc_gmres_restart <= 0 || p->sc_lin_maxit_saddle < 0) { why = "bad FGMRES restart/maxit"; return PMG_ERR_INVALID_ARGUMENT; }
   return PMG_OK;
 }
 
