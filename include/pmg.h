@@ -194,6 +194,37 @@ int  pmg_fgmres_solve(pmg_ctx *ctx, double *sol, const double *rhs, int *iters,
 int  pmg_fgmres_history(const pmg_ctx *ctx, double *out, int cap);
 
 /* ----------------------------------------------------------------------------------- */
+/* batched solves of independent problems                                                */
+/* ----------------------------------------------------------------------------------- */
+/* One independent problem of a batch: its level hierarchy (coarsest first, at least two
+   levels, descriptors as for pmg_level_add) and its own INTERNAL_PROJECT_PRESSURE (1 for
+   an enclosed flow, 0 when the velocity has a natural boundary). */
+typedef struct {
+  int n_levels;
+  const pmg_level_desc *levels;
+  int project_pressure;
+} pmg_instance_desc;
+
+/* Replaces: one TSystemNSE3D solve per problem (src/System/SystemNSE3D.C:1199-1235), run
+   for many problems of different sizes at once.  The hierarchies are merged level by
+   level, aligned at the finest level, into one block-diagonal hierarchy: every smoothing
+   sweep, defect and transfer is one launch for all problems, each problem is solved
+   directly on its own coarsest level (TNSE_MGLevel4::SolveExact), and each problem keeps
+   its own IntoL20 and FGMRES scalars.  The returned context accepts pmg_setup,
+   pmg_refactor, pmg_set_profiling, pmg_timers_*, pmg_synchronize and pmg_ctx_destroy. */
+int  pmg_batch_create(int device, const pmg_params *params, int n_instances,
+                      const pmg_instance_desc *instances, pmg_ctx **out);
+/* Replaces: TFgmresIte::Iterate (src/FE/FgmresIte.C:250-677) once per problem, all problems
+   stepped together.  sol[i] / rhs[i] are problem i's vectors in its own layout (n_dof of
+   its finest level).  iters[i], final_res[i] and status[i] are what pmg_fgmres_solve
+   would report for problem i alone; the call returns PMG_OK when every status is PMG_OK,
+   else the first failing status. */
+int  pmg_batch_fgmres_solve(pmg_ctx *ctx, double *const *sol, const double *const *rhs,
+                            int *iters, double *final_res, int *status, int mem);
+/* number of problems of a batch context (0 for a single-problem context) */
+int  pmg_batch_size(const pmg_ctx *ctx);
+
+/* ----------------------------------------------------------------------------------- */
 /* coloring                                                                              */
 /* ----------------------------------------------------------------------------------- */
 /* Replaces: TParFEMapper3D::colorCell (src/Parallel/ParFEMapper3D.C:750-1084): cells are

@@ -60,6 +60,10 @@ class PmgLevelDesc(C.Structure):
     ]
 
 
+class PmgInstanceDesc(C.Structure):
+    _fields_ = [("n_levels", C.c_int), ("levels", C.POINTER(PmgLevelDesc)), ("project_pressure", C.c_int)]
+
+
 class PmgKstat(C.Structure):
     _fields_ = [("ms", C.c_double), ("calls", C.c_longlong), ("bytes", C.c_double), ("flops", C.c_double)]
 
@@ -114,6 +118,11 @@ PMG_FUNCS = {
     "pmg_vcycle": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_fgmres_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, c_int_p, c_double_p, C.c_int]),
     "pmg_fgmres_history": (C.c_int, [C.c_void_p, c_double_p, C.c_int]),
+    "pmg_batch_create": (C.c_int, [C.c_int, C.POINTER(PmgParams), C.c_int, C.POINTER(PmgInstanceDesc),
+                                   C.POINTER(C.c_void_p)]),
+    "pmg_batch_fgmres_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), c_int_p,
+                                         c_double_p, c_int_p, C.c_int]),
+    "pmg_batch_size": (C.c_int, [C.c_void_p]),
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "pmg_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -528,7 +528,7 @@ void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s) {
 
 void coarse_lu_factor(Coarse &C, hipStream_t s) {
   const int n = C.n;
-  const int pr_max = 6;
+  const int pr_max = 7;   // PR = 7 spills 16 VGPRs to scratch; 8 would spill ~1000
   if ((n + PNT - 1) / PNT > pr_max)
     throw DeviceError("coarse system of " + std::to_string(n) + " unknowns exceeds the dense LU limit (" +
                       std::to_string(pr_max * PNT) + ")");
@@ -543,7 +543,8 @@ void coarse_lu_factor(Coarse &C, hipStream_t s) {
       case 3: k_lu_panel<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
       case 4: k_lu_panel<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
       case 5: k_lu_panel<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      default: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      case 6: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      default: k_lu_panel<7><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
     }
     PMG_HIP_CHECK(hipGetLastError());
     if (j1 < n) {

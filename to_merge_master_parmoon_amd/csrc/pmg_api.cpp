@@ -16,9 +16,22 @@
 #include <string>
 #include <vector>
 
+#include "batch_merge.hpp"
 #include "pmg_internal.hpp"
 
 using namespace pmg;
+
+// One problem of a batch context: its own coarsest level (the operator of its dense coarse
+// system), the coarse factors, and the maps from its numbering into the merged levels.
+struct BatchProblem {
+  int n_levels = 0, clevel = 0, project = 0;
+  int n_dof_fine = 0, n_dof_coarse = 0;
+  DevLevel cl;
+  Coarse co;
+  int *cidx = nullptr, *fidx = nullptr;    // problem dof -> merged index (coarsest, finest level)
+  double *cf = nullptr, *cu = nullptr;     // FP64 coarse right-hand side and solution
+  double *io = nullptr;                    // staging of one problem vector (host memory calls)
+};
 
 struct pmg_ctx {
   int device = 0;
@@ -47,6 +60,18 @@ struct pmg_ctx {
   pmg_timers tm{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<void *> allocs;
+  // batch context (pmg_batch_create): the levels are merged block-diagonal levels
+  bool batch = false;
+  std::vector<BatchProblem> bi;
+  int4 *seg_ch = nullptr;                  // FGMRES chunk table over the merged finest vector
+  int n_seg_ch = 0;
+  int *seg_cptr = nullptr;                 // chunks of problem b: [seg_cptr[b], seg_cptr[b+1])
+  double *seg_part = nullptr;              // one partial sum per chunk
+  double *bX = nullptr, *bB = nullptr;     // merged finest solution and right-hand side
+  double *bH = nullptr;                    // per problem: MGS coefficients of one iteration
+  double *bcoef = nullptr;                 // per problem scalars / combination coefficients
+  int *bmask = nullptr;
+  int b_restart_alloc = 0;
 };
 
 namespace {
@@ -122,7 +147,8 @@ struct RowBytes {
 // ------------------------------------------------------------------ transfers
 // P (fine rows) accumulated cellwise from QQ and divided by the fine-dof multiplicity, as
 // Prolongate does (MGComponents3D.C:468-708); R = P^T, which is what DefectRestriction
-// computes (MGComponents3D.C:996-1240).
+// computes (MGComponents3D.C:996-1240).  Cells without a parent contribute nothing: their
+// rows of P stay empty.
 void build_transfer(pmg_ctx *c, int nloc_f, int nloc_c, int n_fine_cells, const int *parent, const int *child,
                     const double *qq, const std::vector<int> &fglob, const std::vector<int> &cglob, int n_fdof,
                     int n_cdof, Csr &P, Csr &R) {
@@ -135,6 +161,7 @@ void build_transfer(pmg_ctx *c, int nloc_f, int nloc_c, int n_fine_cells, const 
   std::vector<int> mult(n_fdof, 0);
   for (int fc = 0; fc < n_fine_cells; ++fc) {
     const int pc = parent[fc], ch = child[fc];
+    if (pc < 0) continue;   // merged levels: a problem solved directly on this level
     const double *Q = qq + size_t(ch) * nloc_f * nloc_c;
     const int *FineDOF = &fglob[size_t(fc) * nloc_f];
     const int *CoarseDOF = &cglob[size_t(pc) * nloc_c];
@@ -239,8 +266,15 @@ std::vector<int> build_patch_map(const pmg_level_desc *d, const std::vector<int>
 
 // Results reported by the device after the fact: the coarse LU (second stream) pivot check
 // and its time, and the wait-bound flag of the coarse triangular sweeps.
+void check_coarse(pmg_ctx *c, Coarse &C);
 void check_deferred(pmg_ctx *c) {
-  Coarse &C = c->coarse;
+  if (!c->batch) {
+    check_coarse(c, c->coarse);
+    return;
+  }
+  for (auto &b : c->bi) check_coarse(c, b.co);
+}
+void check_coarse(pmg_ctx *c, Coarse &C) {
   if (C.info_pending) {
     C.info_pending = false;
     int cinfo = 0;
@@ -277,6 +311,32 @@ void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
 
 bool mixed(const pmg_ctx *c) { return c->prm.precision != PMG_PREC_FP64; }
 
+// sizes, CSR structure and values (structure of arrays + row-blocked FP64, FP32 copy if
+// fp32) of a level's operator
+void upload_operator(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d, bool fp32) {
+  D.n_udof = d->n_udof; D.n_active = d->n_active; D.hanging_bound = d->hanging_bound;
+  D.n_dirichlet = d->n_dirichlet; D.n_pdof = d->n_pdof; D.n_dof = 3 * d->n_udof + d->n_pdof;
+  D.a_nnz = d->a_nnz; D.b_nnz = d->b_nnz; D.bt_nnz = d->bt_nnz;
+  D.a_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.a_rowptr, d->a_rowptr, D.n_udof + 1);
+  D.a_kcol = dalloc<int>(c, D.a_nnz); h2d(c, D.a_kcol, d->a_kcol, D.a_nnz);
+  D.b_rowptr = dalloc<int>(c, D.n_pdof + 1); h2d(c, D.b_rowptr, d->b_rowptr, D.n_pdof + 1);
+  D.b_kcol = dalloc<int>(c, D.b_nnz); h2d(c, D.b_kcol, d->b_kcol, D.b_nnz);
+  D.bt_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.bt_rowptr, d->bt_rowptr, D.n_udof + 1);
+  D.bt_kcol = dalloc<int>(c, D.bt_nnz); h2d(c, D.bt_kcol, d->bt_kcol, D.bt_nnz);
+  D.a = dalloc<double>(c, 9 * size_t(D.a_nnz));
+  D.b = dalloc<double>(c, 3 * size_t(D.b_nnz));
+  D.bt = dalloc<double>(c, 3 * size_t(D.bt_nnz));
+  D.av64 = dalloc<double>(c, 9 * size_t(D.a_nnz));
+  D.bv64 = dalloc<double>(c, 3 * size_t(D.b_nnz));
+  D.btv64 = dalloc<double>(c, 3 * size_t(D.bt_nnz));
+  if (fp32) {
+    D.av32 = dalloc<float>(c, 9 * size_t(D.a_nnz));
+    D.bv32 = dalloc<float>(c, 3 * size_t(D.b_nnz));
+    D.btv32 = dalloc<float>(c, 3 * size_t(D.bt_nnz));
+  }
+  upload_values(c, D, d);
+}
+
 void ensure_stage(pmg_ctx *c, long n) {
   if (n <= c->stage_len) return;
   for (auto &p : c->stage) {
@@ -293,9 +353,19 @@ void correct_nodes(pmg_ctx *c, const DevLevel &D, double *u) {
 void set_dirichlet(pmg_ctx *c, const DevLevel &D, double *u, const double *f) {
   launch_dirichlet_slices<double>(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, c->stream);
 }
+// IntoL20 of a level's pressure constants: for a single problem it is folded into the
+// type-4 launch; a merged level projects every problem's block on its own
+bool fused_l20(const pmg_ctx *c) { return !c->batch && c->prm.internal_project_pressure; }
+template <typename T>
+void project_pressure(pmg_ctx *c, const DevLevel &D, T *x) {
+  if (c->batch)
+    launch_l20_seg<T>(x + 3L * D.n_udof, D.l20_seg, D.n_l20_seg, c->stream);
+  else if (c->prm.internal_project_pressure)
+    launch_l20<T>(x + 3L * D.n_udof, D.n_pdof, c->red, c->stream);
+}
 void correct_defect(pmg_ctx *c, const DevLevel &D, double *d) {
   launch_dirichlet_slices<double>(d, nullptr, D.n_udof, D.n_active, D.n_udof - D.n_active, c->stream);
-  if (c->prm.internal_project_pressure) launch_l20<double>(d + 3L * D.n_udof, D.n_pdof, c->red, c->stream);
+  project_pressure<double>(c, D, d);
 }
 template <typename T>
 void correct_nodes_t(pmg_ctx *c, const DevLevel &D, T *u) {
@@ -308,7 +378,7 @@ void set_dirichlet_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
 template <typename T>
 void correct_defect_t(pmg_ctx *c, const DevLevel &D, T *d) {
   launch_dirichlet_slices<T>(d, nullptr, D.n_udof, D.n_active, D.n_udof - D.n_active, c->stream);
-  if (c->prm.internal_project_pressure) launch_l20<T>(d + 3L * D.n_udof, D.n_pdof, c->red, c->stream);
+  project_pressure<T>(c, D, d);
 }
 
 template <typename T>
@@ -367,10 +437,12 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
   }
 }
 
+// res_dev (single problem only): ||d||_2 after the projection
 template <typename T>
 void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, double *res_dev) {
   Timed tm(c, &c->tm.defect, D.defect_bytes[sizeof(T) == 8 ? 0 : 1], 0.0);
-  launch_type4<T>(opview<T>(D), 2, u, f, d, c->prm.internal_project_pressure != 0, res_dev, c->red, c->stream);
+  launch_type4<T>(opview<T>(D), 2, u, f, d, fused_l20(c), c->batch ? nullptr : res_dev, c->red, c->stream);
+  if (c->batch) project_pressure<T>(c, D, d);
 }
 
 template <typename T>
@@ -404,6 +476,23 @@ void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
   }
 }
 
+// the direct solves (SolveExact) of the problems whose coarsest level is this merged level
+template <typename T>
+void batch_coarse(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
+  for (int k : D.coarse_insts) {
+    BatchProblem &b = c->bi[k];
+    if (b.co.pending) {
+      PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, b.co.ev_done, 0));
+      b.co.pending = false;
+    }
+    const double n = b.co.n;
+    Timed tm(c, &c->tm.coarse_solve, n * n * 8.0, 2.0 * n * n);
+    launch_gather_d<T>(b.cf, f, b.cidx, b.n_dof_coarse, c->stream);
+    coarse_lu_solve(b.co, b.cl.op64(), b.cf, b.cu, c->stream);
+    launch_scatter_d<T>(u, b.cu, b.cidx, b.n_dof_coarse, c->stream);
+  }
+}
+
 template <typename T>
 struct Vec {
   static T *sol(DevLevel &D) { return D.sol; }
@@ -431,9 +520,16 @@ void cycle(pmg_ctx *c, int i) {
   correct_nodes_t<T>(c, D, u);
   set_dirichlet_t<T>(c, D, u, f);
   if (i == 0) {
-    coarse_solve_t<T>(c, D, u, f);
+    if (c->batch)
+      batch_coarse<T>(c, D, u, f);
+    else
+      coarse_solve_t<T>(c, D, u, f);
     return;
   }
+  // merged levels: problems whose coarsest level this is are solved here; their cells are
+  // not in the color lists and they have no coarser level, so the rest of the cycle leaves
+  // them unchanged (their prolongated correction rows are empty)
+  if (!D.coarse_insts.empty()) batch_coarse<T>(c, D, u, f);
   DevLevel &C = c->L[i - 1];
   for (int j = 0; j < c->prm.sc_pre_smooth_saddle; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D));
   correct_nodes_t<T>(c, D, u);
@@ -471,55 +567,77 @@ void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
 // ------------------------------------------------------------------ set-up
 // value-dependent set-up: the dense coarse LU runs on a second stream while the batched
 // patch LUs of all finer levels occupy the main stream
+// dense coarse system of nr unknowns: allocated on first use or when its size changes
+void coarse_alloc(pmg_ctx *c, Coarse &C, int nr) {
+  if (C.a && C.n != nr) {
+    for (void *p : {(void *)C.a, (void *)C.f, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y,
+                    (void *)C.x, (void *)C.ubuf, (void *)C.sync, (void *)C.flags})
+      dfree(c, p);
+    C.a = nullptr;
+  }
+  if (!C.a) {
+    C.n = nr;
+    C.a = dalloc<double>(c, size_t(C.n) * C.n);
+    C.f = dalloc<double>(c, size_t(C.n) * C.n);
+    C.ipiv = dalloc<int>(c, C.n);
+    C.sigma = dalloc<int>(c, C.n);
+    C.b = dalloc<double>(c, C.n);
+    C.y = dalloc<double>(c, C.n);
+    C.x = dalloc<double>(c, C.n);
+    C.ubuf = dalloc<double>(c, size_t(C.n) * 16);
+    const int nblk = (C.n + 63) / 64;
+    C.sync = dalloc<unsigned>(c, 4);
+    C.flags = dalloc<unsigned>(c, 2 * size_t(nblk));
+    PMG_HIP_CHECK(hipMemsetAsync(C.sync, 0, 4 * sizeof(unsigned), c->stream2));
+    PMG_HIP_CHECK(hipMemsetAsync(C.flags, 0, 2 * size_t(nblk) * sizeof(unsigned), c->stream2));
+    if (!C.err) {
+      C.err = dalloc<int>(c, 1);
+      PMG_HIP_CHECK(hipMemsetAsync(C.err, 0, sizeof(int), c->stream2));
+    }
+    if (!C.info) C.info = dalloc<int>(c, 1);
+  }
+  if (!C.ev_start) {
+    PMG_HIP_CHECK(hipEventCreate(&C.ev_start));
+    PMG_HIP_CHECK(hipEventCreate(&C.ev_done));
+  }
+}
+
+// assemble and factor the reduced coarse system of level D on the second stream
+void start_coarse_factor(pmg_ctx *c, const DevLevel &D, Coarse &C, int pin) {
+  coarse_alloc(c, C, 3 * D.n_active + D.n_pdof - pin);
+  C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;
+  PMG_HIP_CHECK(hipEventRecord(C.ev_start, c->stream2));
+  launch_coarse_build(D, C, c->stream2);
+  coarse_lu_factor(C, c->stream2);
+  PMG_HIP_CHECK(hipEventRecord(C.ev_done, c->stream2));
+  C.pending = true;
+  C.info_pending = true;
+}
+
 void setup_factors(pmg_ctx *c) {
   check_deferred(c);
   auto t0 = std::chrono::steady_clock::now();
-  const bool do_coarse = !c->coarse.ready;
-  Coarse &C = c->coarse;
-  if (do_coarse) {
-    DevLevel &D = c->L[0];
-    const int pin = c->prm.internal_project_pressure ? 1 : 0;
-    const int nr = 3 * D.n_active + D.n_pdof - pin;
-    if (C.a && C.n != nr) {
-      for (void *p : {(void *)C.a, (void *)C.f, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y,
-                      (void *)C.x, (void *)C.ubuf, (void *)C.sync, (void *)C.flags})
-        dfree(c, p);
-      C.a = nullptr;
+  std::vector<Coarse *> todo;
+  std::vector<const DevLevel *> todo_lvl;
+  std::vector<int> todo_pin;
+  if (!c->batch) {
+    if (!c->coarse.ready) {
+      todo.push_back(&c->coarse);
+      todo_lvl.push_back(&c->L[0]);
+      todo_pin.push_back(c->prm.internal_project_pressure ? 1 : 0);
     }
-    if (!C.a) {
-      C.n = nr;
-      C.a = dalloc<double>(c, size_t(C.n) * C.n);
-      C.f = dalloc<double>(c, size_t(C.n) * C.n);
-      C.ipiv = dalloc<int>(c, C.n);
-      C.sigma = dalloc<int>(c, C.n);
-      C.b = dalloc<double>(c, C.n);
-      C.y = dalloc<double>(c, C.n);
-      C.x = dalloc<double>(c, C.n);
-      C.ubuf = dalloc<double>(c, size_t(C.n) * 16);
-      const int nblk = (C.n + 63) / 64;
-      C.sync = dalloc<unsigned>(c, 4);
-      C.flags = dalloc<unsigned>(c, 2 * size_t(nblk));
-      PMG_HIP_CHECK(hipMemsetAsync(C.sync, 0, 4 * sizeof(unsigned), c->stream2));
-      PMG_HIP_CHECK(hipMemsetAsync(C.flags, 0, 2 * size_t(nblk) * sizeof(unsigned), c->stream2));
-      if (!C.err) {
-        C.err = dalloc<int>(c, 1);
-        PMG_HIP_CHECK(hipMemsetAsync(C.err, 0, sizeof(int), c->stream2));
+  } else {
+    for (auto &b : c->bi)
+      if (!b.co.ready) {
+        todo.push_back(&b.co);
+        todo_lvl.push_back(&b.cl);
+        todo_pin.push_back(b.project);
       }
-      if (!C.info) C.info = dalloc<int>(c, 1);
-    }
-    C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;
-    if (!C.ev_start) {
-      PMG_HIP_CHECK(hipEventCreate(&C.ev_start));
-      PMG_HIP_CHECK(hipEventCreate(&C.ev_done));
-    }
+  }
+  if (!todo.empty()) {
     PMG_HIP_CHECK(hipEventRecord(c->evs, c->stream));
     PMG_HIP_CHECK(hipStreamWaitEvent(c->stream2, c->evs, 0));
-    PMG_HIP_CHECK(hipEventRecord(C.ev_start, c->stream2));
-    launch_coarse_build(D, C, c->stream2);
-    coarse_lu_factor(C, c->stream2);
-    PMG_HIP_CHECK(hipEventRecord(C.ev_done, c->stream2));
-    C.pending = true;
-    C.info_pending = true;
+    for (size_t k = 0; k < todo.size(); ++k) start_coarse_factor(c, *todo_lvl[k], *todo[k], todo_pin[k]);
   }
   std::vector<std::vector<int>> infos(c->L.size());
   for (size_t l = 1; l < c->L.size(); ++l) {
@@ -546,7 +664,7 @@ void setup_factors(pmg_ctx *c) {
     D.factors_ready = true;
   }
   c->tm.patch_lu.ms += t_patch;
-  if (do_coarse) C.ready = true;   // usable once ev_done completes; checked in check_deferred
+  for (Coarse *C : todo) C->ready = true;   // usable once ev_done completes; checked in check_deferred
   c->setup_done = true;
 }
 
@@ -584,7 +702,8 @@ double dev_norm2(pmg_ctx *c, const double *x, long n) {
 void defect_nse4_dev(pmg_ctx *c, const double *x, const double *b, double *r) {
   DevLevel &F = c->L.back();
   Timed tm(c, &c->tm.matvec, F.defect_bytes[0], 0.0);
-  launch_type4<double>(F.op64(), 1, x, b, r, c->prm.internal_project_pressure != 0, nullptr, c->red, c->stream);
+  launch_type4<double>(F.op64(), 1, x, b, r, fused_l20(c), nullptr, c->red, c->stream);
+  if (c->batch) project_pressure<double>(c, F, r);
 }
 
 // GeneratePlaneRotation / ApplyPlaneRotation (FgmresIte.C:199-226)
@@ -707,6 +826,206 @@ int fgmres_dev(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *f
   *iters = j; *final_res = res;
   if (res > res_norm_min && res > res0 * red_factor) return PMG_ERR_NOT_CONVERGED;
   return PMG_OK;
+}
+
+// ------------------------------------------------------------------ batched FGMRES
+void ensure_hpin(pmg_ctx *c, long n) {
+  if (n <= c->hpin_len) return;
+  if (c->hpin) (void)hipHostFree(c->hpin);
+  c->hpin = nullptr;
+  PMG_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&c->hpin), sizeof(double) * n));
+  c->hpin_len = int(n);
+}
+
+// per-problem <x, y> -> out[b * ldo] (device), deterministic
+void seg_dot(pmg_ctx *c, const double *x, const double *y, double *out, int ldo) {
+  launch_seg_dot(x, y, c->seg_ch, c->n_seg_ch, c->seg_part, c->stream);
+  launch_seg_reduce(c->seg_part, c->seg_cptr, int(c->bi.size()), out, ldo, c->stream);
+}
+
+struct BState {
+  int st = PMG_OK;
+  bool active = true;
+  int it = 0;
+  double res = 0, res0 = 0, beta = 0;
+  std::vector<std::vector<double>> H;
+  std::vector<double> s, cs, sn;
+};
+
+// UpdateGmresIterate back substitution (FgmresIte.C:228-247): y (in s) for columns 0..k
+void back_solve(std::vector<std::vector<double>> &H, std::vector<double> &s, int k) {
+  for (int i = k; i >= 0; i--) {
+    s[i] /= H[i][i];
+    for (int j = i - 1; j >= 0; j--) s[j] -= H[j][i] * s[i];
+  }
+}
+
+// TFgmresIte::Iterate (FgmresIte.C:250-677) for every problem of a batch at once: the
+// problems share the iteration counter j and the restart cycle; the Arnoldi scalars,
+// Givens rotations, stopping and divergence tests are each problem's own.  A problem that
+// stops gets zero basis vectors from then on (its V-cycle input is zero, so is the output).
+void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &final_res, std::vector<int> &status) {
+  ensure_fgmres(c);
+  const pmg_params &P = c->prm;
+  DevLevel &F = c->L.back();
+  const long n = F.n_dof;
+  const int nb = int(c->bi.size());
+  int maxit = P.sc_lin_maxit_saddle;
+  int restart = P.sc_gmres_restart;
+  if (restart > maxit) restart = maxit;
+  if (restart <= 0) restart = 10;
+  const double res_norm_min = P.sc_lin_res_norm_min_saddle, red_factor = P.sc_lin_red_factor_saddle;
+  const int ldh = restart + 2, ldc = restart + 1;
+  if (restart > c->b_restart_alloc) {
+    dfree(c, c->bH);
+    dfree(c, c->bcoef);
+    dfree(c, c->bmask);
+    c->bH = dalloc<double>(c, size_t(nb) * ldh);
+    c->bcoef = dalloc<double>(c, size_t(nb) * (ldc + 2));
+    c->bmask = dalloc<int>(c, nb);
+    c->b_restart_alloc = restart;
+  }
+  ensure_hpin(c, long(nb) * (ldh + ldc + 4));
+  double *hH = c->hpin, *hS = hH + size_t(nb) * ldh, *hC = hS + nb, *hN = hC + size_t(nb) * ldc;
+  int *hM = reinterpret_cast<int *>(hN + nb);
+  double *dS = c->bcoef, *dC = dS + nb, *dN = dC + size_t(nb) * ldc;
+  auto v = [&](int i) { return c->V + size_t(i) * n; };
+  auto z = [&](int i) { return c->Z + size_t(i) * n; };
+  std::vector<BState> B(nb);
+  for (auto &b : B) {
+    b.H.assign(restart + 1, std::vector<double>(restart, 0.0));
+    b.s.assign(restart + 1, 0.0);
+    b.cs.assign(restart + 1, 0.0);
+    b.sn.assign(restart + 1, 0.0);
+  }
+  auto norms = [&](const double *x) {
+    seg_dot(c, x, x, dN, 1);
+    PMG_HIP_CHECK(hipMemcpyAsync(hN, dN, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  };
+  auto any_active = [&] {
+    for (auto &b : B)
+      if (b.active) return true;
+    return false;
+  };
+  // x_b += sum_{k <= kmax} y_bk z_k for the problems in hM
+  auto combine = [&](int m) {
+    PMG_HIP_CHECK(hipMemcpyAsync(dC, hC, sizeof(double) * nb * ldc, hipMemcpyHostToDevice, c->stream));
+    PMG_HIP_CHECK(hipMemcpyAsync(c->bmask, hM, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    launch_seg_lincomb(c->bX, c->Z, n, dC, ldc, m, c->bmask, c->seg_ch, c->n_seg_ch, c->stream);
+  };
+  defect_nse4_dev(c, c->bX, c->bB, v(0));
+  norms(v(0));
+  for (int b = 0; b < nb; ++b) {
+    BState &S = B[b];
+    S.res = S.res0 = S.beta = std::sqrt(hN[b]);
+    if (!std::isfinite(S.res)) {
+      S.st = PMG_ERR_NONFINITE;
+      S.active = false;
+    } else if (S.beta <= res_norm_min || S.beta == 0.0) {
+      S.active = false;   // sc_minit == 0: nothing to do (FgmresIte.C:330-337)
+    }
+  }
+  int j = 1;
+  while (j <= maxit && any_active()) {
+    for (int b = 0; b < nb; ++b) hS[b] = B[b].active ? 1.0 / B[b].beta : 0.0;
+    PMG_HIP_CHECK(hipMemcpyAsync(dS, hS, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+    launch_seg_scale(v(0), v(0), dS, c->seg_ch, c->n_seg_ch, c->stream);
+    for (auto &S : B) {
+      std::fill(S.s.begin(), S.s.end(), 0.0);
+      S.s[0] = S.beta;
+    }
+    for (int i = 0; i < restart && j <= maxit && any_active(); i++, j++) {
+      for (int k = 0; k < std::max(1, P.sc_amg_prec_it); ++k) vcycle_dev(c, v(i), z(i));
+      {
+        Timed tm(c, &c->tm.matvec, F.matvec_bytes, 0.0);
+        launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
+      }
+      // modified Gram-Schmidt: H[k][i] -> bH[b*ldh + k], ||w||^2 -> bH[b*ldh + i + 1]
+      seg_dot(c, c->W, v(0), c->bH, ldh);
+      for (int k = 0; k <= i; ++k) {
+        const double *vn = (k < i) ? v(k + 1) : c->W;
+        launch_seg_axpy_dot(c->W, c->bH + k, ldh, v(k), vn, c->seg_ch, c->n_seg_ch, c->seg_part, c->stream);
+        launch_seg_reduce(c->seg_part, c->seg_cptr, nb, c->bH + k + 1, ldh, c->stream);
+      }
+      PMG_HIP_CHECK(hipMemcpyAsync(hH, c->bH, sizeof(double) * nb * ldh, hipMemcpyDeviceToHost, c->stream));
+      PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+      bool finish = false;
+      std::fill(hM, hM + nb, 0);
+      std::fill(hC, hC + size_t(nb) * ldc, 0.0);
+      for (int b = 0; b < nb; ++b) {
+        BState &S = B[b];
+        hS[b] = 0.0;
+        if (!S.active) continue;
+        auto &H = S.H;
+        for (int k = 0; k <= i; ++k) H[k][i] = hH[size_t(b) * ldh + k];
+        H[i + 1][i] = std::sqrt(hH[size_t(b) * ldh + i + 1]);
+        for (int k = 0; k < i; k++) app_rot(H[k][i], H[k + 1][i], S.cs[k], S.sn[k]);
+        gen_rot(H[i][i], H[i + 1][i], S.cs[i], S.sn[i]);
+        const double hnext = H[i + 1][i];   // before the rotation zeroes it
+        app_rot(H[i][i], H[i + 1][i], S.cs[i], S.sn[i]);
+        app_rot(S.s[i], S.s[i + 1], S.cs[i], S.sn[i]);
+        S.res = std::fabs(S.s[i + 1]);
+        S.it = j;
+        if (!std::isfinite(S.res)) {
+          S.st = PMG_ERR_NONFINITE;
+          S.active = false;
+        } else if (S.res < res_norm_min || S.res < red_factor * S.res0) {
+          back_solve(H, S.s, i);
+          for (int k = 0; k <= i; ++k) hC[size_t(b) * ldc + k] = S.s[k];
+          hM[b] = 1;
+          finish = true;
+          S.st = PMG_OK;
+          S.active = false;
+        } else if (S.res > P.sc_div_factor * S.res0) {
+          S.st = PMG_ERR_DIVERGED;
+          S.active = false;
+        } else {
+          hS[b] = hnext != 0 ? 1.0 / hnext : 1.0;
+        }
+      }
+      // v_{i+1} = w / H[i+1][i] (a zero norm copies w, as the single-problem path does)
+      PMG_HIP_CHECK(hipMemcpyAsync(dS, hS, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
+      launch_seg_scale(v(i + 1), c->W, dS, c->seg_ch, c->n_seg_ch, c->stream);
+      if (finish) combine(i + 1);
+    }
+    if (!any_active()) break;
+    // end of a restart cycle (or maxit): update with all restart columns, new residual
+    std::fill(hM, hM + nb, 0);
+    std::fill(hC, hC + size_t(nb) * ldc, 0.0);
+    for (int b = 0; b < nb; ++b) {
+      BState &S = B[b];
+      if (!S.active) continue;
+      back_solve(S.H, S.s, restart - 1);
+      for (int k = 0; k < restart; ++k) hC[size_t(b) * ldc + k] = S.s[k];
+      hM[b] = 1;
+    }
+    combine(restart);
+    defect_nse4_dev(c, c->bX, c->bB, v(0));
+    norms(v(0));
+    for (int b = 0; b < nb; ++b) {
+      BState &S = B[b];
+      if (!S.active) continue;
+      S.res = S.res0 = S.beta = std::sqrt(hN[b]);
+      if (!std::isfinite(S.res)) {
+        S.st = PMG_ERR_NONFINITE;
+        S.active = false;
+      } else if (S.beta == 0.0) {
+        S.active = false;
+      }
+    }
+  }
+  for (int b = 0; b < nb; ++b) {
+    BState &S = B[b];
+    if (S.active) {
+      S.it = std::min(j, maxit);
+      if (S.res > res_norm_min && S.res > S.res0 * red_factor) S.st = PMG_ERR_NOT_CONVERGED;
+    }
+    iters[b] = S.it;
+    final_res[b] = S.res;
+    status[b] = S.st;
+  }
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
 int guard(pmg_ctx *c, const std::function<void()> &f) {
@@ -853,6 +1172,10 @@ int pmg_ctx_destroy(pmg_ctx *c) {
   if (c->evs) (void)hipEventDestroy(c->evs);
   if (c->coarse.ev_start) (void)hipEventDestroy(c->coarse.ev_start);
   if (c->coarse.ev_done) (void)hipEventDestroy(c->coarse.ev_done);
+  for (auto &b : c->bi) {
+    if (b.co.ev_start) (void)hipEventDestroy(b.co.ev_start);
+    if (b.co.ev_done) (void)hipEventDestroy(b.co.ev_done);
+  }
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -876,8 +1199,9 @@ int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
 int pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_global, int n_udof, int *n_colors,
                     int *ptr_colors, int *cell_reorder);
 
-int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
-  if (!c || !d) return PMG_ERR_INVALID_ARGUMENT;
+// merged: a level built by merge_hierarchies (cells without parent allowed, the color
+// lists may leave out the cells of problems solved directly on this level)
+static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool merged) {
   return guard(c, [&] {
     PMG_HIP_CHECK(hipSetDevice(c->device));
     if (d->n_ulocal != NU_LOC || d->n_plocal != NP_LOC)
@@ -904,28 +1228,8 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     check_csr(d->bt_rowptr, d->bt_kcol, d->n_udof, d->n_pdof, "BT");
     c->L.emplace_back();
     DevLevel &D = c->L.back();
-    D.n_udof = d->n_udof; D.n_active = d->n_active; D.hanging_bound = d->hanging_bound;
-    D.n_dirichlet = d->n_dirichlet; D.n_pdof = d->n_pdof; D.n_dof = 3 * d->n_udof + d->n_pdof;
+    upload_operator(c, D, d, mixed(c));
     D.n_cells = d->n_cells;
-    D.a_nnz = d->a_nnz; D.b_nnz = d->b_nnz; D.bt_nnz = d->bt_nnz;
-    D.a_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.a_rowptr, d->a_rowptr, D.n_udof + 1);
-    D.a_kcol = dalloc<int>(c, D.a_nnz); h2d(c, D.a_kcol, d->a_kcol, D.a_nnz);
-    D.b_rowptr = dalloc<int>(c, D.n_pdof + 1); h2d(c, D.b_rowptr, d->b_rowptr, D.n_pdof + 1);
-    D.b_kcol = dalloc<int>(c, D.b_nnz); h2d(c, D.b_kcol, d->b_kcol, D.b_nnz);
-    D.bt_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.bt_rowptr, d->bt_rowptr, D.n_udof + 1);
-    D.bt_kcol = dalloc<int>(c, D.bt_nnz); h2d(c, D.bt_kcol, d->bt_kcol, D.bt_nnz);
-    D.a = dalloc<double>(c, 9 * size_t(D.a_nnz));
-    D.b = dalloc<double>(c, 3 * size_t(D.b_nnz));
-    D.bt = dalloc<double>(c, 3 * size_t(D.bt_nnz));
-    D.av64 = dalloc<double>(c, 9 * size_t(D.a_nnz));
-    D.bv64 = dalloc<double>(c, 3 * size_t(D.b_nnz));
-    D.btv64 = dalloc<double>(c, 3 * size_t(D.bt_nnz));
-    if (mixed(c)) {
-      D.av32 = dalloc<float>(c, 9 * size_t(D.a_nnz));
-      D.bv32 = dalloc<float>(c, 3 * size_t(D.b_nnz));
-      D.btv32 = dalloc<float>(c, 3 * size_t(D.bt_nnz));
-    }
-    upload_values(c, D, d);
     // dof maps (UGlobalNumbers + UBeginIndex[cell])
     std::vector<int> ug(size_t(D.n_cells) * NU_LOC), pg(size_t(D.n_cells) * NP_LOC);
     for (int cell = 0; cell < D.n_cells; ++cell) {
@@ -945,10 +1249,21 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
     // coloring (colorCell) - given or computed
     std::vector<int> ptr(D.n_cells + 1), reo(D.n_cells);
     int nc = 0;
-    if (d->ptr_colors && d->cell_reorder && d->n_colors > 0) {
+    if (d->ptr_colors && d->cell_reorder && (d->n_colors > 0 || merged)) {
       nc = d->n_colors;
+      if (nc < 0 || nc > D.n_cells) throw Fail{PMG_ERR_INVALID_ARGUMENT, "bad color count"};
       std::copy(d->ptr_colors, d->ptr_colors + nc + 1, ptr.begin());
       std::copy(d->cell_reorder, d->cell_reorder + D.n_cells, reo.begin());
+      // the color pointers must ascend and (single problems) cover every cell; the cell list
+      // must be a permutation
+      bool ok = ptr[0] == 0 && (merged ? ptr[nc] <= D.n_cells : ptr[nc] == D.n_cells);
+      for (int k = 0; ok && k < nc; ++k) ok = ptr[k] <= ptr[k + 1];
+      std::vector<char> seen(D.n_cells, 0);
+      for (int q = 0; ok && q < D.n_cells; ++q) {
+        ok = reo[q] >= 0 && reo[q] < D.n_cells && !seen[reo[q]];
+        if (ok) seen[reo[q]] = 1;
+      }
+      if (!ok) throw Fail{PMG_ERR_INVALID_ARGUMENT, "cell coloring is not a partition of the cells"};
     } else {
       std::vector<int> ub(D.n_cells);
       for (int i = 0; i < D.n_cells; ++i) ub[i] = i * NU_LOC;
@@ -1036,9 +1351,11 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
       if (mixed(c)) D.rloc32 = dalloc<float>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
       // transfers from the coarser level
       const DevLevel &C = c->L[lvl - 1];
-      for (int fc = 0; fc < D.n_cells; ++fc)
+        for (int fc = 0; fc < D.n_cells; ++fc) {
+        if (merged && d->parent[fc] == -1) continue;
         if (d->parent[fc] < 0 || d->parent[fc] >= C.n_cells || d->child[fc] < 0 || d->child[fc] >= d->n_children)
           throw Fail{PMG_ERR_INVALID_ARGUMENT, "parent/child map out of range"};
+      }
       build_transfer(c, NU_LOC, NU_LOC, D.n_cells, d->parent, d->child, d->qq_u, ug, c->h_u_global[lvl - 1],
                      D.n_udof, C.n_udof, D.pu, D.ru);
       build_transfer(c, NP_LOC, NP_LOC, D.n_cells, d->parent, d->child, d->qq_p, pg, c->h_p_global[lvl - 1],
@@ -1061,8 +1378,21 @@ int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
   });
 }
 
+int pmg_level_add(pmg_ctx *c, const pmg_level_desc *d, int *level_out) {
+  if (!c || !d) return PMG_ERR_INVALID_ARGUMENT;
+  if (c->batch) {
+    c->err = "levels of a batch context are fixed by pmg_batch_create";
+    return PMG_ERR_UNSUPPORTED;
+  }
+  return add_level(c, d, level_out, false);
+}
+
 int pmg_level_update_values(pmg_ctx *c, int level, const pmg_level_desc *d) {
   if (!c || !d) return PMG_ERR_INVALID_ARGUMENT;
+  if (c->batch) {
+    c->err = "new values for a batch: create a new batch context";
+    return PMG_ERR_UNSUPPORTED;
+  }
   return guard(c, [&] {
     DevLevel &D = level_of(c, level);
     if (d->a_nnz != D.a_nnz || d->b_nnz != D.b_nnz || d->bt_nnz != D.bt_nnz)
@@ -1090,6 +1420,7 @@ int pmg_refactor(pmg_ctx *c) {
     PMG_HIP_CHECK(hipSetDevice(c->device));
     for (auto &D : c->L) D.factors_ready = false;
     c->coarse.ready = false;
+    for (auto &b : c->bi) b.co.ready = false;
     setup_factors(c);
   });
 }
@@ -1115,10 +1446,11 @@ int pmg_defect(pmg_ctx *c, int level, const double *u, const double *f, double *
     HostIO io(c, mem, D.n_dof);
     double *du = io.in(0, u), *df = io.in(1, f), *dd = (mem == PMG_MEM_HOST) ? c->stage[2] : d;
     mg_defect<double>(c, D, du, df, dd, c->red.result);
+    if (c->batch) launch_dot(dd, dd, D.n_dof, c->red.result, c->red, c->stream);
     io.out(2, d);
     PMG_HIP_CHECK(hipMemcpyAsync(c->hpin, c->red.result, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     io.sync();
-    if (res) *res = c->hpin[0];
+    if (res) *res = c->batch ? std::sqrt(c->hpin[0]) : c->hpin[0];
   });
 }
 
@@ -1140,8 +1472,8 @@ int pmg_defect_nse4(pmg_ctx *c, int level, const double *x, const double *b, dou
     DevLevel &D = level_of(c, level);
     HostIO io(c, mem, D.n_dof);
     double *dx = io.in(0, x), *db = io.in(1, b), *dr = (mem == PMG_MEM_HOST) ? c->stage[2] : r;
-    launch_type4<double>(D.op64(), 1, dx, db, dr, c->prm.internal_project_pressure != 0, nullptr, c->red,
-                         c->stream);
+    launch_type4<double>(D.op64(), 1, dx, db, dr, fused_l20(c), nullptr, c->red, c->stream);
+    if (c->batch) project_pressure<double>(c, D, dr);
     io.out(2, r);
     io.sync();
   });
@@ -1255,7 +1587,10 @@ int pmg_coarse_solve(pmg_ctx *c, double *u, const double *f, int mem) {
     DevLevel &D = level_of(c, 0);
     HostIO io(c, mem, D.n_dof);
     double *du = (mem == PMG_MEM_HOST) ? c->stage[0] : u, *df = io.in(1, f);
-    coarse_solve_t<double>(c, D, du, df);
+    if (c->batch)
+      batch_coarse<double>(c, D, du, df);
+    else
+      coarse_solve_t<double>(c, D, du, df);
     io.out(0, u);
     io.sync();
     check_deferred(c);
@@ -1278,6 +1613,10 @@ int pmg_vcycle(pmg_ctx *c, const double *v, double *z, int mem) {
 
 int pmg_fgmres_solve(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *final_res, int mem) {
   if (!c || !sol || !rhs) return PMG_ERR_INVALID_ARGUMENT;
+  if (c->batch) {
+    c->err = "a batch context solves with pmg_batch_fgmres_solve";
+    return PMG_ERR_UNSUPPORTED;
+  }
   int it = 0;
   double fr = 0;
   int st = PMG_OK;
@@ -1296,6 +1635,134 @@ int pmg_fgmres_solve(pmg_ctx *c, double *sol, const double *rhs, int *iters, dou
   if (rc) return rc;
   if (st == PMG_ERR_NOT_CONVERGED) c->err = "FGMRES not converged";
   return st;
+}
+
+int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg_instance_desc *inst,
+                     pmg_ctx **out) {
+  if (!out || n_inst <= 0 || !inst) return PMG_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  pmg_ctx *c = nullptr;
+  if (int st = pmg_ctx_create(device, params, &c)) return st;
+  c->batch = true;
+  const int rc = guard(c, [&] {
+    if (c->prm.sc_minit != 0) throw Fail{PMG_ERR_UNSUPPORTED, "a batch needs sc_minit = 0"};
+    MergedHierarchy M;
+    try {
+      merge_hierarchies(n_inst, inst, M);
+    } catch (const Invalid &e) {
+      throw Fail{e.code, e.what};
+    }
+    for (size_t l = 0; l < M.L.size(); ++l) {
+      int lvl = -1;
+      if (int r = add_level(c, &M.L[l].d, &lvl, true)) throw Fail{r, c->err};
+      DevLevel &D = c->L[l];
+      D.coarse_insts = M.L[l].coarse;
+      D.n_l20_seg = int(M.L[l].l20.size() / 2);
+      if (D.n_l20_seg) {
+        D.l20_seg = dalloc<int>(c, M.L[l].l20.size());
+        h2d(c, D.l20_seg, M.L[l].l20.data(), M.L[l].l20.size());
+      }
+      PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+      M.L[l] = MergedLevel{};   // host copy no longer needed
+    }
+    c->bi.resize(n_inst);
+    std::vector<int4> chunks;
+    std::vector<int> cptr(1, 0);
+    for (int b = 0; b < n_inst; ++b) {
+      BatchProblem &B = c->bi[b];
+      B.n_levels = inst[b].n_levels;
+      B.clevel = M.clevel[b];
+      B.project = inst[b].project_pressure ? 1 : 0;
+      upload_operator(c, B.cl, &inst[b].levels[0], false);
+      B.n_dof_coarse = B.cl.n_dof;
+      B.n_dof_fine = int(M.fidx[b].size());
+      B.cidx = dalloc<int>(c, M.cidx[b].size());
+      h2d(c, B.cidx, M.cidx[b].data(), M.cidx[b].size());
+      B.fidx = dalloc<int>(c, M.fidx[b].size());
+      h2d(c, B.fidx, M.fidx[b].data(), M.fidx[b].size());
+      B.cf = dalloc<double>(c, B.n_dof_coarse);
+      B.cu = dalloc<double>(c, B.n_dof_coarse);
+      B.io = dalloc<double>(c, 2 * size_t(B.n_dof_fine));
+      for (const auto &r : M.fseg[b])
+        for (long s0 = r[0]; s0 < r[1]; s0 += SEG_CHUNK)
+          chunks.push_back(make_int4(int(s0), int(std::min<long>(SEG_CHUNK, r[1] - s0)), b, 0));
+      cptr.push_back(int(chunks.size()));
+      PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    c->n_seg_ch = int(chunks.size());
+    c->seg_ch = dalloc<int4>(c, chunks.size());
+    h2d(c, c->seg_ch, chunks.data(), chunks.size());
+    c->seg_cptr = dalloc<int>(c, cptr.size());
+    h2d(c, c->seg_cptr, cptr.data(), cptr.size());
+    c->seg_part = dalloc<double>(c, chunks.size());
+    const long n = c->L.back().n_dof;
+    c->bX = dalloc<double>(c, n);
+    c->bB = dalloc<double>(c, n);
+    launch_fill<double>(c->bX, 0.0, n, c->stream);
+    launch_fill<double>(c->bB, 0.0, n, c->stream);
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+  if (rc) {
+    pmg_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return PMG_OK;
+}
+
+int pmg_batch_size(const pmg_ctx *c) { return c && c->batch ? int(c->bi.size()) : 0; }
+
+int pmg_batch_fgmres_solve(pmg_ctx *c, double *const *sol, const double *const *rhs, int *iters, double *final_res,
+                           int *status, int mem) {
+  if (!c || !sol || !rhs) return PMG_ERR_INVALID_ARGUMENT;
+  if (!c->batch) {
+    c->err = "not a batch context";
+    return PMG_ERR_UNSUPPORTED;
+  }
+  const int nb = int(c->bi.size());
+  std::vector<int> it(nb, 0), st(nb, PMG_OK);
+  std::vector<double> fr(nb, 0.0);
+  const int rc = guard(c, [&] {
+    PMG_NEED_READY();
+    if (mem != PMG_MEM_HOST && mem != PMG_MEM_DEVICE) throw Fail{PMG_ERR_INVALID_ARGUMENT, "bad memspace"};
+    for (int b = 0; b < nb; ++b)
+      if (!sol[b] || !rhs[b]) throw Fail{PMG_ERR_INVALID_ARGUMENT, "null problem vector"};
+    for (int b = 0; b < nb; ++b) {
+      BatchProblem &B = c->bi[b];
+      const double *xs = sol[b], *fs = rhs[b];
+      if (mem == PMG_MEM_HOST) {
+        PMG_HIP_CHECK(hipMemcpyAsync(B.io, sol[b], sizeof(double) * B.n_dof_fine, hipMemcpyHostToDevice, c->stream));
+        PMG_HIP_CHECK(hipMemcpyAsync(B.io + B.n_dof_fine, rhs[b], sizeof(double) * B.n_dof_fine,
+                                     hipMemcpyHostToDevice, c->stream));
+        xs = B.io;
+        fs = B.io + B.n_dof_fine;
+      }
+      launch_scatter_d<double>(c->bX, xs, B.fidx, B.n_dof_fine, c->stream);
+      launch_scatter_d<double>(c->bB, fs, B.fidx, B.n_dof_fine, c->stream);
+    }
+    fgmres_batch(c, it, fr, st);
+    for (int b = 0; b < nb; ++b) {
+      BatchProblem &B = c->bi[b];
+      double *xd = mem == PMG_MEM_HOST ? B.io : sol[b];
+      launch_gather_d<double>(xd, c->bX, B.fidx, B.n_dof_fine, c->stream);
+      if (mem == PMG_MEM_HOST)
+        PMG_HIP_CHECK(hipMemcpyAsync(sol[b], B.io, sizeof(double) * B.n_dof_fine, hipMemcpyDeviceToHost, c->stream));
+    }
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    check_deferred(c);
+  });
+  for (int b = 0; b < nb; ++b) {
+    if (iters) iters[b] = it[b];
+    if (final_res) final_res[b] = fr[b];
+    if (status) status[b] = rc ? rc : st[b];
+  }
+  if (rc) return rc;
+  for (int b = 0; b < nb; ++b)
+    if (st[b] != PMG_OK) {
+      c->err = "problem " + std::to_string(b) + ": " + pmg_status_string(st[b]);
+      return st[b];
+    }
+  return PMG_OK;
 }
 
 int pmg_fgmres_history(const pmg_ctx *c, double *out, int cap) {

@@ -126,6 +126,11 @@ struct DevLevel {
   std::vector<double> res_bytes[2];
   double defect_bytes[2] = {0, 0}, matvec_bytes = 0;
   double transfer_bytes = 0;
+  // merged (batched) levels: the problems solved directly on this level, and the
+  // (offset, length) pressure blocks that get their own IntoL20
+  std::vector<int> coarse_insts;
+  int *l20_seg = nullptr;
+  int n_l20_seg = 0;
 
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
@@ -230,6 +235,24 @@ void launch_scale_dev(double *y, const double *x, const double *den, long n, hip
 void launch_lincomb(double *x, const double *zbase, long stride, const double *coef, int m, long n,
                     hipStream_t s);
 void launch_scal(double *y, const double *x, double a, long n, hipStream_t s);  // y = a x
+
+// batched path (k_batch.hip): IntoL20 per (offset, length) pressure block; copies between a
+// problem's numbering and the merged numbering (dst[i] = src[idx[i]], dst[idx[i]] = src[i]);
+// BLAS-1 with one scalar per problem over a chunk table (start, length, problem, -)
+constexpr int SEG_CHUNK = 4096;
+template <typename T>
+void launch_l20_seg(T *p, const int *seg, int nseg, hipStream_t s);
+template <typename T>
+void launch_gather_d(double *dst, const T *src, const int *idx, int n, hipStream_t s);
+template <typename T>
+void launch_scatter_d(T *dst, const double *src, const int *idx, int n, hipStream_t s);
+void launch_seg_dot(const double *x, const double *y, const int4 *ch, int nch, double *part, hipStream_t s);
+void launch_seg_axpy_dot(double *w, const double *h, int ldh, const double *v, const double *vn, const int4 *ch,
+                         int nch, double *part, hipStream_t s);
+void launch_seg_reduce(const double *part, const int *cptr, int nb, double *out, int ldo, hipStream_t s);
+void launch_seg_scale(double *y, const double *x, const double *coef, const int4 *ch, int nch, hipStream_t s);
+void launch_seg_lincomb(double *x, const double *z, long stride, const double *coef, int ldc, int m, const int *mask,
+                        const int4 *ch, int nch, hipStream_t s);
 
 }  // namespace pmg
 

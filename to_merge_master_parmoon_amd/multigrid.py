@@ -236,3 +236,84 @@ def build_hierarchy(problem, params=None, device: int = 0, **kw) -> NSEMultiGrid
         mg.add_level(d)
     mg.setup()
     return mg
+
+
+class NSEBatch:
+    """Many independent problems solved together (pmg_batch_create / pmg_batch_fgmres_solve).
+
+    ``problems`` are generated hierarchies (``problem.Problem``); ``project`` gives each its
+    INTERNAL_PROJECT_PRESSURE (default: 1 when all six faces are Dirichlet).  The levels are
+    merged block-diagonally, aligned at the finest level, so every sweep, defect and transfer
+    is one launch for all problems; ``fgmres`` returns per problem what ``NSEMultiGrid.fgmres``
+    returns for that problem alone.
+    """
+
+    def __init__(self, problems, params: _abi.PmgParams | None = None, device: int = 0, project=None, **kw):
+        self._lib = _abi.load_pmg()
+        self.params = params if params is not None else _abi.default_params(**kw)
+        self.problems = list(problems)
+        if project is None:
+            project = [1 if p.config.dirichlet_faces == 63 else 0 for p in self.problems]
+        self._levels = [(_abi.PmgLevelDesc * p.n_levels)(*p.descs) for p in self.problems]
+        insts = (_abi.PmgInstanceDesc * len(self.problems))()
+        for i, p in enumerate(self.problems):
+            insts[i].n_levels = p.n_levels
+            insts[i].levels = C.cast(self._levels[i], C.POINTER(_abi.PmgLevelDesc))
+            insts[i].project_pressure = int(project[i])
+        ctx = C.c_void_p()
+        st = self._lib.pmg_batch_create(device, C.byref(self.params), len(self.problems), insts, C.byref(ctx))
+        if st != 0:
+            raise PmgError(st, "pmg_batch_create")
+        self._ctx = ctx
+        self.n_dof = [p.n_dof(p.n_levels - 1) for p in self.problems]
+
+    def _check(self, st: int, what: str = ""):
+        if st != 0:
+            msg = self._lib.pmg_last_error_string(self._ctx)
+            raise PmgError(st, f"{what} {msg.decode() if msg else ''}".strip())
+        return st
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.pmg_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setup(self):
+        self._check(self._lib.pmg_setup(self._ctx), "pmg_setup")
+
+    def refactor(self):
+        self._check(self._lib.pmg_refactor(self._ctx), "pmg_refactor")
+
+    def __len__(self) -> int:
+        return self._lib.pmg_batch_size(self._ctx)
+
+    def fgmres(self, sols, rhss, raise_on_error: bool = True):
+        """sols / rhss: one float64 vector per problem (sols updated in place)."""
+        nb = len(self.problems)
+        sp = (C.c_void_p * nb)(*[_ptr(s) for s in sols])
+        rp = (C.c_void_p * nb)(*[_ptr(r) for r in rhss])
+        it = (C.c_int * nb)()
+        res = (C.c_double * nb)()
+        stv = (C.c_int * nb)()
+        st = self._lib.pmg_batch_fgmres_solve(self._ctx, sp, rp, it, res, stv, MEM_HOST)
+        if raise_on_error:
+            self._check(st, "batch fgmres")
+        return list(it), list(res), list(stv)
+
+    def synchronize(self):
+        self._check(self._lib.pmg_synchronize(self._ctx))
+
+    def set_profiling(self, on: bool):
+        self._lib.pmg_set_profiling(self._ctx, 1 if on else 0)
+
+    def timers(self) -> dict:
+        return NSEMultiGrid.timers(self)
+
+    def reset_timers(self):
+        self._lib.pmg_timers_reset(self._ctx)

@@ -8,6 +8,13 @@ configurations restate the benchmark workloads:
   b = curl(psi) with 8 seeded Fourier modes scaled to max|b| = 1 (seed 1001),
   f = 8 seeded Fourier modes (seed 1002), u = 0 on the boundary; levels 4^3 ... n^3.
 * ``quadlin(...)`` - the closed-form AnsatzQuadLin case (u quadratic, p linear).
+* ``batch_instance`` / ``batched_oseen`` - the batched workload: independent unit-cube
+  Oseen problems of n^3 cells, n drawn from {8, 12, 16, 20, 24} (seed 2001), nu
+  log-uniform in [1e-3, 1] (seed 2002), Dirichlet on all six faces or on x=0, y=0, z=0
+  only (probability 1/2 each, seed 2003), coarsest-grid interior vertices moved by
+  U[-0.1h, 0.1h]^3 (seed 4000+i; the finer levels refine the perturbed trilinear cells),
+  Fourier convection (seed 3000+i) and right-hand side (seed 6000+i); levels n, n/2, ...
+  down to the first size <= 4 or odd (12 -> 6 -> 3, 20 -> 10 -> 5).
 """
 from __future__ import annotations
 
@@ -126,3 +133,36 @@ def quadlin(coarse=(1, 1, 1), n_levels: int = 3, nu: float = 1.0, conv=(0.0, 0.0
     return Problem(cx=coarse[0], cy=coarse[1], cz=coarse[2], n_levels=n_levels, nu=nu, conv_kind=kind,
                    conv_const=conv, rhs_kind=_abi.RHS_EXACT, exact_kind=_abi.EXACT_QUADLIN,
                    dirichlet_faces=dirichlet_faces)
+
+
+DIRICHLET_ALL = 63
+DIRICHLET_X0_Y0_Z0 = 1 | 4 | 16
+
+
+def coarsest_of(n: int) -> tuple[int, int]:
+    """(coarsest size, number of levels) of the halving hierarchy of an n^3 cube."""
+    m, levels = n, 1
+    while m > 4 and m % 2 == 0:
+        m //= 2
+        levels += 1
+    return m, levels
+
+
+def batch_instance(n: int, nu: float, dirichlet_faces: int, i: int, perturb: float = 0.1) -> Problem:
+    """Problem i of the batched workload (see the module docstring)."""
+    m, levels = coarsest_of(n)
+    return Problem(cx=m, cy=m, cz=m, n_levels=levels, nu=nu, conv_kind=_abi.CONV_FOURIER, seed_conv=3000 + i,
+                   conv_max=1.0, rhs_kind=_abi.RHS_FOURIER, seed_rhs=6000 + i, dirichlet_faces=dirichlet_faces,
+                   perturb=perturb, seed_perturb=4000 + i)
+
+
+def batched_specs(count: int = 64, sizes=(8, 12, 16, 20, 24)):
+    """(n, nu, dirichlet_faces) of the batched workload's problems."""
+    ns = np.random.default_rng(2001).choice(np.asarray(sizes), size=count)
+    nus = 10.0 ** np.random.default_rng(2002).uniform(-3.0, 0.0, size=count)
+    six = np.random.default_rng(2003).random(count) < 0.5
+    return [(int(ns[i]), float(nus[i]), DIRICHLET_ALL if six[i] else DIRICHLET_X0_Y0_Z0) for i in range(count)]
+
+
+def batched_oseen(count: int = 64, sizes=(8, 12, 16, 20, 24)) -> list[Problem]:
+    return [batch_instance(n, nu, f, i) for i, (n, nu, f) in enumerate(batched_specs(count, sizes))]
