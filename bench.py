@@ -83,6 +83,89 @@ def run_gpu(prob, precision: int, steps: int, warmup: int, log):
     return dict(times=times, iters=iters, x=x, hist=hist, timers=tm, mg=mg)
 
 
+def run_batched(args, log):
+    """The batched workload: independent perturbed-cube Oseen problems of 8^3..24^3 cells
+    (problem.batched_specs) solved in one batch context, each FGMRES step one merged
+    V-cycle for all problems; timed beside the same problems solved one context at a time."""
+    from to_merge_master_parmoon_amd import problem
+    from to_merge_master_parmoon_amd.multigrid import DeviceVector, NSEBatch, build_hierarchy
+    prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]
+    specs = problem.batched_specs(args.batch_count)
+    t0 = time.time()
+    probs = []
+    for i, (n, nu, f) in enumerate(specs):
+        probs.append(problem.batch_instance(n, nu, f, i))
+        if (i + 1) % 8 == 0:
+            log(f"generated {i + 1}/{len(specs)} problems ({time.time() - t0:.1f}s)")
+    ndof = [p.rhs(-1).size for p in probs]
+    log(f"{len(probs)} problems, {sum(ndof)} dofs in total")
+
+    def params_for(p):
+        sp = solver_params(p)
+        sp.sc_lin_maxit_saddle = args.batch_maxit
+        return sp
+
+    def batched(p):
+        t = time.time()
+        B = NSEBatch(probs, params=params_for(p))
+        log(f"batch context ({time.time() - t:.1f}s)")
+        rh = [DeviceVector(B, ndof[i], probs[i].rhs(-1)) for i in range(len(probs))]
+        zs = [DeviceVector(B, ndof[i], np.zeros(ndof[i])) for i in range(len(probs))]
+        import ctypes as C
+        nb = len(probs)
+        sp = (C.c_void_p * nb)(*[z.ptr for z in zs])
+        rp = (C.c_void_p * nb)(*[r.ptr for r in rh])
+        it, res, stv = (C.c_int * nb)(), (C.c_double * nb)(), (C.c_int * nb)()
+        times = []
+        for k in range(args.warmup + args.steps):
+            for z in zs:
+                z.upload(np.zeros(z.n))
+            B.synchronize()
+            t = time.perf_counter()
+            B.refactor()
+            B._lib.pmg_batch_fgmres_solve(B._ctx, sp, rp, it, res, stv, 1)
+            B.synchronize()
+            dt = time.perf_counter() - t
+            if k >= args.warmup:
+                times.append(dt)
+            log(f"batch {['fp64', 'fp32', 'bf16'][p]} step {k}: {dt * 1e3:.1f} ms, "
+                f"converged {sum(1 for s in stv if s == 0)}/{nb}")
+        xs = [z.download() for z in zs]
+        B.close()
+        return float(np.mean(times)), list(it), list(stv), xs
+
+    t_b, it_b, st_b, x_b = batched(prec)
+    t_64 = it_64 = None
+    if prec != 0 and not args.no_fp64_ref:
+        t_64, it_64, _, x_64 = batched(0)
+    # one problem at a time, same precision, each in its own context
+    seq = 0.0
+    for i, pr in enumerate(probs):
+        mg = build_hierarchy(pr, params_for(prec))
+        rhs = DeviceVector(mg, ndof[i], pr.rhs(-1))
+        sol = DeviceVector(mg, ndof[i], np.zeros(ndof[i]))
+        mg.synchronize()
+        t = time.perf_counter()
+        mg.refactor()
+        st = mg._lib.pmg_fgmres_solve(mg._ctx, sol.ptr, rhs.ptr, None, None, 1)
+        mg.synchronize()
+        seq += time.perf_counter() - t
+        mg.close()
+    log(f"one at a time: {seq * 1e3:.1f} ms")
+    conv = sum(1 for s in st_b if s == 0)
+    cfg = {"workload": f"batched_oseen_{len(probs)}_problems_8-24^3_q2p1disc_fgmres30_vcycle22",
+           "problems": len(probs), "dofs": int(sum(ndof)), "precision": args.precision,
+           "maxit": args.batch_maxit, "converged": conv,
+           "iterations": it_b, "one_at_a_time_s": round(seq, 4), "speedup_vs_one_at_a_time": round(seq / t_b, 3)}
+    if t_64 is not None:
+        cfg.update({"fp64_batch_s": round(t_64, 4), "speedup_vs_fp64": round(t_64 / t_b, 3),
+                    "fp64_iterations": it_64})
+    return {"metric": "Batched Oseen Q2/P1disc solves s to 1e-10 (all problems, one batch)",
+            "value": round(t_b, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(t_b * 1e3, 3), "higher_is_better": False, "scaling": "strong",
+            "vs_baseline": None, "dtype": args.precision, "data": "synthetic", "config": cfg}
+
+
 def measured_traffic(kernel: str, precision: str, n: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (calibrated against a copy kernel), for the workload it was measured on."""
@@ -168,12 +251,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the FP64 GPU solve timed beside it")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--workload", choices=["oseen32", "batched"], default="oseen32",
+                    help="batched: the independent-problems workload instead of the headline solve")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--batch-maxit", type=int, default=100)
     args = ap.parse_args()
     log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition "
               "of this solver is described in DESIGN.md and not launched.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        print(json.dumps(run_batched(args, log)), flush=True)
+        return
     from to_merge_master_parmoon_amd import problem
     prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]
     t0 = time.time()
