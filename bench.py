@@ -34,13 +34,17 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+USE_GRAPH = 1
+
+
 def solver_params(precision: int):
     from to_merge_master_parmoon_amd import _abi
-    # config 2/3: FGMRES(30), SC_LIN_RED_FACTOR_SADDLE = 1e-10, SC_LIN_RES_NORM_MIN_SADDLE = 0,
+    # FGMRES(30), SC_LIN_RED_FACTOR_SADDLE = 1e-10, SC_LIN_RES_NORM_MIN_SADDLE = 0,
     # SC_LIN_MAXIT_SADDLE = 1000, SC_AMG_PREC_IT = 1, V(2,2), damp = alpha = beta = 1
     return _abi.default_params(sc_gmres_restart=30, sc_lin_red_factor_saddle=1e-10,
                                sc_lin_res_norm_min_saddle=0.0, sc_lin_maxit_saddle=1000,
-                               sc_pre_smooth_saddle=2, sc_post_smooth_saddle=2, precision=precision)
+                               sc_pre_smooth_saddle=2, sc_post_smooth_saddle=2, precision=precision,
+                               use_graph=USE_GRAPH)
 
 
 def run_gpu(prob, precision: int, steps: int, warmup: int, log):
@@ -255,7 +259,10 @@ def main():
                     help="batched: the independent-problems workload instead of the headline solve")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-maxit", type=int, default=100)
+    ap.add_argument("--no-graph", action="store_true", help="launch every V-cycle kernel by kernel")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = 0 if args.no_graph else 1
     log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition "

@@ -73,6 +73,8 @@ typedef struct {
   int    sc_amg_prec_it;
   int    precision;                            /* pmg_precision                           */
   int    bf16_refine_steps;                    /* must be 0: no local refinement; FGMRES refines */
+  int    use_graph;                            /* 1: every V-cycle replays a HIP graph captured on
+                                                  first use (default); 0: launched kernel by kernel */
 } pmg_params;
 
 /*

@@ -146,3 +146,23 @@ def test_errors_are_status_codes(oseen8):
     with pytest.raises(_abi.PmgError) as e:
         mg.fgmres(np.zeros_like(f), f)
     assert e.value.status == 7
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_vcycle_graph_replays_match_eager(oseen16, precision):
+    """Every replay of the captured V-cycle equals the kernel-by-kernel cycle bit for bit,
+    with other work of the same context between the replays."""
+    mg_e = _hier(oseen16, use_graph=0, precision=precision)
+    mg_g = _hier(oseen16, use_graph=1, precision=precision)
+    n = mg_g.n_dof[-1]
+    for k in range(4):
+        v = _rand(n, 300 + k)
+        z_e = mg_e.cycle(v)
+        mg_g.defect(2, v, v)
+        z_g = mg_g.cycle(v)
+        assert np.array_equal(z_e, z_g), f"replay {k}"
+    f = oseen16.rhs(-1).copy()
+    x_e, x_g = np.zeros_like(f), np.zeros_like(f)
+    it_e = mg_e.fgmres(x_e, f)[0]
+    it_g = mg_g.fgmres(x_g, f)[0]
+    assert it_e == it_g and np.array_equal(x_e, x_g)

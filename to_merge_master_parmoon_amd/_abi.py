@@ -42,6 +42,7 @@ class PmgParams(C.Structure):
         ("sc_amg_prec_it", C.c_int),
         ("precision", C.c_int),
         ("bf16_refine_steps", C.c_int),
+        ("use_graph", C.c_int),
     ]
 
 

@@ -72,6 +72,9 @@ struct pmg_ctx {
   double *bcoef = nullptr;                 // per problem scalars / combination coefficients
   int *bmask = nullptr;
   int b_restart_alloc = 0;
+  // the V-cycle as a HIP graph (captured on first use, dropped when levels or parameters change)
+  hipGraph_t vgraph = nullptr;
+  hipGraphExec_t vexec = nullptr;
 };
 
 namespace {
@@ -545,23 +548,76 @@ void cycle(pmg_ctx *c, int i) {
   for (int j = 0; j < c->prm.sc_post_smooth_saddle; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D));
 }
 
-// TMultiGridIte::Iterate with zero start (MultiGridIte.C:76-130): z = M^{-1} v, device vectors
-void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
+// the main stream waits for coarse factorizations still running on the second stream
+void resolve_pending(pmg_ctx *c) {
+  auto wait = [&](Coarse &C) {
+    if (!C.pending) return;
+    PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, C.ev_done, 0));
+    C.pending = false;
+  };
+  wait(c->coarse);
+  for (auto &b : c->bi) wait(b.co);
+}
+
+void drop_graph(pmg_ctx *c) {
+  if (c->vexec) (void)hipGraphExecDestroy(c->vexec);
+  if (c->vgraph) (void)hipGraphDestroy(c->vgraph);
+  c->vexec = nullptr;
+  c->vgraph = nullptr;
+}
+
+// one V-cycle on the finest level's own vectors: right-hand side already in rhs (rhs32),
+// zero start, result in sol (sol32)
+void vcycle_body(pmg_ctx *c) {
   DevLevel &F = c->L.back();
   const int top = int(c->L.size()) - 1;
   if (!mixed(c)) {
     launch_fill<double>(F.sol, 0.0, F.n_dof, c->stream);
-    launch_copy<double>(F.rhs, v, F.n_dof, c->stream);
     set_dirichlet(c, F, F.sol, F.rhs);
     cycle<double>(c, top);
-    launch_copy<double>(z, F.sol, F.n_dof, c->stream);
   } else {
     launch_fill<float>(F.sol32, 0.0f, F.n_dof, c->stream);
-    launch_d2f(F.rhs32, v, F.n_dof, c->stream);
     set_dirichlet_t<float>(c, F, F.sol32, F.rhs32);
     cycle<float>(c, top);
-    launch_f2d(z, F.sol32, F.n_dof, c->stream);
   }
+}
+
+// Capture vcycle_body into a graph.  Every launch of the cycle is a kernel (fills and
+// counter resets included), its buffers are fixed per context, and the device-side
+// generation counters of the coarse sweeps advance on every replay.
+void capture_vcycle(pmg_ctx *c) {
+  drop_graph(c);
+  PMG_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    vcycle_body(c);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  PMG_HIP_CHECK(hipStreamEndCapture(c->stream, &c->vgraph));
+  PMG_HIP_CHECK(hipGraphInstantiate(&c->vexec, c->vgraph, nullptr, nullptr, 0));
+}
+
+// TMultiGridIte::Iterate with zero start (MultiGridIte.C:76-130): z = M^{-1} v, device vectors
+void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
+  DevLevel &F = c->L.back();
+  if (!mixed(c))
+    launch_copy<double>(F.rhs, v, F.n_dof, c->stream);
+  else
+    launch_d2f(F.rhs32, v, F.n_dof, c->stream);
+  if (c->prm.use_graph && !c->profiling) {
+    resolve_pending(c);
+    if (!c->vexec) capture_vcycle(c);
+    PMG_HIP_CHECK(hipGraphLaunch(c->vexec, c->stream));
+  } else {
+    vcycle_body(c);
+  }
+  if (!mixed(c))
+    launch_copy<double>(z, F.sol, F.n_dof, c->stream);
+  else
+    launch_f2d(z, F.sol32, F.n_dof, c->stream);
 }
 
 // ------------------------------------------------------------------ set-up
@@ -1097,6 +1153,7 @@ void pmg_params_default(pmg_params *p) {
   p->sc_amg_prec_it = 1;
   p->precision = PMG_PREC_FP64;
   p->bf16_refine_steps = 0;
+  p->use_graph = 1;
 }
 
 const char *pmg_status_string(int s) {
@@ -1165,6 +1222,7 @@ int pmg_ctx_destroy(pmg_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  drop_graph(c);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->hpin) (void)hipHostFree(c->hpin);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1193,6 +1251,7 @@ int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
   if (prec_change && !c->L.empty()) { c->err = "precision is fixed once levels exist"; return PMG_ERR_UNSUPPORTED; }
   c->prm = *p;
   if (pin_change) c->coarse.ready = false;
+  drop_graph(c);
   return PMG_OK;
 }
 
@@ -1204,6 +1263,7 @@ int pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_
 static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool merged) {
   return guard(c, [&] {
     PMG_HIP_CHECK(hipSetDevice(c->device));
+    drop_graph(c);
     if (d->n_ulocal != NU_LOC || d->n_plocal != NP_LOC)
       throw Fail{PMG_ERR_UNSUPPORTED, "only Q2/P1disc hexahedra (27 + 4 local dofs)"};
     if (d->n_udof <= 0 || d->n_active < 0 || d->n_active > d->n_udof || d->n_pdof <= 0 || d->n_cells <= 0)
