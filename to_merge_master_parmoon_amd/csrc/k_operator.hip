@@ -89,9 +89,9 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
       const int *__restrict__ kc = op.a_kcol + k0;
       const T *__restrict__ v = op.av + 9L * k0;
       if (r < NA) {
-        row_a9<T, 4>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+        row_a9<T>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
         const int q0 = op.bt_rowptr[r], m = op.bt_rowptr[r + 1] - q0;
-        row_bt<T, 2>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, xp, s1, s2, s3);
+        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, xp, s1, s2, s3);
       } else {
         // rows N_Active..N_UDOF: diagonal blocks only (MGComponents3D.C:3136-3156)
         row_a_diag<T, 1>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
@@ -114,7 +114,7 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
       const int pr = r - N;
       T s = 0;
       const int k0 = op.b_rowptr[pr], n = op.b_rowptr[pr + 1] - k0;
-      row_b<T, 2>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, x1, x2, x3, s);
+      row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, x1, x2, x3, s);
       s = red16(s);
       if (lane == 0) {
         const T v = (MODE == 0) ? s : T(bv[3 * N + pr] - s);

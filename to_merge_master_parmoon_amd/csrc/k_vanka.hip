@@ -298,9 +298,9 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
       const int *__restrict__ kc = op.a_kcol + k0;
       const T *__restrict__ v = op.av + 9L * k0;
       if (UDOF < NA) {
-        row_a9<T, 4>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
+        row_a9<T>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
         const int q0 = op.bt_rowptr[UDOF], m = op.bt_rowptr[UDOF + 1] - q0;
-        row_bt<T, 2>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
+        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
       } else {
         row_a_diag<T, 1>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
       }
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
       const int PDOF = p_global[long(cell) * NP_LOC + m];
       T s = 0;
       const int k0 = op.b_rowptr[PDOF], n = op.b_rowptr[PDOF + 1] - k0;
-      row_b<T, 2>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
+      row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
       s = red16(s);
       if (lane == 0) out[ip[3 * NU_LOC + m]] = f[3 * N + PDOF] - s;
     }
@@ -348,7 +348,87 @@ struct PackedSizes {
 
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));   // register-promotable 16 B
 
+// FP32 sweeps with every factor entry a lane needs held in registers before the dependent
+// chain starts: the unit-lower sweep keeps row lane (L(lane, k), k < 64) and row 64+lane
+// (L(64+lane, k)) of L, the upper sweep the rows of U, each as one register per column k
+// (uniform index in the unrolled loops).  Entries outside the step's range are zero, so a
+// step is one broadcast and two fmas with no select and no LDS access; in the upper sweep
+// a lane's value is final once its own pivot step is passed (later steps only touch rows
+// above the pivot), so the quotients x / U(r,r) are formed for all lanes after the chain.
 template <typename T, typename F>
+__device__ __forceinline__ void sweeps_in_registers(v4u *s4, const F *sl0, const F *sl1, const v4u *rU, int lane,
+                                                    bool two, T &x0, T &x1) {
+  using PS = PackedSizes<F>;
+  {
+    T L0[64], L1[NLOC - 1];
+#pragma unroll
+    for (int k = 0; k < NLOC - 1; ++k) {
+      const int o = lu_loff(k) - k - 1;
+      if (k < 64) {
+        const T a = Ld<T, F>::get(sl0, o);
+        L0[k] = lane > k ? a : T(0);
+      }
+      const T b = Ld<T, F>::get(sl1, o);
+      L1[k] = (two && lane + 64 > k) ? b : T(0);
+    }
+    // pin every entry in a register here (an empty asm that "modifies" it), so that the
+    // LDS reads stay ahead of the chain instead of being sunk into it
+#pragma unroll
+    for (int k = 0; k < NLOC - 1; ++k) {
+      if (k < 64) asm volatile("" : "+v"(L0[k]));
+      asm volatile("" : "+v"(L1[k]));
+    }
+#pragma unroll
+    for (int k = 0; k < NLOC - 1; ++k) {
+      if (k < 64) {
+        const T bk = bcast(x0, k);
+        x0 = tfma(-bk, L0[k], x0);
+        x1 = tfma(-bk, L1[k], x1);
+      } else {
+        const T bk = bcast(x1, k - 64);
+        x1 = tfma(-bk, L1[k], x1);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PS::RU; ++q) s4[lane + 64 * q] = rU[q];
+  __syncthreads();
+  const T dg0 = Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(lane) + lane);
+  const T dg1 = two ? Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(64 + lane) + 64 + lane) : T(1);
+  const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
+  T U0[NLOC], U1[NLOC - 64];
+#pragma unroll
+  for (int k = 0; k < NLOC; ++k) {
+    const int o = lu_uoff(k);
+    const T a = Ld<T, F>::get(sl0, o);
+    U0[k] = lane < k ? a : T(0);
+    if (k >= 64) {
+      const T b = Ld<T, F>::get(sl1, o);
+      U1[k - 64] = (two && lane + 64 < k) ? b : T(0);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NLOC; ++k) {
+    asm volatile("" : "+v"(U0[k]));
+    if (k >= 64) asm volatile("" : "+v"(U1[k - 64]));
+  }
+#pragma unroll
+  for (int k = NLOC - 1; k >= 0; --k) {
+    if (k >= 64) {
+      const T xk = bcast(x1 * rc1, k - 64);
+      x0 = tfma(-xk, U0[k], x0);
+      x1 = tfma(-xk, U1[k - 64], x1);
+    } else {
+      const T xk = bcast(x0 * rc0, k);
+      x0 = tfma(-xk, U0[k], x0);
+    }
+  }
+  x0 = x0 * rc0;
+  x1 = x1 * rc1;
+}
+
+template <typename T, typename F, bool REG>
 __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, const int *__restrict__ cells,
                                                     int ncells, int pos0, const T *__restrict__ rloc,
                                                     const int *__restrict__ u_global,
@@ -381,57 +461,61 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   T x0 = rs[lane];
   T x1 = two ? rs[64 + lane] : T(0);
   __syncthreads();
-  // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1.  Row
-  // 64+lane is below every k < 64, row lane is above every k >= 64, so per step only one
-  // of the two rows needs the "below the pivot" test.
-#pragma unroll
-  for (int k = 0; k < NLOC - 1; ++k) {
-    const int o = lu_loff(k) - k - 1;
-    if (k < 64) {
-      const T bk = bcast(x0, k);
-      const T l0 = Ld<T, F>::get(sl0, o), l1 = Ld<T, F>::get(sl1, o);
-      const T y0 = tfma(-bk, l0, x0);
-      x0 = lane > k ? y0 : x0;
-      x1 = tfma(-bk, l1, x1);  // lanes without a second row carry an unused value
-    } else {
-      const T bk = bcast(x1, k - 64);
-      const T l1 = Ld<T, F>::get(sl1, o);
-      const T y1 = tfma(-bk, l1, x1);
-      x1 = lane + 64 > k ? y1 : x1;
+  if constexpr (REG) {
+    sweeps_in_registers<T, F>(s4, sl0, sl1, rU, lane, two, x0, x1);
+  } else {
+    // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1.  Row
+    // 64+lane is below every k < 64, row lane is above every k >= 64, so per step only one
+    // of the two rows needs the "below the pivot" test.
+  #pragma unroll
+    for (int k = 0; k < NLOC - 1; ++k) {
+      const int o = lu_loff(k) - k - 1;
+      if (k < 64) {
+        const T bk = bcast(x0, k);
+        const T l0 = Ld<T, F>::get(sl0, o), l1 = Ld<T, F>::get(sl1, o);
+        const T y0 = tfma(-bk, l0, x0);
+        x0 = lane > k ? y0 : x0;
+        x1 = tfma(-bk, l1, x1);  // lanes without a second row carry an unused value
+      } else {
+        const T bk = bcast(x1, k - 64);
+        const T l1 = Ld<T, F>::get(sl1, o);
+        const T y1 = tfma(-bk, l1, x1);
+        x1 = lane + 64 > k ? y1 : x1;
+      }
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PS::RU; ++q) s4[lane + 64 * q] = rU[q];
-  __syncthreads();
-  // upper sweep (dtrsm U): x_k /= U(k,k), then eliminate column k above the diagonal;
-  // U(r,k) at lu_uoff(k) + r.  Row lane is above every k >= 64, row 64+lane below every
-  // k < 64.
-  // Diagonal of U and its reciprocal per lane, computed in parallel before the chain.  The
-  // quotient x_k / U(k,k) is then q = x r, e = fma(-q, U, x), q' = fma(e, r, q): with r the
-  // correctly rounded reciprocal this is the correctly rounded quotient (Markstein), i.e.
-  // the same value as the division of the column-oriented dtrsm, in three dependent steps.
-  const T dg0 = Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(lane) + lane);
-  const T dg1 = two ? Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(64 + lane) + 64 + lane) : T(1);
-  const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
-#pragma unroll
-  for (int k = NLOC - 1; k >= 0; --k) {
-    const int o = lu_uoff(k);
-    if (k >= 64) {
-      T q1 = x1 * rc1;
-      if constexpr (sizeof(T) == 8) q1 = tfma(tfma(-q1, dg1, x1), rc1, q1);
-      const T xk = bcast(q1, k - 64);
-      const T u0 = Ld<T, F>::get(sl0, o), u1 = Ld<T, F>::get(sl1, o);
-      x0 = tfma(-xk, u0, x0);  // every row lane < 64 <= k lies above the pivot
-      const T y1 = tfma(-xk, u1, x1);
-      x1 = (lane + 64 == k) ? xk : (lane + 64 < k ? y1 : x1);
-    } else {
-      T q0 = x0 * rc0;
-      if constexpr (sizeof(T) == 8) q0 = tfma(tfma(-q0, dg0, x0), rc0, q0);
-      const T xk = bcast(q0, k);
-      const T u0 = Ld<T, F>::get(sl0, o);
-      const T y0 = tfma(-xk, u0, x0);
-      x0 = (lane == k) ? xk : (lane < k ? y0 : x0);
+    __syncthreads();
+  #pragma unroll
+    for (int q = 0; q < PS::RU; ++q) s4[lane + 64 * q] = rU[q];
+    __syncthreads();
+    // upper sweep (dtrsm U): x_k /= U(k,k), then eliminate column k above the diagonal;
+    // U(r,k) at lu_uoff(k) + r.  Row lane is above every k >= 64, row 64+lane below every
+    // k < 64.
+    // Diagonal of U and its reciprocal per lane, computed in parallel before the chain.  The
+    // quotient x_k / U(k,k) is then q = x r, e = fma(-q, U, x), q' = fma(e, r, q): with r the
+    // correctly rounded reciprocal this is the correctly rounded quotient (Markstein), i.e.
+    // the same value as the division of the column-oriented dtrsm, in three dependent steps.
+    const T dg0 = Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(lane) + lane);
+    const T dg1 = two ? Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(64 + lane) + 64 + lane) : T(1);
+    const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
+  #pragma unroll
+    for (int k = NLOC - 1; k >= 0; --k) {
+      const int o = lu_uoff(k);
+      if (k >= 64) {
+        T q1 = x1 * rc1;
+        if constexpr (sizeof(T) == 8) q1 = tfma(tfma(-q1, dg1, x1), rc1, q1);
+        const T xk = bcast(q1, k - 64);
+        const T u0 = Ld<T, F>::get(sl0, o), u1 = Ld<T, F>::get(sl1, o);
+        x0 = tfma(-xk, u0, x0);  // every row lane < 64 <= k lies above the pivot
+        const T y1 = tfma(-xk, u1, x1);
+        x1 = (lane + 64 == k) ? xk : (lane + 64 < k ? y1 : x1);
+      } else {
+        T q0 = x0 * rc0;
+        if constexpr (sizeof(T) == 8) q0 = tfma(tfma(-q0, dg0, x0), rc0, q0);
+        const T xk = bcast(q0, k);
+        const T u0 = Ld<T, F>::get(sl0, o);
+        const T y0 = tfma(-xk, u0, x0);
+        x0 = (lane == k) ? xk : (lane < k ? y0 : x0);
+      }
     }
   }
   // u[UDOF] += damp * delta, p[PDOF] += damp * delta
@@ -476,7 +560,10 @@ void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, in
                            const int *p_global, const uint8_t *invperm, const int *items,
                            const T *u, const T *f, T *rloc, hipStream_t s) {
   if (ncells <= 0) return;
-  const int grid = chunk_grid(long(ncells) * (NU_LOC + NP_LOC), 32, 8192);
+  // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
+  // two on the big levels
+  const long n_items = long(ncells) * (NU_LOC + NP_LOC);
+  const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
   k_vanka_res<T><<<grid, 256, 0, s>>>(op, cells, ncells, pos0, u_global, p_global, invperm, items, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
@@ -491,8 +578,15 @@ template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc, const int *u_global,
                         const int *p_global, int n_udof, T *u, T damp, hipStream_t s) {
   if (ncells <= 0) return;
-  k_vanka_solve<T, F><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(lu, cells, ncells, pos0, rloc, u_global,
-                                                                    p_global, n_udof, u, damp);
+  // FP32: colors of up to 2048 patches (a few waves per SIMD at most) are bound by the
+  // dependent sweep chain and take the register-resident sweeps; larger colors need the
+  // occupancy of the leaner LDS form
+  if (sizeof(T) == 4 && ncells <= 2048)
+    k_vanka_solve<T, F, sizeof(T) == 4><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(
+        lu, cells, ncells, pos0, rloc, u_global, p_global, n_udof, u, damp);
+  else
+    k_vanka_solve<T, F, false><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(lu, cells, ncells, pos0, rloc,
+                                                                           u_global, p_global, n_udof, u, damp);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_solve<double, double>(const double *, const int *, int, int, const double *,

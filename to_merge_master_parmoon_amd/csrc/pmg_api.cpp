@@ -14,6 +14,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "batch_merge.hpp"
@@ -321,21 +322,28 @@ void upload_operator(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d, bool fp32
   D.n_dirichlet = d->n_dirichlet; D.n_pdof = d->n_pdof; D.n_dof = 3 * d->n_udof + d->n_pdof;
   D.a_nnz = d->a_nnz; D.b_nnz = d->b_nnz; D.bt_nnz = d->bt_nnz;
   D.a_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.a_rowptr, d->a_rowptr, D.n_udof + 1);
-  D.a_kcol = dalloc<int>(c, D.a_nnz); h2d(c, D.a_kcol, d->a_kcol, D.a_nnz);
+  // the streamed arrays (column indices, row-blocked values) carry ROW_PAD zeroed entries
+  // after their end: the row kernels' 4-entry windows of rows shorter than 4 read past it
+  auto padded = [&](auto *&ptr, size_t n) {
+    using E = std::remove_reference_t<decltype(*ptr)>;
+    ptr = dalloc<E>(c, n + ROW_PAD);
+    launch_fill<E>(ptr + n, E(0), ROW_PAD, c->stream);
+  };
+  padded(D.a_kcol, D.a_nnz); h2d(c, D.a_kcol, d->a_kcol, D.a_nnz);
   D.b_rowptr = dalloc<int>(c, D.n_pdof + 1); h2d(c, D.b_rowptr, d->b_rowptr, D.n_pdof + 1);
-  D.b_kcol = dalloc<int>(c, D.b_nnz); h2d(c, D.b_kcol, d->b_kcol, D.b_nnz);
+  padded(D.b_kcol, D.b_nnz); h2d(c, D.b_kcol, d->b_kcol, D.b_nnz);
   D.bt_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.bt_rowptr, d->bt_rowptr, D.n_udof + 1);
-  D.bt_kcol = dalloc<int>(c, D.bt_nnz); h2d(c, D.bt_kcol, d->bt_kcol, D.bt_nnz);
+  padded(D.bt_kcol, D.bt_nnz); h2d(c, D.bt_kcol, d->bt_kcol, D.bt_nnz);
   D.a = dalloc<double>(c, 9 * size_t(D.a_nnz));
   D.b = dalloc<double>(c, 3 * size_t(D.b_nnz));
   D.bt = dalloc<double>(c, 3 * size_t(D.bt_nnz));
-  D.av64 = dalloc<double>(c, 9 * size_t(D.a_nnz));
-  D.bv64 = dalloc<double>(c, 3 * size_t(D.b_nnz));
-  D.btv64 = dalloc<double>(c, 3 * size_t(D.bt_nnz));
+  padded(D.av64, 9 * size_t(D.a_nnz));
+  padded(D.bv64, 3 * size_t(D.b_nnz));
+  padded(D.btv64, 3 * size_t(D.bt_nnz));
   if (fp32) {
-    D.av32 = dalloc<float>(c, 9 * size_t(D.a_nnz));
-    D.bv32 = dalloc<float>(c, 3 * size_t(D.b_nnz));
-    D.btv32 = dalloc<float>(c, 3 * size_t(D.bt_nnz));
+    padded(D.av32, 9 * size_t(D.a_nnz));
+    padded(D.bv32, 3 * size_t(D.b_nnz));
+    padded(D.btv32, 3 * size_t(D.bt_nnz));
   }
   upload_values(c, D, d);
 }

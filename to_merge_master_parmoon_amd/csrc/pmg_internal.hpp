@@ -26,6 +26,7 @@ static_assert(LU_UOFF + LU_U <= LU_STRIDE, "packed factors exceed the patch stri
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
+constexpr int ROW_PAD = 64;         // zeroed tail entries of the streamed operator arrays
 // per-cell gather map of the patch matrix (ints): offsets of the A entries (j, m) into the
 // row-blocked A values (block 0; -1: no entry), row lengths of the 27 velocity rows
 // (negative: Dirichlet row), BT offsets (j, pressure m), BT row lengths, B offsets

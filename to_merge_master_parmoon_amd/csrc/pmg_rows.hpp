@@ -1,10 +1,11 @@
 // pmg_rows.hpp - one CSR row of the type-4 operator against a vector, by 16 lanes.
 //
-// Row-blocked values (see OpView): the n entries of a row are read in batches of
-// U*16: every lane first loads the column indices and the values of its U entries of the
-// batch (all independent loads), then the U gathers of x, then accumulates, so a whole
-// row of up to 16*U entries costs one index -> gather round trip.  Entries past the end of
-// the row are clamped to the last entry and their x values zeroed (no branches).
+// Row-blocked values (see OpView): the n entries of a row are read in batches of 64:
+// every lane first loads the column indices and the values of its 4 consecutive entries
+// of the batch as 16-byte vectors (all independent loads; a 16-lane group reads 256 B
+// (FP32) or 512 B (FP64) of each block per batch), then the 4 gathers of x, then
+// accumulates, so a row of up to 64 entries costs one index -> gather round trip.
+// Entries a lane does not own are masked (zero x values, no branches).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,9 +21,150 @@ __device__ inline T red16(T v) {
   return v;
 }
 
+// 16-byte vector loads of 4 consecutive entries (the addresses are 4-byte aligned; the
+// hardware runs in unaligned mode, so one dwordx4 per float4/int4 and two per double4)
+typedef float pmg_vf4 __attribute__((ext_vector_type(4)));
+typedef double pmg_vd2 __attribute__((ext_vector_type(2)));
+typedef int pmg_vi4 __attribute__((ext_vector_type(4)));
+__device__ inline void ld4(const float *p, float (&o)[4]) {
+  pmg_vf4 a;
+  __builtin_memcpy(&a, p, 16);
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+__device__ inline void ld4(const double *p, double (&o)[4]) {
+  pmg_vd2 a, b;
+  __builtin_memcpy(&a, p, 16);
+  __builtin_memcpy(&b, p + 2, 16);
+  o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+__device__ inline void ld4(const int *p, int (&o)[4]) {
+  pmg_vi4 a;
+  __builtin_memcpy(&a, p, 16);
+  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+
+// Batches of 64 entries: lane l holds entries j0 + 4l .. j0 + 4l + 3.  The window is
+// clamped into the row (start min(j0 + 4l, n - 4)), so the loads of a row with n >= 4
+// entries never leave it; ok[u] marks the entries this lane owns.  A row shorter than 4
+// reads up to 3 entries past its end (the next row, or the zeroed tail padding of
+// ROW_PAD entries every streamed operator array carries).
+__device__ inline int row_window(int j0, int n, int lane, bool (&ok)[4]) {
+  const int jb = j0 + 4 * lane;
+  const int ld = n >= 4 ? (jb < n - 4 ? jb : n - 4) : 0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) ok[u] = (ld + u >= jb) && (ld + u < n);
+  return ld;
+}
+
+// A row with all nine blocks: s_r += sum_c A_rc(row, col) x_c(col)
+template <typename T>
+__device__ inline void row_a9_v(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                              const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
+                              T &s1, T &s2, T &s3) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    bool ok[4];
+    const int ld = row_window(j0, n, lane, ok);
+    int c[4];
+    ld4(kc + ld, c);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = ok[u] ? c[u] : 0;
+    T a[9][4];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) ld4(v + b * n + ld, a[b]);
+    T y1[4], y2[4], y3[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      y1[u] = x1[c[u]];
+      y2[u] = x2[c[u]];
+      y3[u] = x3[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T z1 = ok[u] ? y1[u] : T(0), z2 = ok[u] ? y2[u] : T(0), z3 = ok[u] ? y3[u] : T(0);
+      s1 += a[0][u] * z1 + a[1][u] * z2 + a[2][u] * z3;
+      s2 += a[3][u] * z1 + a[4][u] * z2 + a[5][u] * z3;
+      s3 += a[6][u] * z1 + a[7][u] * z2 + a[8][u] * z3;
+    }
+  }
+}
+
+// A row of a Dirichlet dof: only the diagonal blocks A11, A22, A33
+template <typename T, int U>
+__device__ inline void row_a_diag(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                                  const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
+                                  T &s1, T &s2, T &s3) {
+  for (int j0 = 0; j0 < n; j0 += 16 * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + 16 * u + lane;
+      if (j < n) {
+        const int c = kc[j];
+        s1 += v[j] * x1[c];
+        s2 += v[4 * n + j] * x2[c];
+        s3 += v[8 * n + j] * x3[c];
+      }
+    }
+  }
+}
+
+// BT row (velocity row, pressure columns): s_r += BT_r(row, q) p(q)
+template <typename T>
+__device__ inline void row_bt_v(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
+                              const T *__restrict__ p, T &s1, T &s2, T &s3) {
+  for (int j0 = 0; j0 < m; j0 += 64) {
+    bool ok[4];
+    const int ld = row_window(j0, m, lane, ok);
+    int c[4];
+    ld4(kc + ld, c);
+    T b[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) ld4(w + r * m + ld, b[r]);
+    T pv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pv[u] = p[ok[u] ? c[u] : 0];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T z = ok[u] ? pv[u] : T(0);
+      s1 += b[0][u] * z;
+      s2 += b[1][u] * z;
+      s3 += b[2][u] * z;
+    }
+  }
+}
+
+// B row (pressure row, velocity columns): s += sum_c B_c(row, col) x_c(col)
+template <typename T>
+__device__ inline void row_b_v(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                             const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3, T &s) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    bool ok[4];
+    const int ld = row_window(j0, n, lane, ok);
+    int c[4];
+    ld4(kc + ld, c);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = ok[u] ? c[u] : 0;
+    T b[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) ld4(v + r * n + ld, b[r]);
+    T y1[4], y2[4], y3[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      y1[u] = x1[c[u]];
+      y2[u] = x2[c[u]];
+      y3[u] = x3[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T z1 = ok[u] ? y1[u] : T(0), z2 = ok[u] ? y2[u] : T(0), z3 = ok[u] ? y3[u] : T(0);
+      s += b[0][u] * z1 + b[1][u] * z2 + b[2][u] * z3;
+    }
+  }
+}
+
+// ---- one entry per lane per step (FP32)
 // A row with all nine blocks: s_r += sum_c A_rc(row, col) x_c(col)
 template <typename T, int U>
-__device__ inline void row_a9(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+__device__ inline void row_a9_s(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
                               const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
                               T &s1, T &s2, T &s3) {
   for (int j0 = 0; j0 < n; j0 += 16 * U) {
@@ -58,28 +200,9 @@ __device__ inline void row_a9(const int *__restrict__ kc, const T *__restrict__ 
   }
 }
 
-// A row of a Dirichlet dof: only the diagonal blocks A11, A22, A33
-template <typename T, int U>
-__device__ inline void row_a_diag(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
-                                  const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
-                                  T &s1, T &s2, T &s3) {
-  for (int j0 = 0; j0 < n; j0 += 16 * U) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int j = j0 + 16 * u + lane;
-      if (j < n) {
-        const int c = kc[j];
-        s1 += v[j] * x1[c];
-        s2 += v[4 * n + j] * x2[c];
-        s3 += v[8 * n + j] * x3[c];
-      }
-    }
-  }
-}
-
 // BT row (velocity row, pressure columns): s_r += BT_r(row, q) p(q)
 template <typename T, int U>
-__device__ inline void row_bt(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
+__device__ inline void row_bt_s(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
                               const T *__restrict__ p, T &s1, T &s2, T &s3) {
   for (int j0 = 0; j0 < m; j0 += 16 * U) {
     int jj[U];
@@ -109,7 +232,7 @@ __device__ inline void row_bt(const int *__restrict__ kc, const T *__restrict__ 
 
 // B row (pressure row, velocity columns): s += sum_c B_c(row, col) x_c(col)
 template <typename T, int U>
-__device__ inline void row_b(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+__device__ inline void row_b_s(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
                              const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3, T &s) {
   for (int j0 = 0; j0 < n; j0 += 16 * U) {
     int jj[U];
@@ -140,6 +263,36 @@ __device__ inline void row_b(const int *__restrict__ kc, const T *__restrict__ v
       s += b[0][u] * z1 + b[1][u] * z2 + b[2][u] * z3;
     }
   }
+}
+
+// FP64 rows read 4-entry vectors per lane (fewer, wider loads: measured faster); FP32
+// rows read one entry per lane per 16-entry step, which keeps the 16 lanes' gathers of x
+// on consecutive columns of the row (the vector form spreads them over 4x as many cache
+// lines, and the FP32 kernels are bound by those gathers: measured slower)
+template <typename T>
+__device__ inline void row_a9(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                              const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
+                              T &s1, T &s2, T &s3) {
+  if constexpr (sizeof(T) == 8)
+    row_a9_v<T>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+  else
+    row_a9_s<T, 4>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+}
+template <typename T>
+__device__ inline void row_bt(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
+                              const T *__restrict__ p, T &s1, T &s2, T &s3) {
+  if constexpr (sizeof(T) == 8)
+    row_bt_v<T>(kc, w, m, lane, p, s1, s2, s3);
+  else
+    row_bt_s<T, 2>(kc, w, m, lane, p, s1, s2, s3);
+}
+template <typename T>
+__device__ inline void row_b(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                             const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3, T &s) {
+  if constexpr (sizeof(T) == 8)
+    row_b_v<T>(kc, v, n, lane, x1, x2, x3, s);
+  else
+    row_b_s<T, 2>(kc, v, n, lane, x1, x2, x3, s);
 }
 
 }  // namespace pmg
