@@ -54,6 +54,27 @@ def _load():
         "oracle_lu_solve": (None, [C.c_int, _dp, _ip, _dp]),
         "oracle_color_cells": (C.c_int, [C.c_int, C.c_int, _ip, _ip, _ip, _ip, _ip]),
         "oracle_patch_matrix": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+        # 2D twins (pmg_oracle2d.h)
+        "oracle2d_create": (C.c_int, [C.POINTER(PmgParams), C.c_int, C.c_int, C.c_double, C.c_int,
+                                      C.POINTER(C.c_void_p)]),
+        "oracle2d_destroy": (None, [C.c_void_p]),
+        "oracle2d_sizes": (None, [C.c_void_p, C.c_int, _ip, _ip, _ip, _ip, _ip]),
+        "oracle2d_n_dof": (C.c_int, [C.c_void_p, C.c_int]),
+        "oracle2d_rhs": (None, [C.c_void_p, C.c_int, _dp]),
+        "oracle2d_exact": (None, [C.c_void_p, C.c_int, _dp]),
+        "oracle2d_dof_coords": (None, [C.c_void_p, C.c_int, _dp]),
+        "oracle2d_nnz": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _ip]),
+        "oracle2d_csr": (None, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip, _dp]),
+        "oracle2d_cell_dofs": (None, [C.c_void_p, C.c_int, _ip, _ip]),
+        "oracle2d_matvec": (None, [C.c_void_p, C.c_int, _dp, _dp]),
+        "oracle2d_defect": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]),
+        "oracle2d_cell_vanka": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+        "oracle2d_prolongate": (None, [C.c_void_p, C.c_int, _dp, _dp]),
+        "oracle2d_restrict_defect": (None, [C.c_void_p, C.c_int, _dp, _dp]),
+        "oracle2d_solve_exact": (C.c_int, [C.c_void_p, _dp, _dp]),
+        "oracle2d_vcycle": (C.c_int, [C.c_void_p, _dp, _dp]),
+        "oracle2d_fgmres": (C.c_int, [C.c_void_p, _dp, _dp, _ip, _dp]),
+        "oracle2d_patch_matrix": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -211,4 +232,120 @@ class Oracle:
     def patch_matrix(self, lvl, cell):
         s = np.zeros((85, 85))  # s[col, row]
         self._lib.oracle_patch_matrix(self._h, lvl, cell, dp(s))
+        return s
+
+
+class Oracle2D:
+    """The 2D twins on the CPU (pmg_oracle2d.h): unit square, Q2/P1disc quadrilaterals,
+    'coarse' x 'coarse' cells refined n_levels - 1 times, Dirichlet data on the boundary."""
+
+    STOKES_CURL, QUADLIN = 0, 1
+
+    def __init__(self, params, coarse=4, n_levels=5, nu=1.0, rhs_kind=0):
+        self._lib = _load()
+        self._params = params
+        h = C.c_void_p()
+        st = self._lib.oracle2d_create(C.byref(params), coarse, n_levels, nu, rhs_kind, C.byref(h))
+        if st:
+            raise RuntimeError(f"oracle2d_create failed ({st})")
+        self._h = h
+        self.n_levels = n_levels
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.oracle2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sizes(self, lvl):
+        v = [C.c_int() for _ in range(5)]
+        self._lib.oracle2d_sizes(self._h, lvl, *[C.byref(x) for x in v])
+        return dict(zip(["n_udof", "n_active", "n_pdof", "n_cells", "n_colors"], [x.value for x in v]))
+
+    def n_dof(self, lvl):
+        return self._lib.oracle2d_n_dof(self._h, lvl)
+
+    def _vec(self, fn, lvl, n=None):
+        out = np.empty(n if n is not None else self.n_dof(lvl))
+        fn(self._h, lvl, dp(out))
+        return out
+
+    def rhs(self, lvl):
+        return self._vec(self._lib.oracle2d_rhs, lvl)
+
+    def exact(self, lvl):
+        return self._vec(self._lib.oracle2d_exact, lvl)
+
+    def dof_coords(self, lvl):
+        return self._vec(self._lib.oracle2d_dof_coords, lvl, 2 * self.sizes(lvl)["n_udof"]).reshape(-1, 2)
+
+    def csr(self, lvl):
+        na, nb, nbt = C.c_int(), C.c_int(), C.c_int()
+        self._lib.oracle2d_nnz(self._h, lvl, C.byref(na), C.byref(nb), C.byref(nbt))
+        sz = self.sizes(lvl)
+        nu, npd = sz["n_udof"], sz["n_pdof"]
+        arp, akc, a4 = np.empty(nu + 1, np.int32), np.empty(na.value, np.int32), np.empty(4 * na.value)
+        brp, bkc, b2 = np.empty(npd + 1, np.int32), np.empty(nb.value, np.int32), np.empty(2 * nb.value)
+        trp, tkc, t2 = np.empty(nu + 1, np.int32), np.empty(nbt.value, np.int32), np.empty(2 * nbt.value)
+        self._lib.oracle2d_csr(self._h, lvl, ip(arp), ip(akc), dp(a4), ip(brp), ip(bkc), dp(b2), ip(trp), ip(tkc),
+                               dp(t2))
+        return dict(a_rowptr=arp, a_kcol=akc, a=a4.reshape(4, -1), b_rowptr=brp, b_kcol=bkc, b=b2.reshape(2, -1),
+                    bt_rowptr=trp, bt_kcol=tkc, bt=t2.reshape(2, -1))
+
+    def cell_dofs(self, lvl):
+        nc = self.sizes(lvl)["n_cells"]
+        ug, pg = np.empty(9 * nc, np.int32), np.empty(3 * nc, np.int32)
+        self._lib.oracle2d_cell_dofs(self._h, lvl, ip(ug), ip(pg))
+        return ug.reshape(nc, 9), pg.reshape(nc, 3)
+
+    def matvec(self, lvl, x):
+        y = np.empty_like(x)
+        self._lib.oracle2d_matvec(self._h, lvl, dp(x), dp(y))
+        return y
+
+    def defect(self, lvl, u, f):
+        d = np.empty_like(u)
+        r = C.c_double()
+        self._lib.oracle2d_defect(self._h, lvl, dp(u), dp(f), dp(d), C.byref(r))
+        return d, r.value
+
+    def cell_vanka(self, lvl, u, f):
+        u = u.copy()
+        info = self._lib.oracle2d_cell_vanka(self._h, lvl, dp(u), dp(f))
+        return u, info
+
+    def prolongate(self, fine, uc):
+        out = np.empty(self.n_dof(fine))
+        self._lib.oracle2d_prolongate(self._h, fine, dp(np.ascontiguousarray(uc)), dp(out))
+        return out
+
+    def restrict_defect(self, fine, d):
+        out = np.empty(self.n_dof(fine - 1))
+        d = np.ascontiguousarray(d).copy()
+        self._lib.oracle2d_restrict_defect(self._h, fine, dp(d), dp(out))
+        return out
+
+    def solve_exact(self, f):
+        u = np.empty_like(f)
+        info = self._lib.oracle2d_solve_exact(self._h, dp(u), dp(f))
+        return u, info
+
+    def vcycle(self, v):
+        z = np.empty_like(v)
+        st = self._lib.oracle2d_vcycle(self._h, dp(v), dp(z))
+        return z, st
+
+    def fgmres(self, sol, rhs):
+        it, r = C.c_int(), C.c_double()
+        st = self._lib.oracle2d_fgmres(self._h, dp(sol), dp(rhs), C.byref(it), C.byref(r))
+        return it.value, r.value, st
+
+    def patch_matrix(self, lvl, cell):
+        s = np.zeros((21, 21))  # s[col, row]
+        self._lib.oracle2d_patch_matrix(self._h, lvl, cell, dp(s))
         return s
