@@ -84,7 +84,7 @@ class PmgpConfig(C.Structure):
         ("perturb", C.c_double), ("seed_perturb", C.c_ulonglong),
         ("conv_kind", C.c_int), ("conv_const", C.c_double * 3), ("seed_conv", C.c_ulonglong),
         ("conv_max", C.c_double), ("rhs_kind", C.c_int), ("seed_rhs", C.c_ulonglong),
-        ("exact_kind", C.c_int), ("n_modes", C.c_int),
+        ("exact_kind", C.c_int), ("n_modes", C.c_int), ("mass", C.c_double),
     ]
 
 

@@ -449,6 +449,7 @@ void assemble(const pmgp_problem &P, Level &L) {
         for (int j = 0; j < 27; ++j) {
           double val = c.nu * (gx[i] * gx[j] + gy[i] * gy[j] + gz[i] * gz[j]);
           val += (b[0] * gx[j] + b[1] * gy[j] + b[2] * gz[j]) * t0;
+          if (c.mass != 0.0) val += c.mass * Bv[q][j] * t0;
           A[i * 27 + j] += mult * val;
         }
       }
@@ -511,6 +512,8 @@ void assemble(const pmgp_problem &P, Level &L) {
     if (c.exact_kind == PMGP_EXACT_QUADLIN) {
       double gu[3][3], lap[3], pp, gpv[3];
       exact_quadlin(&L.dof_xyz[3 * size_t(i)], u, gu, lap, pp, gpv);
+    } else if (c.conv_kind == PMGP_CONV_CHANNEL) {
+      conv_at(P, &L.dof_xyz[3 * size_t(i)], u);   // inflow profile, zero on the walls
     }
     for (int r = 0; r < 3; ++r) L.rhs[size_t(r) * L.n_udof + i] = u[r];
   }

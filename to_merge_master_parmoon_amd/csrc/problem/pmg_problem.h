@@ -18,6 +18,8 @@
 extern "C" {
 #endif
 
+/* PMGP_CONV_CHANNEL: b = (conv_max 16 y(1-y) z(1-z), 0, 0) with y, z scaled by ly, lz; the
+   same profile is the Dirichlet value of the velocity (inflow at x = 0, zero on the walls) */
 enum { PMGP_CONV_NONE = 0, PMGP_CONV_FOURIER = 1, PMGP_CONV_CONST = 2, PMGP_CONV_CHANNEL = 3 };
 enum { PMGP_RHS_ZERO = 0, PMGP_RHS_FOURIER = 1, PMGP_RHS_EXACT = 2 };
 enum { PMGP_EXACT_NONE = 0, PMGP_EXACT_QUADLIN = 1 };
@@ -38,6 +40,7 @@ typedef struct {
   unsigned long long seed_rhs;
   int exact_kind;              /* PMGP_EXACT_*: Dirichlet data and f from a known (u,p)  */
   int n_modes;                 /* Fourier modes for b and f (default 8)                  */
+  double mass;                 /* sigma of the sigma (u, v) term a time step adds to A   */
 } pmgp_config;
 
 typedef struct pmgp_problem pmgp_problem;

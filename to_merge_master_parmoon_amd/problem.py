@@ -15,6 +15,11 @@ configurations restate the benchmark workloads:
   U[-0.1h, 0.1h]^3 (seed 4000+i; the finer levels refine the perturbed trilinear cells),
   Fourier convection (seed 3000+i) and right-hand side (seed 6000+i); levels n, n/2, ...
   down to the first size <= 4 or odd (12 -> 6 -> 3, 20 -> 10 -> 5).
+* ``channel`` - one Crank-Nicolson step system of the time-dependent channel workload:
+  box [0,2.5] x [0,0.41]^2, hex cells 8k x k x k on the finest level (96 x 16 x 16 for
+  k = 12 ... 16), parabolic inflow with mean velocity 1 (max 2.25) at x = 0, no-slip walls,
+  natural outflow at x = 2.5, Re = 100 (nu = 0.0041), Oseen convection by the inflow profile,
+  sigma = 1 / (theta dt) = 200 mass term (dt = 0.01, theta = 1/2), no pressure projection.
 """
 from __future__ import annotations
 
@@ -166,3 +171,14 @@ def batched_specs(count: int = 64, sizes=(8, 12, 16, 20, 24)):
 
 def batched_oseen(count: int = 64, sizes=(8, 12, 16, 20, 24)) -> list[Problem]:
     return [batch_instance(n, nu, f, i) for i, (n, nu, f) in enumerate(batched_specs(count, sizes))]
+
+
+CHANNEL_DIRICHLET = 1 | 4 | 8 | 16 | 32     # all faces but the outflow x = 2.5
+
+
+def channel(n_levels: int = 4, coarse=(12, 2, 2), nu: float = 0.0041, u_max: float = 2.25,
+            dt: float = 0.01, theta: float = 0.5) -> Problem:
+    """The Crank-Nicolson step system of the channel workload (see the module docstring)."""
+    return Problem(cx=coarse[0], cy=coarse[1], cz=coarse[2], lx=2.5, ly=0.41, lz=0.41, n_levels=n_levels, nu=nu,
+                   conv_kind=_abi.CONV_CHANNEL, conv_max=u_max, rhs_kind=_abi.RHS_ZERO,
+                   dirichlet_faces=CHANNEL_DIRICHLET, mass=1.0 / (theta * dt))
