@@ -130,11 +130,13 @@ void merge_hierarchies(int n_inst, const pmg_instance_desc *inst, MergedHierarch
     ML.p_global.resize(size_t(nc) * NPL);
     ML.parent.assign(nc, -1);
     ML.child.assign(nc, 0);
+    ML.cell_problem.assign(nc, -1);
     for (const Part &p : parts[l]) {
       const pmg_level_desc *d = p.d;
       const Part *pc = p.li > 0 ? &parts[l - 1][part_of[l - 1][p.inst]] : nullptr;
       for (int cell = 0; cell < d->n_cells; ++cell) {
         const int mc = p.cell_off + cell;
+        ML.cell_problem[mc] = p.inst;
         ML.u_begin[mc] = mc * NUL;
         ML.p_begin[mc] = mc * NPL;
         for (int j = 0; j < NUL; ++j) {

@@ -29,6 +29,7 @@ struct MergedLevel {
   std::vector<int> ptr_colors, cell_reorder;  // swept cells by color, then the others
   std::vector<int> coarse;               // problems whose coarsest level this is
   std::vector<int> l20;                  // (offset, length) of every projected pressure block
+  std::vector<int> cell_problem;         // problem of every merged cell
 };
 
 struct MergedHierarchy {

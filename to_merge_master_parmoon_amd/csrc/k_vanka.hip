@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<T> op, const int *__res
 // items[] lists slot*32 + row sorted by the global row and each block takes one contiguous
 // run of it, so neighbouring rows share cache lines.
 template <typename T>
-__global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__restrict__ cells, int ncells,
+__global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__restrict__ cells, long n_items,
                                                    int pos0, const int *__restrict__ u_global,
                                                    const int *__restrict__ p_global,
                                                    const uint8_t *__restrict__ invperm,
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
   const T *__restrict__ u3 = u + 2 * N;
   const T *__restrict__ up = u + 3 * N;
   int i0, i1;
-  chunk_of_block(long(ncells) * (NU_LOC + NP_LOC), i0, i1);
+  chunk_of_block(n_items, i0, i1);
   for (int i = i0 + grp; i < i1; i += 16) {
     const int it = items[i];
     const int slot = it >> 5, r = it & 31;
@@ -430,7 +430,8 @@ __device__ __forceinline__ void sweeps_in_registers(v4u *s4, const F *sl0, const
 
 template <typename T, typename F, bool REG>
 __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, const int *__restrict__ cells,
-                                                    int ncells, int pos0, const T *__restrict__ rloc,
+                                                    int ncells, int pos0, const int *__restrict__ slots,
+                                                    const T *__restrict__ rloc,
                                                     const int *__restrict__ u_global,
                                                     const int *__restrict__ p_global, int N,
                                                     T *__restrict__ u, T damp) {
@@ -441,8 +442,8 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   // each lane reads its rows at constant per-step offsets from these bases
   const F *sl0 = reinterpret_cast<const F *>(s4) + lane;
   const F *sl1 = sl0 + 64;
-  const int slot = blockIdx.x;
-  if (slot >= ncells) return;
+  if (int(blockIdx.x) >= ncells) return;
+  const int slot = slots ? slots[blockIdx.x] : int(blockIdx.x);
   const int pos = pos0 + slot;
   const v4u *__restrict__ gL = reinterpret_cast<const v4u *>(lu + long(pos) * LU_STRIDE);
   const v4u *__restrict__ gU = reinterpret_cast<const v4u *>(lu + long(pos) * LU_STRIDE + LU_UOFF);
@@ -556,44 +557,43 @@ void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s) {
 }
 
 template <typename T>
-void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0, const int *u_global,
+void launch_vanka_residual(const OpView<T> &op, const int *cells, long n_items, int pos0, const int *u_global,
                            const int *p_global, const uint8_t *invperm, const int *items,
                            const T *u, const T *f, T *rloc, hipStream_t s) {
-  if (ncells <= 0) return;
+  if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
-  const long n_items = long(ncells) * (NU_LOC + NP_LOC);
   const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
-  k_vanka_res<T><<<grid, 256, 0, s>>>(op, cells, ncells, pos0, u_global, p_global, invperm, items, u, f, rloc);
+  k_vanka_res<T><<<grid, 256, 0, s>>>(op, cells, n_items, pos0, u_global, p_global, invperm, items, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template void launch_vanka_residual<double>(const OpView<double> &, const int *, int, int, const int *,
+template void launch_vanka_residual<double>(const OpView<double> &, const int *, long, int, const int *,
                                             const int *, const uint8_t *, const int *, const double *,
                                             const double *, double *, hipStream_t);
-template void launch_vanka_residual<float>(const OpView<float> &, const int *, int, int, const int *, const int *,
+template void launch_vanka_residual<float>(const OpView<float> &, const int *, long, int, const int *, const int *,
                                            const uint8_t *, const int *, const float *, const float *, float *,
                                            hipStream_t);
 
 template <typename T, typename F>
-void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc, const int *u_global,
-                        const int *p_global, int n_udof, T *u, T damp, hipStream_t s) {
+void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,
+                        const int *u_global, const int *p_global, int n_udof, T *u, T damp, hipStream_t s) {
   if (ncells <= 0) return;
   // FP32: colors of up to 2048 patches (a few waves per SIMD at most) are bound by the
   // dependent sweep chain and take the register-resident sweeps; larger colors need the
   // occupancy of the leaner LDS form
   if (sizeof(T) == 4 && ncells <= 2048)
     k_vanka_solve<T, F, sizeof(T) == 4><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(
-        lu, cells, ncells, pos0, rloc, u_global, p_global, n_udof, u, damp);
+        lu, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u, damp);
   else
-    k_vanka_solve<T, F, false><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(lu, cells, ncells, pos0, rloc,
-                                                                           u_global, p_global, n_udof, u, damp);
+    k_vanka_solve<T, F, false><<<ncells, 64, PackedSizes<F>::LDS_BYTES, s>>>(
+        lu, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u, damp);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template void launch_vanka_solve<double, double>(const double *, const int *, int, int, const double *,
+template void launch_vanka_solve<double, double>(const double *, const int *, int, int, const int *, const double *,
                                                  const int *, const int *, int, double *, double, hipStream_t);
-template void launch_vanka_solve<float, float>(const float *, const int *, int, int, const float *, const int *,
+template void launch_vanka_solve<float, float>(const float *, const int *, int, int, const int *, const float *, const int *,
                                                const int *, int, float *, float, hipStream_t);
-template void launch_vanka_solve<float, uint16_t>(const uint16_t *, const int *, int, int, const float *,
+template void launch_vanka_solve<float, uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
                                                   const int *, const int *, int, float *, float, hipStream_t);
 
 

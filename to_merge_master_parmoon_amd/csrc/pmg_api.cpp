@@ -76,6 +76,10 @@ struct pmg_ctx {
   // the V-cycle as a HIP graph (captured on first use, dropped when levels or parameters change)
   hipGraph_t vgraph = nullptr;
   hipGraphExec_t vexec = nullptr;
+  // problems the merged sweeps and coarse solves still cover (batch), and the number of
+  // patches they had at the last compaction
+  std::vector<char> b_live;
+  long b_live_cells = 0;
 };
 
 namespace {
@@ -420,24 +424,38 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
   if (relax) launch_copy<T>(aux, u, D.n_dof, c->stream);
   const OpView<T> op = opview<T>(D);
   for (int col = 0; col < D.n_colors; ++col) {
-    const int p0 = D.ptr_colors[col], nc = D.ptr_colors[col + 1] - p0;
+    const int p0 = D.ptr_colors[col];
+    // all patches of the color, or (compacted merged level) those of unfinished problems
+    int nc = D.ptr_colors[col + 1] - p0;
+    long n_items = long(nc) * (NU_LOC + NP_LOC);
+    const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
+    const int *slots = nullptr;
+    double part = 1.0;
+    if (D.compacted) {
+      n_items = D.act_items_off[col + 1] - D.act_items_off[col];
+      items = D.act_items + D.act_items_off[col];
+      slots = D.act_slots + D.act_slots_off[col];
+      part = nc ? double(D.act_slots_off[col + 1] - D.act_slots_off[col]) / nc : 0.0;
+      nc = int(D.act_slots_off[col + 1] - D.act_slots_off[col]);
+      if (nc == 0) continue;
+    }
     {
-      const double rb = D.res_bytes[vs == 8 ? 0 : 1][col];
+      const double rb = D.res_bytes[vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0);
-      launch_vanka_residual<T>(op, D.cell_reorder, nc, p0, D.u_global, D.p_global, D.invperm,
-                               D.res_items + size_t(p0) * (NU_LOC + NP_LOC), u, f, rloc, c->stream);
+      launch_vanka_residual<T>(op, D.cell_reorder, n_items, p0, D.u_global, D.p_global, D.invperm, items, u, f,
+                               rloc, c->stream);
     }
     Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));
     if constexpr (sizeof(T) == 8) {
-      launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global, D.n_udof, u,
-                                         damp, c->stream);
+      launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
+                                         D.n_udof, u, damp, c->stream);
     } else {
       if (c->prm.precision == PMG_PREC_FP32_BF16)
-        launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global,
+        launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                             D.n_udof, u, damp, c->stream);
       else
-        launch_vanka_solve<float, float>(D.lu32, D.cell_reorder, nc, p0, rloc, D.u_global, D.p_global, D.n_udof,
-                                         u, damp, c->stream);
+        launch_vanka_solve<float, float>(D.lu32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
+                                         D.n_udof, u, damp, c->stream);
     }
   }
   if (relax) launch_relax<T>(u, aux, T(alpha), D.n_dof, c->stream);
@@ -491,6 +509,7 @@ void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
 template <typename T>
 void batch_coarse(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
   for (int k : D.coarse_insts) {
+    if (!c->b_live.empty() && !c->b_live[k]) continue;   // finished: its input is zero
     BatchProblem &b = c->bi[k];
     if (b.co.pending) {
       PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, b.co.ev_done, 0));
@@ -924,6 +943,60 @@ void back_solve(std::vector<std::vector<double>> &H, std::vector<double> &s, int
   }
 }
 
+// Leave finished problems out of the merged sweeps and coarse solves.  Their FGMRES input
+// is zero from then on, so their V-cycle output stays zero either way; the lists are only
+// rebuilt (and the captured graph dropped) once the live patches fall below 3/4 of those
+// at the last rebuild, or when every problem is live again.
+void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
+  long cells = 0, total = 0;
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    const DevLevel &D = c->L[l];
+    total += D.n_cells;
+    for (int q = 0; q < D.n_cells; ++q) cells += live[D.h_pos_problem[q]] ? 1 : 0;
+  }
+  const bool all = cells == total;
+  if (all) {
+    bool any = false;
+    for (size_t l = 1; l < c->L.size(); ++l) any = any || c->L[l].compacted;
+    c->b_live.assign(live.size(), 1);
+    c->b_live_cells = total;
+    if (!any) return;
+    for (size_t l = 1; l < c->L.size(); ++l) c->L[l].compacted = false;
+    drop_graph(c);
+    return;
+  }
+  if (c->b_live_cells > 0 && cells * 4 > c->b_live_cells * 3) return;
+  for (size_t l = 1; l < c->L.size(); ++l) {
+    DevLevel &D = c->L[l];
+    if (!D.act_items) {
+      D.act_items = dalloc<int>(c, D.h_res_items.size());
+      D.act_slots = dalloc<int>(c, D.n_cells);
+    }
+    std::vector<int> items, slots;
+    items.reserve(D.h_res_items.size());
+    D.act_items_off.assign(1, 0);
+    D.act_slots_off.assign(1, 0);
+    for (int col = 0; col < D.n_colors; ++col) {
+      const int p0 = D.ptr_colors[col], p1 = D.ptr_colors[col + 1];
+      for (long q = long(p0) * (NU_LOC + NP_LOC); q < long(p1) * (NU_LOC + NP_LOC); ++q) {
+        const int it = D.h_res_items[q];
+        if (live[D.h_pos_problem[p0 + (it >> 5)]]) items.push_back(it);
+      }
+      for (int sl = 0; sl < p1 - p0; ++sl)
+        if (live[D.h_pos_problem[p0 + sl]]) slots.push_back(sl);
+      D.act_items_off.push_back(long(items.size()));
+      D.act_slots_off.push_back(long(slots.size()));
+    }
+    h2d(c, D.act_items, items.data(), items.size());
+    h2d(c, D.act_slots, slots.data(), slots.size());
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    D.compacted = true;
+  }
+  c->b_live = live;
+  c->b_live_cells = cells;
+  drop_graph(c);
+}
+
 // TFgmresIte::Iterate (FgmresIte.C:250-677) for every problem of a batch at once: the
 // problems share the iteration counter j and the restart cycle; the Arnoldi scalars,
 // Givens rotations, stopping and divergence tests are each problem's own.  A problem that
@@ -977,6 +1050,12 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
     PMG_HIP_CHECK(hipMemcpyAsync(dC, hC, sizeof(double) * nb * ldc, hipMemcpyHostToDevice, c->stream));
     PMG_HIP_CHECK(hipMemcpyAsync(c->bmask, hM, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
     launch_seg_lincomb(c->bX, c->Z, n, dC, ldc, m, c->bmask, c->seg_ch, c->n_seg_ch, c->stream);
+  };
+  std::vector<char> live(nb, 1);
+  batch_compact(c, live);
+  auto recompact = [&] {
+    for (int b = 0; b < nb; ++b) live[b] = B[b].active ? 1 : 0;
+    batch_compact(c, live);
   };
   defect_nse4_dev(c, c->bX, c->bB, v(0));
   norms(v(0));
@@ -1052,6 +1131,7 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
       PMG_HIP_CHECK(hipMemcpyAsync(dS, hS, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
       launch_seg_scale(v(i + 1), c->W, dS, c->seg_ch, c->n_seg_ch, c->stream);
       if (finish) combine(i + 1);
+      recompact();
     }
     if (!any_active()) break;
     // end of a restart cycle (or maxit): update with all restart columns, new residual
@@ -1078,7 +1158,11 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
         S.active = false;
       }
     }
+    recompact();
   }
+  // every problem live again for later calls on this context
+  std::fill(live.begin(), live.end(), 1);
+  batch_compact(c, live);
   for (int b = 0; b < nb; ++b) {
     BState &S = B[b];
     if (S.active) {
@@ -1360,6 +1444,7 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
       }
       D.res_items = dalloc<int>(c, items.size());
       h2d(c, D.res_items, items.data(), items.size());
+      if (merged) D.h_res_items = items;
     }
     {
       RowBytes rb;
@@ -1725,6 +1810,12 @@ int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg
       if (int r = add_level(c, &M.L[l].d, &lvl, true)) throw Fail{r, c->err};
       DevLevel &D = c->L[l];
       D.coarse_insts = M.L[l].coarse;
+      if (l > 0) {
+        std::vector<int> reo(D.n_cells);
+        PMG_HIP_CHECK(hipMemcpy(reo.data(), D.cell_reorder, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost));
+        D.h_pos_problem.resize(D.n_cells);
+        for (int q = 0; q < D.n_cells; ++q) D.h_pos_problem[q] = M.L[l].cell_problem[reo[q]];
+      }
       D.n_l20_seg = int(M.L[l].l20.size() / 2);
       if (D.n_l20_seg) {
         D.l20_seg = dalloc<int>(c, M.L[l].l20.size());

@@ -132,6 +132,13 @@ struct DevLevel {
   std::vector<int> coarse_insts;
   int *l20_seg = nullptr;
   int n_l20_seg = 0;
+  // merged levels with finished problems left out of the sweeps: per color, the residual
+  // work items and the patch slots of the problems still iterating
+  bool compacted = false;
+  std::vector<int> h_res_items;      // host copy of res_items
+  std::vector<int> h_pos_problem;    // problem of the cell at each color position
+  std::vector<long> act_items_off, act_slots_off;   // per color (n_colors + 1)
+  int *act_items = nullptr, *act_slots = nullptr;
 
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
@@ -189,14 +196,17 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
 // mixed path, computed in FP32 from the FP32 operator copy into lu32 or bf16 lu16
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 template <typename T>
-void launch_vanka_residual(const OpView<T> &op, const int *cells, int ncells, int pos0,
+// n_items (slot * 32 + local row) work items of one color starting at position pos0
+void launch_vanka_residual(const OpView<T> &op, const int *cells, long n_items, int pos0,
                            const int *u_global, const int *p_global, const uint8_t *invperm,
                            const int *items, const T *u, const T *f, T *rloc, hipStream_t s);
 // row-blocked operator values from the structure-of-arrays FP64 values
 template <typename T>
 void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s);
 template <typename T, typename F>
-void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const T *rloc,
+// ncells patches of one color: slots (may be null: 0..ncells-1) are their positions
+// relative to pos0 and their local residual slots
+void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,
                         const int *u_global, const int *p_global, int n_udof, T *u, T damp,
                         hipStream_t s);
 
