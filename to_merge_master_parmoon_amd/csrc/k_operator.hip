@@ -190,6 +190,31 @@ __global__ void k_dir(T *__restrict__ u, const T *__restrict__ f, int N, int sta
   const long o = long(c) * N + start + j;
   u[o] = f ? f[o] : T(0);
 }
+// IntoL20 of the P1disc constants in two passes without atomics: every block sums its
+// chunk of constants (k_l20_part), then every block adds all partials in the same order
+// and subtracts the mean from its chunk (k_l20_sub)
+template <typename T>
+__global__ __launch_bounds__(256) void k_l20_part(const T *__restrict__ p, int NP, double *__restrict__ part) {
+  __shared__ double sh[8];
+  const int nc = NP / 4;
+  const int c0 = int(long(nc) * blockIdx.x / gridDim.x), c1 = int(long(nc) * (blockIdx.x + 1) / gridDim.x);
+  double s = 0, d = 0;
+  for (int i = c0 + threadIdx.x; i < c1; i += 256) s += double(p[4 * i]);
+  block_sum2(s, d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_l20_sub(T *__restrict__ p, int NP, const double *__restrict__ part) {
+  __shared__ double sh[8];
+  double s = 0, d = 0;
+  for (int i = threadIdx.x; i < int(gridDim.x); i += 256) s += part[i];
+  block_sum2(s, d, sh);
+  const double m = s / double(NP) * 4.0;
+  const int nc = NP / 4;
+  const int c0 = int(long(nc) * blockIdx.x / gridDim.x), c1 = int(long(nc) * (blockIdx.x + 1) / gridDim.x);
+  for (int i = c0 + threadIdx.x; i < c1; i += 256) p[4 * i] = T(p[4 * i] - T(m));
+}
+
 // standalone IntoL20 of the P1disc constants (single block)
 template <typename T>
 __global__ __launch_bounds__(256) void k_l20(T *p, int NP) {
@@ -202,31 +227,39 @@ __global__ __launch_bounds__(256) void k_l20(T *p, int NP) {
 }
 
 // ------------------------------------------------------------------ BLAS-1 for FGMRES
-__global__ __launch_bounds__(256) void k_dot(const double *__restrict__ x, const double *__restrict__ y,
-                                             long n, double *out, double *partials, unsigned *counter) {
+// <x, y> and w -= h v ; <w, vnext>: each block writes its partial sum, k_sum_partials adds
+// them in block order (no per-block fence and counter atomic as a last-block finish needs)
+__global__ __launch_bounds__(256) void k_dot_part(const double *__restrict__ x, const double *__restrict__ y,
+                                                  long n, double *__restrict__ part) {
   __shared__ double sh[8];
-  double s = 0;
+  double s = 0, d = 0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += long(gridDim.x) * 256) s += x[i] * y[i];
-  double t, u;
-  if (last_block_sum2(s, 0.0, partials, counter, t, u, sh) && threadIdx.x == 0) *out = t;
+  block_sum2(s, d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// w -= h[0] v ; out = <w, vnext>
-__global__ __launch_bounds__(256) void k_axpy_dot(double *__restrict__ w, const double *h,
-                                                  const double *__restrict__ v,
-                                                  const double *__restrict__ vn, long n, double *out,
-                                                  double *partials, unsigned *counter) {
+__global__ __launch_bounds__(256) void k_axpy_dot_part(double *__restrict__ w, const double *h,
+                                                       const double *__restrict__ v, const double *__restrict__ vn,
+                                                       long n, double *__restrict__ part) {
   __shared__ double sh[8];
   const double a = -h[0];
-  double s = 0;
+  double s = 0, d = 0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += long(gridDim.x) * 256) {
     const double wi = w[i] + a * v[i];
     w[i] = wi;
     if (vn) s += wi * vn[i];
   }
   if (!vn) return;
-  double t, u;
-  if (last_block_sum2(s, 0.0, partials, counter, t, u, sh) && threadIdx.x == 0) *out = t;
+  block_sum2(s, d, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ part, int np, double *out) {
+  __shared__ double sh[8];
+  double s = 0, d = 0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  block_sum2(s, d, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
 __global__ void k_scale_dev(double *__restrict__ y, const double *__restrict__ x, const double *den, long n) {
   const double d = 1.0 / den[0];
   for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x)
@@ -260,6 +293,20 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
   const int grid = chunk_grid(long(op.n_udof) + op.n_pdof, 64, RED_BLOCKS);
   const int want_norm = res != nullptr;
   const int proj = (mode != 0 && project) ? 1 : 0;
+  if (proj && !want_norm) {
+    // the product without a grid-wide reduction (no per-block fence and counter atomic),
+    // then the projection in two small passes
+    if (mode == 1)
+      k_type4<1, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, 0, red.partials, red.counter, nullptr, nullptr);
+    else
+      k_type4<2, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, 0, red.partials, red.counter, nullptr, nullptr);
+    PMG_HIP_CHECK(hipGetLastError());
+    const int g = std::max(1, std::min(64, op.n_pdof / 4 / 2048));
+    k_l20_part<T><<<g, 256, 0, s>>>(y + 3L * op.n_udof, op.n_pdof, red.partials);
+    k_l20_sub<T><<<g, 256, 0, s>>>(y + 3L * op.n_udof, op.n_pdof, red.partials);
+    PMG_HIP_CHECK(hipGetLastError());
+    return;
+  }
   double *slots = red.result + 8;
   if (mode == 0)
     k_type4<0, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, want_norm, red.partials, red.counter, res, slots);
@@ -371,12 +418,16 @@ PMG_INST_VEC(float)
 template void launch_fill<int>(int *, int, long, hipStream_t);
 
 void launch_dot(const double *x, const double *y, long n, double *out, RedScratch &red, hipStream_t s) {
-  k_dot<<<grid_for(n, 256, 512), 256, 0, s>>>(x, y, n, out, red.partials, red.counter);
+  const int g = grid_for(n, 256, 512);
+  k_dot_part<<<g, 256, 0, s>>>(x, y, n, red.partials);
+  k_sum_partials<<<1, 256, 0, s>>>(red.partials, g, out);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_axpy_dev_dot(double *w, const double *h, const double *v, const double *vnext, long n, double *out,
                          RedScratch &red, hipStream_t s) {
-  k_axpy_dot<<<grid_for(n, 256, 512), 256, 0, s>>>(w, h, v, vnext, n, out, red.partials, red.counter);
+  const int g = grid_for(n, 256, 512);
+  k_axpy_dot_part<<<g, 256, 0, s>>>(w, h, v, vnext, n, red.partials);
+  if (vnext) k_sum_partials<<<1, 256, 0, s>>>(red.partials, g, out);
   PMG_HIP_CHECK(hipGetLastError());
 }
 void launch_scale_dev(double *y, const double *x, const double *den, long n, hipStream_t s) {
