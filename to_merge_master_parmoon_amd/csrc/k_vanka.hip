@@ -269,11 +269,9 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<T> op, const int *__res
 // items[] lists slot*32 + row sorted by the global row and each block takes one contiguous
 // run of it, so neighbouring rows share cache lines.
 template <typename T>
-__global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__restrict__ cells, long n_items,
-                                                   int pos0, const int *__restrict__ u_global,
-                                                   const int *__restrict__ p_global,
+__global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, int pos0,
                                                    const uint8_t *__restrict__ invperm,
-                                                   const int *__restrict__ items,
+                                                   const int *__restrict__ items, const int *__restrict__ item_rows,
                                                    const T *__restrict__ u, const T *__restrict__ f,
                                                    T *__restrict__ rloc) {
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
@@ -285,14 +283,15 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
   int i0, i1;
   chunk_of_block(n_items, i0, i1);
   for (int i = i0 + grp; i < i1; i += 16) {
-    const int it = items[i];
+    // the item and its global row are independent loads (the row was looked up through
+    // the cell's dof map at set-up)
+    const int it = items[i], grow = item_rows[i];
     const int slot = it >> 5, r = it & 31;
     const int pos = pos0 + slot;
-    const int cell = cells[pos];
     const uint8_t *ip = invperm + long(pos) * PERM_STRIDE;
     T *out = rloc + long(slot) * RLOC_STRIDE;
     if (r < NU_LOC) {
-      const int UDOF = u_global[long(cell) * NU_LOC + r];
+      const int UDOF = grow;
       T s1 = 0, s2 = 0, s3 = 0;
       const int k0 = op.a_rowptr[UDOF], n = op.a_rowptr[UDOF + 1] - k0;
       const int *__restrict__ kc = op.a_kcol + k0;
@@ -314,7 +313,7 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, const int *__re
       }
     } else {
       const int m = r - NU_LOC;
-      const int PDOF = p_global[long(cell) * NP_LOC + m];
+      const int PDOF = grow;
       T s = 0;
       const int k0 = op.b_rowptr[PDOF], n = op.b_rowptr[PDOF + 1] - k0;
       row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
@@ -445,6 +444,23 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   if (int(blockIdx.x) >= ncells) return;
   const int slot = slots ? slots[blockIdx.x] : int(blockIdx.x);
   const int pos = pos0 + slot;
+  const bool two = lane < NLOC - 64;  // lanes owning a second row 64 + lane
+  // offsets in u of this lane's rows, looked up ahead of the sweeps (off the chain's end)
+  const int cell = cells[pos];
+  int o0, o1 = 0;
+  {
+    const int c = lane / NU_LOC, j = lane - c * NU_LOC;   // lane < 64 < 81: velocity
+    o0 = c * N + u_global[long(cell) * NU_LOC + j];
+  }
+  if (two) {
+    const int i = 64 + lane;
+    if (i < 3 * NU_LOC) {
+      const int c = i / NU_LOC, j = i - c * NU_LOC;
+      o1 = c * N + u_global[long(cell) * NU_LOC + j];
+    } else {
+      o1 = 3 * N + p_global[long(cell) * NP_LOC + (i - 3 * NU_LOC)];
+    }
+  }
   const v4u *__restrict__ gL = reinterpret_cast<const v4u *>(lu + long(pos) * LU_STRIDE);
   const v4u *__restrict__ gU = reinterpret_cast<const v4u *>(lu + long(pos) * LU_STRIDE + LU_UOFF);
   {
@@ -458,7 +474,6 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
 #pragma unroll
   for (int q = 0; q < PS::RU; ++q) rU[q] = gU[lane + 64 * q];
   const T *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
-  const bool two = lane < NLOC - 64;  // lanes owning a second row 64 + lane
   T x0 = rs[lane];
   T x1 = two ? rs[64 + lane] : T(0);
   __syncthreads();
@@ -520,24 +535,8 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
     }
   }
   // u[UDOF] += damp * delta, p[PDOF] += damp * delta
-  const int cell = cells[pos];
-  {
-    const int i = lane;  // < 64 < 81: velocity
-    const int c = i / NU_LOC, j = i - c * NU_LOC;
-    const long o = long(c) * N + u_global[long(cell) * NU_LOC + j];
-    u[o] += damp * x0;
-  }
-  if (two) {
-    const int i = 64 + lane;
-    if (i < 3 * NU_LOC) {
-      const int c = i / NU_LOC, j = i - c * NU_LOC;
-      const long o = long(c) * N + u_global[long(cell) * NU_LOC + j];
-      u[o] += damp * x1;
-    } else {
-      const long o = 3L * N + p_global[long(cell) * NP_LOC + (i - 3 * NU_LOC)];
-      u[o] += damp * x1;
-    }
-  }
+  u[o0] += damp * x0;
+  if (two) u[o1] += damp * x1;
 }
 
 }  // namespace
@@ -557,22 +556,19 @@ void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s) {
 }
 
 template <typename T>
-void launch_vanka_residual(const OpView<T> &op, const int *cells, long n_items, int pos0, const int *u_global,
-                           const int *p_global, const uint8_t *invperm, const int *items,
-                           const T *u, const T *f, T *rloc, hipStream_t s) {
+void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
+                           const int *items, const int *item_rows, const T *u, const T *f, T *rloc, hipStream_t s) {
   if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
   const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
-  k_vanka_res<T><<<grid, 256, 0, s>>>(op, cells, n_items, pos0, u_global, p_global, invperm, items, u, f, rloc);
+  k_vanka_res<T><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, item_rows, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template void launch_vanka_residual<double>(const OpView<double> &, const int *, long, int, const int *,
-                                            const int *, const uint8_t *, const int *, const double *,
-                                            const double *, double *, hipStream_t);
-template void launch_vanka_residual<float>(const OpView<float> &, const int *, long, int, const int *, const int *,
-                                           const uint8_t *, const int *, const float *, const float *, float *,
-                                           hipStream_t);
+template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,
+                                            const int *, const double *, const double *, double *, hipStream_t);
+template void launch_vanka_residual<float>(const OpView<float> &, long, int, const uint8_t *, const int *,
+                                           const int *, const float *, const float *, float *, hipStream_t);
 
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,

@@ -429,11 +429,13 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     int nc = D.ptr_colors[col + 1] - p0;
     long n_items = long(nc) * (NU_LOC + NP_LOC);
     const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
+    const int *item_rows = D.res_rows + size_t(p0) * (NU_LOC + NP_LOC);
     const int *slots = nullptr;
     double part = 1.0;
     if (D.compacted) {
       n_items = D.act_items_off[col + 1] - D.act_items_off[col];
       items = D.act_items + D.act_items_off[col];
+      item_rows = D.act_rows + D.act_items_off[col];
       slots = D.act_slots + D.act_slots_off[col];
       part = nc ? double(D.act_slots_off[col + 1] - D.act_slots_off[col]) / nc : 0.0;
       nc = int(D.act_slots_off[col + 1] - D.act_slots_off[col]);
@@ -442,8 +444,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     {
       const double rb = D.res_bytes[vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0);
-      launch_vanka_residual<T>(op, D.cell_reorder, n_items, p0, D.u_global, D.p_global, D.invperm, items, u, f,
-                               rloc, c->stream);
+      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, item_rows, u, f, rloc, c->stream);
     }
     Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));
     if constexpr (sizeof(T) == 8) {
@@ -970,17 +971,22 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
     DevLevel &D = c->L[l];
     if (!D.act_items) {
       D.act_items = dalloc<int>(c, D.h_res_items.size());
+      D.act_rows = dalloc<int>(c, D.h_res_items.size());
       D.act_slots = dalloc<int>(c, D.n_cells);
     }
-    std::vector<int> items, slots;
+    std::vector<int> items, rows, slots;
     items.reserve(D.h_res_items.size());
+    rows.reserve(D.h_res_items.size());
     D.act_items_off.assign(1, 0);
     D.act_slots_off.assign(1, 0);
     for (int col = 0; col < D.n_colors; ++col) {
       const int p0 = D.ptr_colors[col], p1 = D.ptr_colors[col + 1];
       for (long q = long(p0) * (NU_LOC + NP_LOC); q < long(p1) * (NU_LOC + NP_LOC); ++q) {
         const int it = D.h_res_items[q];
-        if (live[D.h_pos_problem[p0 + (it >> 5)]]) items.push_back(it);
+        if (live[D.h_pos_problem[p0 + (it >> 5)]]) {
+          items.push_back(it);
+          rows.push_back(D.h_res_rows[q]);
+        }
       }
       for (int sl = 0; sl < p1 - p0; ++sl)
         if (live[D.h_pos_problem[p0 + sl]]) slots.push_back(sl);
@@ -988,6 +994,7 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
       D.act_slots_off.push_back(long(slots.size()));
     }
     h2d(c, D.act_items, items.data(), items.size());
+    h2d(c, D.act_rows, rows.data(), rows.size());
     h2d(c, D.act_slots, slots.data(), slots.size());
     PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
     D.compacted = true;
@@ -1429,7 +1436,7 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     D.cell_reorder = dalloc<int>(c, D.n_cells); h2d(c, D.cell_reorder, reo.data(), D.n_cells);
     {
       // residual work items of each color sorted by the operator row they stream
-      std::vector<int> items(size_t(D.n_cells) * (NU_LOC + NP_LOC));
+      std::vector<int> items(size_t(D.n_cells) * (NU_LOC + NP_LOC)), rows(items.size());
       std::vector<std::pair<int, int>> key;
       for (int k = 0; k < nc; ++k) {
         key.clear();
@@ -1440,11 +1447,21 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
             key.emplace_back(D.n_udof + pg[size_t(cell) * NP_LOC + j], slot * 32 + NU_LOC + j);
         }
         std::sort(key.begin(), key.end());
-        for (size_t i = 0; i < key.size(); ++i) items[size_t(ptr[k]) * (NU_LOC + NP_LOC) + i] = key[i].second;
+        for (size_t i = 0; i < key.size(); ++i) {
+          items[size_t(ptr[k]) * (NU_LOC + NP_LOC) + i] = key[i].second;
+          // global row: velocity dof, or pressure dof (keys of pressure rows are offset by n_udof)
+          rows[size_t(ptr[k]) * (NU_LOC + NP_LOC) + i] = key[i].first < D.n_udof ? key[i].first
+                                                                                 : key[i].first - D.n_udof;
+        }
       }
       D.res_items = dalloc<int>(c, items.size());
       h2d(c, D.res_items, items.data(), items.size());
-      if (merged) D.h_res_items = items;
+      D.res_rows = dalloc<int>(c, rows.size());
+      h2d(c, D.res_rows, rows.data(), rows.size());
+      if (merged) {
+        D.h_res_items = items;
+        D.h_res_rows = rows;
+      }
     }
     {
       RowBytes rb;

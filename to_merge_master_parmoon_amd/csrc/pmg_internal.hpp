@@ -98,6 +98,7 @@ struct DevLevel {
   // per color, the (slot * 32 + local row) work items of the residual kernel sorted by the
   // global row they read (velocity rows, then pressure rows)
   int *res_items = nullptr;
+  int *res_rows = nullptr;           // global row of every work item
   // cells
   int *u_global = nullptr;           // n_cells * 27 (UGlobalNumbers + UBeginIndex[cell])
   int *p_global = nullptr;           // n_cells * 4
@@ -135,10 +136,11 @@ struct DevLevel {
   // merged levels with finished problems left out of the sweeps: per color, the residual
   // work items and the patch slots of the problems still iterating
   bool compacted = false;
-  std::vector<int> h_res_items;      // host copy of res_items
+  std::vector<int> h_res_items;      // host copies of res_items, res_rows
+  std::vector<int> h_res_rows;
   std::vector<int> h_pos_problem;    // problem of the cell at each color position
   std::vector<long> act_items_off, act_slots_off;   // per color (n_colors + 1)
-  int *act_items = nullptr, *act_slots = nullptr;
+  int *act_items = nullptr, *act_rows = nullptr, *act_slots = nullptr;
 
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
@@ -196,10 +198,11 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
 // mixed path, computed in FP32 from the FP32 operator copy into lu32 or bf16 lu16
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 template <typename T>
-// n_items (slot * 32 + local row) work items of one color starting at position pos0
-void launch_vanka_residual(const OpView<T> &op, const int *cells, long n_items, int pos0,
-                           const int *u_global, const int *p_global, const uint8_t *invperm,
-                           const int *items, const T *u, const T *f, T *rloc, hipStream_t s);
+// n_items (slot * 32 + local row) work items of one color starting at position pos0, and
+// the global row of each (velocity or pressure dof)
+void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
+                           const int *items, const int *item_rows, const T *u, const T *f, T *rloc,
+                           hipStream_t s);
 // row-blocked operator values from the structure-of-arrays FP64 values
 template <typename T>
 void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s);
