@@ -170,6 +170,62 @@ def run_batched(args, log):
             "vs_baseline": None, "dtype": args.precision, "data": "synthetic", "config": cfg}
 
 
+def run_reassembly(args, log):
+    """One fixed-point step's device work before the solve on the 32^3 hierarchy: velocity at
+    the Gauss points of every level (pmg_eval_velocity), device re-assembly of A11..A33
+    (pmg_assemble_oseen) and the value-dependent set-up (pmg_refactor); timed beside the host
+    assembly of the same blocks by the problem generator (one process, all host threads)."""
+    import ctypes as C
+    from to_merge_master_parmoon_amd import problem
+    from to_merge_master_parmoon_amd.multigrid import DeviceVector, build_hierarchy
+    prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]
+    t0 = time.time()
+    prob = problem.oseen3d(args.n)
+    host_assembly_s = time.time() - t0
+    mg = build_hierarchy(prob, solver_params(prec))
+    L = prob.n_levels
+    for l in range(L):
+        mg.level_geometry(l, prob.cell_vertices(l))
+    lib, ctx = mg._lib, mg._ctx
+    u = [DeviceVector(mg, mg.n_dof[l], np.random.default_rng(l).standard_normal(mg.n_dof[l])) for l in range(L)]
+    bq = [DeviceVector(mg, prob.descs[l].n_cells * 81) for l in range(L)]
+
+    def step(refactor=True):
+        for l in range(L):
+            mg._check(lib.pmg_eval_velocity(ctx, l, u[l].ptr, bq[l].ptr, 1))
+            mg._check(lib.pmg_assemble_oseen(ctx, l, bq[l].ptr, 1e-2, 0.0, 1))
+        if refactor:
+            mg.refactor()
+
+    for _ in range(args.warmup):
+        step()
+    mg.synchronize()
+    times, times_asm = [], []
+    for k in range(args.steps):
+        mg.synchronize()
+        t = time.perf_counter()
+        step(refactor=False)
+        mg.synchronize()
+        times_asm.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        step()
+        mg.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"step {k}: eval + assembly {times_asm[-1] * 1e3:.2f} ms, with set-up {times[-1] * 1e3:.2f} ms")
+    mg.close()
+    nnz = prob.descs[-1].a_nnz
+    cells = prob.descs[-1].n_cells
+    return {"metric": f"Oseen3D {args.n}^3 Q2/P1disc fixed-point step device work s (eval + re-assembly + set-up)",
+            "value": round(float(np.mean(times)), 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(1e3 * float(np.mean(times)), 3), "higher_is_better": False,
+            "scaling": "strong", "vs_baseline": None, "dtype": "f64 assembly", "data": "synthetic",
+            "config": {"workload": f"oseen3d_{args.n}^3_reassembly_{L}_levels", "precision": args.precision,
+                       "eval_plus_assembly_s": round(float(np.mean(times_asm)), 5),
+                       "finest_cells": cells, "finest_nnz_per_block": nnz,
+                       "host_generator_assembly_s": round(host_assembly_s, 3),
+                       "host_generator_threads": os.cpu_count()}}
+
+
 def measured_traffic(kernel: str, precision: str, n: int):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (calibrated against a copy kernel), for the workload it was measured on."""
@@ -255,8 +311,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the FP64 GPU solve timed beside it")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--workload", choices=["oseen32", "batched"], default="oseen32",
-                    help="batched: the independent-problems workload instead of the headline solve")
+    ap.add_argument("--workload", choices=["oseen32", "batched", "reassembly"], default="oseen32",
+                    help="batched: the independent-problems workload; reassembly: one fixed-point step's "
+                         "device re-assembly and set-up (instead of the headline solve)")
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-maxit", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true", help="launch every V-cycle kernel by kernel")
@@ -270,6 +327,9 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         print(json.dumps(run_batched(args, log)), flush=True)
+        return
+    if args.workload == "reassembly":
+        print(json.dumps(run_reassembly(args, log)), flush=True)
         return
     from to_merge_master_parmoon_amd import problem
     prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]

@@ -196,6 +196,28 @@ int  pmg_fgmres_solve(pmg_ctx *ctx, double *sol, const double *rhs, int *iters,
 int  pmg_fgmres_history(const pmg_ctx *ctx, double *out, int cap);
 
 /* ----------------------------------------------------------------------------------- */
+/* device re-assembly of the linearised velocity blocks                                  */
+/* ----------------------------------------------------------------------------------- */
+/* Cell geometry of a level for the re-assembly: 8 corner vertices (x, y, z) per cell in
+   the order a + 2b + 4c of the trilinear map (a, b, c = 0/1 along xi, eta, zeta), the map
+   TFEDatabase3D applies for the hexahedra of the collection. */
+int  pmg_level_geometry(pmg_ctx *ctx, int level, const double *cell_vertices);
+/* The velocity of a level vector u ([u1 u2 u3 p]) at the 27 Gauss points of every cell:
+   b_quad[(cell * 27 + q) * 3 + c], q = i + 3 j + 9 k for the points (g_i, g_j, g_k),
+   g = (-sqrt(3/5), 0, sqrt(3/5)) - the 'param' values (u1old, u2old, u3old) that the
+   reference's assembly passes to NSType4Galerkin3D (src/FE/NSE3D_FixPo.C:1979-1981). */
+int  pmg_eval_velocity(pmg_ctx *ctx, int level, const double *u, double *b_quad, int mem);
+/* Replaces: re-assembly of A11..A33 by NSType4Galerkin3D (src/FE/NSE3D_FixPo.C:1919-2071)
+   in the fixed-point loop (src/System/SystemTNSE3D.C:725-795, 1400-1415) on the device:
+   A11 = A22 = A33 = nu (grad phi_j, grad phi_i) + ((b . grad) phi_j, phi_i)
+   + sigma (phi_j, phi_i) on the active rows (sigma: the mass term of a time step),
+   A12..A32 = 0, Dirichlet rows unchanged, with b given at the Gauss points as for
+   pmg_eval_velocity.  B and BT keep their values.  Marks the level's factors stale
+   (pmg_refactor or pmg_setup before the next solve). */
+int  pmg_assemble_oseen(pmg_ctx *ctx, int level, const double *b_quad, double nu, double sigma,
+                        int mem);
+
+/* ----------------------------------------------------------------------------------- */
 /* batched solves of independent problems                                                */
 /* ----------------------------------------------------------------------------------- */
 /* One independent problem of a batch: its level hierarchy (coarsest first, at least two

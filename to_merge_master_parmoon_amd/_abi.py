@@ -124,6 +124,9 @@ PMG_FUNCS = {
     "pmg_batch_fgmres_solve": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), c_int_p,
                                          c_double_p, c_int_p, C.c_int]),
     "pmg_batch_size": (C.c_int, [C.c_void_p]),
+    "pmg_level_geometry": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "pmg_eval_velocity": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "pmg_assemble_oseen": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int]),
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "pmg_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -147,6 +150,8 @@ PROBLEM_FUNCS = {
     "pmgp_cell_centers": (c_double_p, [C.c_void_p, C.c_int]),
     "pmgp_conv_scale": (C.c_double, [C.c_void_p]),
     "pmgp_level_dims": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p]),
+    "pmgp_cell_vertices": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "pmgp_conv_quad": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
 }
 
 

@@ -1498,6 +1498,13 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
       D.sol32 = dalloc<float>(c, D.n_dof); D.rhs32 = dalloc<float>(c, D.n_dof);
       D.aux0_32 = dalloc<float>(c, D.n_dof); D.aux1_32 = dalloc<float>(c, D.n_dof);
     }
+    {
+      // gather map of the patch matrices (patch LU, device re-assembly): every level
+      const std::vector<int> pm = build_patch_map(d, ug, pg);
+      D.pmap = dalloc<int>(c, pm.size());
+      h2d(c, D.pmap, pm.data(), pm.size());
+      PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
     if (lvl > 0) {
       const size_t lun = size_t(D.n_cells) * LU_STRIDE + LU_PAD;
       if (c->prm.precision == PMG_PREC_FP64) {
@@ -1511,11 +1518,6 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
         PMG_HIP_CHECK(hipMemsetAsync(D.lu16, 0, lun * sizeof(uint16_t), c->stream));
       }
       D.invperm = dalloc<uint8_t>(c, size_t(D.n_cells) * PERM_STRIDE);
-      {
-        const std::vector<int> pm = build_patch_map(d, ug, pg);
-        D.pmap = dalloc<int>(c, pm.size());
-        h2d(c, D.pmap, pm.data(), pm.size());
-      }
       D.patch_info = dalloc<int>(c, D.n_cells);
       D.rloc = dalloc<double>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
       if (mixed(c)) D.rloc32 = dalloc<float>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
@@ -1805,6 +1807,63 @@ int pmg_fgmres_solve(pmg_ctx *c, double *sol, const double *rhs, int *iters, dou
   if (rc) return rc;
   if (st == PMG_ERR_NOT_CONVERGED) c->err = "FGMRES not converged";
   return st;
+}
+
+int pmg_level_geometry(pmg_ctx *c, int level, const double *cell_vertices) {
+  if (!c || !cell_vertices) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (c->batch) throw Fail{PMG_ERR_UNSUPPORTED, "re-assembly of a batch context"};
+    DevLevel &D = level_of(c, level);
+    const size_t n = size_t(D.n_cells) * 24;
+    if (!D.geom) D.geom = dalloc<double>(c, n);
+    h2d(c, D.geom, cell_vertices, n);
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int pmg_eval_velocity(pmg_ctx *c, int level, const double *u, double *b_quad, int mem) {
+  if (!c || !u || !b_quad) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    const long nq = long(D.n_cells) * 81;
+    HostIO io(c, mem, std::max<long>(D.n_dof, nq));
+    const double *du = u;
+    double *db = b_quad;
+    if (mem == PMG_MEM_HOST) {
+      PMG_HIP_CHECK(hipMemcpyAsync(c->stage[0], u, sizeof(double) * D.n_dof, hipMemcpyHostToDevice, c->stream));
+      du = c->stage[0];
+      db = c->stage[2];
+    }
+    launch_eval_velocity(D, du, db, c->stream);
+    if (mem == PMG_MEM_HOST)
+      PMG_HIP_CHECK(hipMemcpyAsync(b_quad, db, sizeof(double) * nq, hipMemcpyDeviceToHost, c->stream));
+    io.sync();
+  });
+}
+
+int pmg_assemble_oseen(pmg_ctx *c, int level, const double *b_quad, double nu, double sigma, int mem) {
+  if (!c || !b_quad) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (c->batch) throw Fail{PMG_ERR_UNSUPPORTED, "re-assembly of a batch context"};
+    DevLevel &D = level_of(c, level);
+    if (!D.geom) throw Fail{PMG_ERR_NOT_READY, "pmg_level_geometry not called for this level"};
+    const long nq = long(D.n_cells) * 81;
+    HostIO io(c, mem, nq);
+    const double *db = b_quad;
+    if (mem == PMG_MEM_HOST) {
+      PMG_HIP_CHECK(hipMemcpyAsync(c->stage[0], b_quad, sizeof(double) * nq, hipMemcpyHostToDevice, c->stream));
+      db = c->stage[0];
+    }
+    check_deferred(c);
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream2));   // the coarse LU may still read the values
+    launch_assemble_oseen(D, D.geom, db, nu, sigma, c->stream);
+    launch_row_blocked<double>(D, D.av64, D.bv64, D.btv64, c->stream);
+    if (D.av32) launch_row_blocked<float>(D, D.av32, D.bv32, D.btv32, c->stream);
+    io.sync();
+    D.factors_ready = false;
+    if (level == 0) c->coarse.ready = false;
+    c->setup_done = false;
+  });
 }
 
 int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg_instance_desc *inst,

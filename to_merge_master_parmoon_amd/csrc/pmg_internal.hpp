@@ -141,6 +141,7 @@ struct DevLevel {
   std::vector<int> h_pos_problem;    // problem of the cell at each color position
   std::vector<long> act_items_off, act_slots_off;   // per color (n_colors + 1)
   int *act_items = nullptr, *act_rows = nullptr, *act_slots = nullptr;
+  double *geom = nullptr;            // n_cells * 8 corners * 3 (device re-assembly)
 
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
@@ -249,6 +250,13 @@ void launch_scale_dev(double *y, const double *x, const double *den, long n, hip
 void launch_lincomb(double *x, const double *zbase, long stride, const double *coef, int m, long n,
                     hipStream_t s);
 void launch_scal(double *y, const double *x, double a, long n, hipStream_t s);  // y = a x
+
+// device re-assembly (k_assemble.hip): A11 = A22 = A33 of the active rows from the element
+// matrices with convection b given at the 27 Gauss points of every cell (bq[cell][q][3]);
+// the Q2 velocity u of a level evaluated at those points
+void launch_assemble_oseen(const DevLevel &L, const double *geom, const double *bq, double nu, double sigma,
+                           hipStream_t s);
+void launch_eval_velocity(const DevLevel &L, const double *u, double *bq, hipStream_t s);
 
 // batched path (k_batch.hip): IntoL20 per (offset, length) pressure block; copies between a
 // problem's numbering and the merged numbering (dst[i] = src[idx[i]], dst[idx[i]] = src[i]);

@@ -687,4 +687,31 @@ int pmgp_level_dims(const pmgp_problem *p, int level, int *nx, int *ny, int *nz)
   return PMG_OK;
 }
 
+int pmgp_cell_vertices(const pmgp_problem *p, int level, double *out) {
+  const Level &L = p->levels[level];
+  for (int c = 0; c < L.n_cells; ++c) {
+    double X[8][3];
+    cell_corners(L, c, X);
+    for (int k = 0; k < 8; ++k)
+      for (int r = 0; r < 3; ++r) out[(size_t(c) * 8 + k) * 3 + r] = X[k][r];
+  }
+  return PMG_OK;
+}
+
+int pmgp_conv_quad(const pmgp_problem *p, int level, double *out) {
+  static const double gp[3] = {-0.7745966692414833770358531, 0.0, 0.7745966692414833770358531};
+  const Level &L = p->levels[level];
+  for (int c = 0; c < L.n_cells; ++c) {
+    double X[8][3];
+    cell_corners(L, c, X);
+    for (int q = 0; q < 27; ++q) {
+      double x[3], J[3][3], b[3];
+      trilinear(X, gp[q % 3], gp[(q / 3) % 3], gp[q / 9], x, J);
+      conv_at(*p, x, b);
+      for (int r = 0; r < 3; ++r) out[(size_t(c) * 27 + q) * 3 + r] = b[r];
+    }
+  }
+  return PMG_OK;
+}
+
 }  // extern "C"

@@ -63,6 +63,11 @@ const double *pmgp_cell_centers(const pmgp_problem *p, int level);
 double pmgp_conv_scale(const pmgp_problem *p);
 /* cells per direction of a level */
 int  pmgp_level_dims(const pmgp_problem *p, int level, int *nx, int *ny, int *nz);
+/* cell corners of a level (8 per cell, index a + 2b + 4c, 3 coordinates each) */
+int  pmgp_cell_vertices(const pmgp_problem *p, int level, double *out);
+/* the convection field of the configuration at the 27 Gauss points of every cell
+   (out[(cell * 27 + q) * 3 + c], q = i + 3j + 9k), the values the assembly used */
+int  pmgp_conv_quad(const pmgp_problem *p, int level, double *out);
 /* mass-lumped L2 helpers are not needed: errors are measured on dof vectors */
 
 #ifdef __cplusplus

@@ -168,6 +168,21 @@ class NSEMultiGrid:
         self._check(self._lib.pmg_coarse_solve(self._ctx, _ptr(u), _ptr(f), MEM_HOST), "coarse solve")
         return u
 
+    # ------------------------------------------------------------------ re-assembly
+    def level_geometry(self, level: int, cell_vertices: np.ndarray):
+        cv = np.ascontiguousarray(cell_vertices, dtype=np.float64)
+        self._check(self._lib.pmg_level_geometry(self._ctx, level, _ptr(cv)), "level_geometry")
+
+    def eval_velocity(self, level: int, u: np.ndarray, n_cells: int) -> np.ndarray:
+        out = np.empty(n_cells * 81)
+        self._check(self._lib.pmg_eval_velocity(self._ctx, level, _ptr(u), _ptr(out), MEM_HOST), "eval_velocity")
+        return out
+
+    def assemble_oseen(self, level: int, b_quad: np.ndarray, nu: float, sigma: float = 0.0):
+        """NSType4Galerkin3D on the device; call refactor() (or setup()) before solving."""
+        bq = np.ascontiguousarray(b_quad, dtype=np.float64)
+        self._check(self._lib.pmg_assemble_oseen(self._ctx, level, _ptr(bq), nu, sigma, MEM_HOST), "assemble_oseen")
+
     # ------------------------------------------------------------------ drivers
     def cycle(self, v: np.ndarray) -> np.ndarray:
         z = np.empty_like(v)

@@ -89,6 +89,19 @@ class Problem:
         d = self.descs[lvl]
         return self._vec(self._lib.pmgp_dof_coords, lvl, 3 * d.n_udof).reshape(-1, 3)
 
+    def cell_vertices(self, lvl: int = -1) -> np.ndarray:
+        lvl = lvl % self.n_levels
+        out = np.empty(self.descs[lvl].n_cells * 24)
+        self._lib.pmgp_cell_vertices(self._h, lvl, out.ctypes.data_as(_abi.c_double_p))
+        return out
+
+    def conv_quad(self, lvl: int = -1) -> np.ndarray:
+        """The configuration's convection field at the 27 Gauss points of every cell."""
+        lvl = lvl % self.n_levels
+        out = np.empty(self.descs[lvl].n_cells * 81)
+        self._lib.pmgp_conv_quad(self._h, lvl, out.ctypes.data_as(_abi.c_double_p))
+        return out
+
     def sizes(self, lvl: int = -1) -> dict:
         d = self.descs[lvl % self.n_levels]
         return dict(n_udof=d.n_udof, n_active=d.n_active, n_dirichlet=d.n_dirichlet, n_pdof=d.n_pdof,
