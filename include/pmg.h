@@ -75,6 +75,11 @@ typedef struct {
   int    bf16_refine_steps;                    /* must be 0: no local refinement; FGMRES refines */
   int    use_graph;                            /* 1: every V-cycle replays a HIP graph captured on
                                                   first use (default); 0: launched kernel by kernel */
+  int    shared_block;                         /* 1 (default): when A11 = A22 = A33 and the other
+                                                  A blocks vanish on the active rows (Galerkin Oseen
+                                                  form, NSE3D_FixPo.C:1919-2071), checked after every
+                                                  value change, the velocity rows stream that one
+                                                  block; 0: always the nine blocks */
 } pmg_params;
 
 /*
@@ -139,6 +144,10 @@ int  pmg_setup(pmg_ctx *ctx);
    complete solve including its set-up, call this instead of re-adding levels. */
 int  pmg_refactor(pmg_ctx *ctx);
 int  pmg_level_info(const pmg_ctx *ctx, int level, int *n_dof, int *n_colors, int *max_cells_per_color);
+/* *shared_block = 1 when the level's velocity rows stream one shared A block (see
+   pmg_params.shared_block), 0 when they stream the nine blocks.  No reference counterpart:
+   the reference always multiplies all nine blocks (MGComponents3D.C:3038-3209). */
+int  pmg_level_form(const pmg_ctx *ctx, int level, int *shared_block);
 
 /* ----------------------------------------------------------------------------------- */
 /* level operations (TNSE_MGLevel / TNSE_MGLevel4 virtuals and MGComponents3D.C)         */

@@ -43,6 +43,7 @@ class PmgParams(C.Structure):
         ("precision", C.c_int),
         ("bf16_refine_steps", C.c_int),
         ("use_graph", C.c_int),
+        ("shared_block", C.c_int),
     ]
 
 
@@ -105,6 +106,7 @@ PMG_FUNCS = {
     "pmg_setup": (C.c_int, [C.c_void_p]),
     "pmg_refactor": (C.c_int, [C.c_void_p]),
     "pmg_level_info": (C.c_int, [C.c_void_p, C.c_int, c_int_p, c_int_p, c_int_p]),
+    "pmg_level_form": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
     "pmg_defect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, c_double_p, C.c_int]),
     "pmg_matvec": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_defect_nse4": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),

@@ -66,7 +66,8 @@ __device__ inline bool last_block_sum2(double pa, double pb, double *partials, u
   return true;
 }
 
-template <int MODE, typename T>
+// SH: the velocity coupling is the shared block op.a1 (one value stream instead of nine)
+template <int MODE, typename T, bool SH>
 __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict__ x,
                                                const T *__restrict__ bv, T *__restrict__ y,
                                                int project, int want_norm, double *partials,
@@ -88,13 +89,17 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
       const int k0 = op.a_rowptr[r], n = op.a_rowptr[r + 1] - k0;
       const int *__restrict__ kc = op.a_kcol + k0;
       const T *__restrict__ v = op.av + 9L * k0;
-      if (r < NA) {
+      if constexpr (SH) {
+        row_a1<T>(kc, op.a1 + k0, n, lane, x1, x2, x3, s1, s2, s3);
+      } else if (r < NA) {
         row_a9<T>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
-        const int q0 = op.bt_rowptr[r], m = op.bt_rowptr[r + 1] - q0;
-        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, xp, s1, s2, s3);
       } else {
         // rows N_Active..N_UDOF: diagonal blocks only (MGComponents3D.C:3136-3156)
         row_a_diag<T, 1>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+      }
+      if (r < NA) {
+        const int q0 = op.bt_rowptr[r], m = op.bt_rowptr[r + 1] - q0;
+        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, xp, s1, s2, s3);
       }
       s1 = red16(s1);
       s2 = red16(s2);
@@ -293,13 +298,23 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
   const int grid = chunk_grid(long(op.n_udof) + op.n_pdof, 64, RED_BLOCKS);
   const int want_norm = res != nullptr;
   const int proj = (mode != 0 && project) ? 1 : 0;
+  auto go = [&](auto mode_c, int pr, int wn, double *rs, double *sl) {
+    constexpr int M = decltype(mode_c)::value;
+    if (op.a1)
+      k_type4<M, T, true><<<grid, 256, 0, s>>>(op, x, b, y, pr, wn, red.partials, red.counter, rs, sl);
+    else
+      k_type4<M, T, false><<<grid, 256, 0, s>>>(op, x, b, y, pr, wn, red.partials, red.counter, rs, sl);
+  };
+  using M0 = std::integral_constant<int, 0>;
+  using M1 = std::integral_constant<int, 1>;
+  using M2 = std::integral_constant<int, 2>;
   if (proj && !want_norm) {
     // the product without a grid-wide reduction (no per-block fence and counter atomic),
     // then the projection in two small passes
     if (mode == 1)
-      k_type4<1, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, 0, red.partials, red.counter, nullptr, nullptr);
+      go(M1{}, 0, 0, nullptr, nullptr);
     else
-      k_type4<2, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, 0, red.partials, red.counter, nullptr, nullptr);
+      go(M2{}, 0, 0, nullptr, nullptr);
     PMG_HIP_CHECK(hipGetLastError());
     const int g = std::max(1, std::min(64, op.n_pdof / 4 / 2048));
     k_l20_part<T><<<g, 256, 0, s>>>(y + 3L * op.n_udof, op.n_pdof, red.partials);
@@ -309,11 +324,11 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
   }
   double *slots = red.result + 8;
   if (mode == 0)
-    k_type4<0, T><<<grid, 256, 0, s>>>(op, x, b, y, 0, want_norm, red.partials, red.counter, res, slots);
+    go(M0{}, 0, want_norm, res, slots);
   else if (mode == 1)
-    k_type4<1, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res, slots);
+    go(M1{}, proj, want_norm, res, slots);
   else
-    k_type4<2, T><<<grid, 256, 0, s>>>(op, x, b, y, proj, want_norm, red.partials, red.counter, res, slots);
+    go(M2{}, proj, want_norm, res, slots);
   PMG_HIP_CHECK(hipGetLastError());
   if (proj) {
     const int nc = op.n_pdof / 4;
@@ -345,7 +360,35 @@ __global__ void k_row_blocked(const int *__restrict__ rowptr, int rows, const do
       for (int b = 0; b < NB; ++b) out[long(b) * n + j] = T(src[long(b) * nnz + k0 + j]);
   }
 }
+// 1 in part[block] when the block's entries break the shared velocity block form: on every
+// row A22 = A11 and A33 = A11, on the active rows (entries [0, n_act)) also A_rc = 0 for
+// r != c.  NaN compares unequal, so such values keep the nine-block form.
+__global__ __launch_bounds__(256) void k_shared_check(const double *__restrict__ a, long nnz, long n_act,
+                                                      double *__restrict__ part) {
+  int bad = 0;
+  for (long k = blockIdx.x * 256L + threadIdx.x; k < nnz; k += long(gridDim.x) * 256) {
+    const double v = a[k];
+    bad |= (a[4 * nnz + k] != v) | (a[8 * nnz + k] != v);
+    if (k < n_act)
+      bad |= (a[nnz + k] != 0.0) | (a[2 * nnz + k] != 0.0) | (a[3 * nnz + k] != 0.0) |
+             (a[5 * nnz + k] != 0.0) | (a[6 * nnz + k] != 0.0) | (a[7 * nnz + k] != 0.0);
+  }
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) part[blockIdx.x] = bad ? 1.0 : 0.0;
+}
 }  // namespace
+
+bool shared_velocity_block(const DevLevel &L, double *part, hipStream_t s) {
+  constexpr int G = 256;
+  k_shared_check<<<G, 256, 0, s>>>(L.a, L.a_nnz, L.a_act_entries, part);
+  PMG_HIP_CHECK(hipGetLastError());
+  double h[G];
+  PMG_HIP_CHECK(hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, s));
+  PMG_HIP_CHECK(hipStreamSynchronize(s));
+  for (int g = 0; g < G; ++g)
+    if (h[g] != 0.0) return false;
+  return true;
+}
 
 template <typename T>
 void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s) {

@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<T> op, const int *__res
 // components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061).
 // items[] lists slot*32 + row sorted by the global row and each block takes one contiguous
 // run of it, so neighbouring rows share cache lines.
-template <typename T>
+template <typename T, bool SH>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, int pos0,
                                                    const uint8_t *__restrict__ invperm,
                                                    const int *__restrict__ items, const int *__restrict__ item_rows,
@@ -296,12 +296,16 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
       const int k0 = op.a_rowptr[UDOF], n = op.a_rowptr[UDOF + 1] - k0;
       const int *__restrict__ kc = op.a_kcol + k0;
       const T *__restrict__ v = op.av + 9L * k0;
-      if (UDOF < NA) {
+      if constexpr (SH) {
+        row_a1<T>(kc, op.a1 + k0, n, lane, u1, u2, u3, s1, s2, s3);
+      } else if (UDOF < NA) {
         row_a9<T>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
-        const int q0 = op.bt_rowptr[UDOF], m = op.bt_rowptr[UDOF + 1] - q0;
-        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
       } else {
         row_a_diag<T, 1>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
+      }
+      if (UDOF < NA) {
+        const int q0 = op.bt_rowptr[UDOF], m = op.bt_rowptr[UDOF + 1] - q0;
+        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
       }
       s1 = red16(s1);
       s2 = red16(s2);
@@ -562,7 +566,10 @@ void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const ui
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
   const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
-  k_vanka_res<T><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, item_rows, u, f, rloc);
+  if (op.a1)
+    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, item_rows, u, f, rloc);
+  else
+    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, item_rows, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,

@@ -139,13 +139,14 @@ struct Timed {
 // right-hand side read, output written; gathers of x counted as cache hits), for values of
 // vs bytes.  Dirichlet rows read only the diagonal blocks, as the kernels do.
 struct RowBytes {
-  double vel(const pmg_level_desc *d, int i, int vs) const {
+  // shared: one velocity block instead of nine (three on Dirichlet rows)
+  double vel(const pmg_level_desc *d, int i, int vs, bool shared) const {
     const double nA = d->a_rowptr[i + 1] - d->a_rowptr[i];
     if (i < d->n_active) {
       const double nBT = d->bt_rowptr[i + 1] - d->bt_rowptr[i];
-      return nA * (9.0 * vs + 4) + nBT * (3.0 * vs + 4) + 16;
+      return nA * ((shared ? 1.0 : 9.0) * vs + 4) + nBT * (3.0 * vs + 4) + 16;
     }
-    return nA * (3.0 * vs + 4) + 8;
+    return nA * ((shared ? 1.0 : 3.0) * vs + 4) + 8;
   }
   double pre(const pmg_level_desc *d, int i, int vs) const {
     return double(d->b_rowptr[i + 1] - d->b_rowptr[i]) * (3.0 * vs + 4) + 8;
@@ -306,15 +307,30 @@ void check_coarse(pmg_ctx *c, Coarse &C) {
   if (e) throw Fail{PMG_ERR_DEVICE, "coarse triangular sweep: wait bound exceeded"};
 }
 
+// the streamed forms of new operator values (CSR arrays D.a, D.b, D.bt on the device):
+// row-blocked FP64 (and FP32) copies and, when the A blocks have it, the shared velocity
+// block form (FP64: block 0 of D.a read in place; FP32: its own copy)
+void refresh_values(pmg_ctx *c, DevLevel &D) {
+  launch_row_blocked<double>(D, D.av64, D.bv64, D.btv64, c->stream);
+  if (D.av32) launch_row_blocked<float>(D, D.av32, D.bv32, D.btv32, c->stream);
+  D.a_shared = c->prm.shared_block && shared_velocity_block(D, c->red.partials, c->stream);
+  if (D.a_shared && D.av32) {
+    if (!D.a1_32) {
+      D.a1_32 = dalloc<float>(c, size_t(D.a_nnz) + ROW_PAD);
+      launch_fill<float>(D.a1_32 + D.a_nnz, 0.0f, ROW_PAD, c->stream);
+    }
+    launch_d2f(D.a1_32, D.a, D.a_nnz, c->stream);
+  }
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
 void upload_values(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d) {
   for (int b = 0; b < 9; ++b) h2d(c, D.a + size_t(b) * D.a_nnz, d->a[b], D.a_nnz);
   for (int b = 0; b < 3; ++b) {
     h2d(c, D.b + size_t(b) * D.b_nnz, d->b[b], D.b_nnz);
     h2d(c, D.bt + size_t(b) * D.bt_nnz, d->bt[b], D.bt_nnz);
   }
-  launch_row_blocked<double>(D, D.av64, D.bv64, D.btv64, c->stream);
-  if (D.av32) launch_row_blocked<float>(D, D.av32, D.bv32, D.btv32, c->stream);
-  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  refresh_values(c, D);
 }
 
 bool mixed(const pmg_ctx *c) { return c->prm.precision != PMG_PREC_FP64; }
@@ -325,6 +341,7 @@ void upload_operator(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d, bool fp32
   D.n_udof = d->n_udof; D.n_active = d->n_active; D.hanging_bound = d->hanging_bound;
   D.n_dirichlet = d->n_dirichlet; D.n_pdof = d->n_pdof; D.n_dof = 3 * d->n_udof + d->n_pdof;
   D.a_nnz = d->a_nnz; D.b_nnz = d->b_nnz; D.bt_nnz = d->bt_nnz;
+  D.a_act_entries = d->a_rowptr[d->n_active];
   D.a_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.a_rowptr, d->a_rowptr, D.n_udof + 1);
   // the streamed arrays (column indices, row-blocked values) carry ROW_PAD zeroed entries
   // after their end: the row kernels' 4-entry windows of rows shorter than 4 read past it
@@ -442,7 +459,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
       if (nc == 0) continue;
     }
     {
-      const double rb = D.res_bytes[vs == 8 ? 0 : 1][col] * part;
+      const double rb = D.res_bytes[D.a_shared][vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0);
       launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, item_rows, u, f, rloc, c->stream);
     }
@@ -470,7 +487,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
 // res_dev (single problem only): ||d||_2 after the projection
 template <typename T>
 void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, double *res_dev) {
-  Timed tm(c, &c->tm.defect, D.defect_bytes[sizeof(T) == 8 ? 0 : 1], 0.0);
+  Timed tm(c, &c->tm.defect, D.defect_bytes[D.a_shared][sizeof(T) == 8 ? 0 : 1], 0.0);
   launch_type4<T>(opview<T>(D), 2, u, f, d, fused_l20(c), c->batch ? nullptr : res_dev, c->red, c->stream);
   if (c->batch) project_pressure<T>(c, D, d);
 }
@@ -738,7 +755,7 @@ void setup_factors(pmg_ctx *c) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
     c->tm.patch_lu.calls += 1;
-    c->tm.patch_lu.bytes += double(D.n_cells) * (NLOC * NLOC * 8.0) + D.defect_bytes[0];
+    c->tm.patch_lu.bytes += double(D.n_cells) * (NLOC * NLOC * 8.0) + D.defect_bytes[0][0];
     c->tm.patch_lu.flops += double(D.n_cells) * (2.0 / 3.0) * NLOC * NLOC * NLOC;
     for (int p = 0; p < D.n_cells; ++p)
       if (infos[l][p])
@@ -785,7 +802,7 @@ double dev_norm2(pmg_ctx *c, const double *x, long n) {
 
 void defect_nse4_dev(pmg_ctx *c, const double *x, const double *b, double *r) {
   DevLevel &F = c->L.back();
-  Timed tm(c, &c->tm.matvec, F.defect_bytes[0], 0.0);
+  Timed tm(c, &c->tm.matvec, F.defect_bytes[F.a_shared][0], 0.0);
   launch_type4<double>(F.op64(), 1, x, b, r, fused_l20(c), nullptr, c->red, c->stream);
   if (c->batch) project_pressure<double>(c, F, r);
 }
@@ -863,7 +880,7 @@ int fgmres_dev(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *f
     for (int i = 0; i < restart && j <= maxite; i++, j++) {
       for (int k = 0; k < std::max(1, P.sc_amg_prec_it); ++k) vcycle_dev(c, v(i), z(i));
       {
-        Timed tm(c, &c->tm.matvec, F.matvec_bytes, 0.0);
+        Timed tm(c, &c->tm.matvec, F.matvec_bytes[F.a_shared], 0.0);
         launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
       }
       // modified Gram-Schmidt, H[k][i] -> Hdev[k], ||w||^2 -> Hdev[i+1]
@@ -1088,7 +1105,7 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
     for (int i = 0; i < restart && j <= maxit && any_active(); i++, j++) {
       for (int k = 0; k < std::max(1, P.sc_amg_prec_it); ++k) vcycle_dev(c, v(i), z(i));
       {
-        Timed tm(c, &c->tm.matvec, F.matvec_bytes, 0.0);
+        Timed tm(c, &c->tm.matvec, F.matvec_bytes[F.a_shared], 0.0);
         launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
       }
       // modified Gram-Schmidt: H[k][i] -> bH[b*ldh + k], ||w||^2 -> bH[b*ldh + i + 1]
@@ -1253,6 +1270,7 @@ void pmg_params_default(pmg_params *p) {
   p->precision = PMG_PREC_FP64;
   p->bf16_refine_steps = 0;
   p->use_graph = 1;
+  p->shared_block = 1;
 }
 
 const char *pmg_status_string(int s) {
@@ -1465,29 +1483,30 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     }
     {
       RowBytes rb;
-      for (int pi = 0; pi < 2; ++pi) {
-        const int vs = pi == 0 ? 8 : 4;
-        double vb = 0;
-        for (int i = 0; i < D.n_udof; ++i) vb += rb.vel(d, i, vs);
-        double pb = 0;
-        for (int i = 0; i < D.n_pdof; ++i) pb += rb.pre(d, i, vs);
-        // defect: rows + x read + f read + d write; matvec (FP64 only): no f
-        D.defect_bytes[pi] = vb + pb + 3.0 * vs * D.n_dof;
-        if (pi == 0) D.matvec_bytes = vb + pb + 2.0 * vs * D.n_dof;
-        D.res_bytes[pi].assign(nc, 0.0);
-        for (int k = 0; k < nc; ++k) {
-          double b = 0;
-          for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
-            const int cell = reo[q];
-            b += 4 + (NU_LOC + NP_LOC) * 4.0;  // cell index, work items
-            // row, f, local residual, dof index, inverse-permutation byte
-            for (int j = 0; j < NU_LOC; ++j)
-              b += rb.vel(d, ug[size_t(cell) * NU_LOC + j], vs) + 3 * vs + 3 * vs + 4 + 3;
-            for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], vs) + vs + vs + 4 + 1;
+      for (int sh = 0; sh < 2; ++sh)
+        for (int pi = 0; pi < 2; ++pi) {
+          const int vs = pi == 0 ? 8 : 4;
+          double vb = 0;
+          for (int i = 0; i < D.n_udof; ++i) vb += rb.vel(d, i, vs, sh);
+          double pb = 0;
+          for (int i = 0; i < D.n_pdof; ++i) pb += rb.pre(d, i, vs);
+          // defect: rows + x read + f read + d write; matvec (FP64 only): no f
+          D.defect_bytes[sh][pi] = vb + pb + 3.0 * vs * D.n_dof;
+          if (pi == 0) D.matvec_bytes[sh] = vb + pb + 2.0 * vs * D.n_dof;
+          D.res_bytes[sh][pi].assign(nc, 0.0);
+          for (int k = 0; k < nc; ++k) {
+            double b = 0;
+            for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
+              const int cell = reo[q];
+              b += 4 + (NU_LOC + NP_LOC) * 4.0;  // cell index, work items
+              // row, f, local residual, dof index, inverse-permutation byte
+              for (int j = 0; j < NU_LOC; ++j)
+                b += rb.vel(d, ug[size_t(cell) * NU_LOC + j], vs, sh) + 3 * vs + 3 * vs + 4 + 3;
+              for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], vs) + vs + vs + 4 + 1;
+            }
+            D.res_bytes[sh][pi][k] = b;
           }
-          D.res_bytes[pi][k] = b;
         }
-      }
     }
     // vectors
     D.sol = dalloc<double>(c, D.n_dof); D.rhs = dalloc<double>(c, D.n_dof);
@@ -1595,6 +1614,12 @@ int pmg_refactor(pmg_ctx *c) {
     for (auto &b : c->bi) b.co.ready = false;
     setup_factors(c);
   });
+}
+
+int pmg_level_form(const pmg_ctx *c, int level, int *shared_block) {
+  if (!c || !shared_block || level < 0 || level >= int(c->L.size())) return PMG_ERR_INVALID_ARGUMENT;
+  *shared_block = c->L[level].a_shared ? 1 : 0;
+  return PMG_OK;
 }
 
 int pmg_level_info(const pmg_ctx *c, int level, int *n_dof, int *n_colors, int *mcpc) {
@@ -1857,8 +1882,7 @@ int pmg_assemble_oseen(pmg_ctx *c, int level, const double *b_quad, double nu, d
     check_deferred(c);
     PMG_HIP_CHECK(hipStreamSynchronize(c->stream2));   // the coarse LU may still read the values
     launch_assemble_oseen(D, D.geom, db, nu, sigma, c->stream);
-    launch_row_blocked<double>(D, D.av64, D.bv64, D.btv64, c->stream);
-    if (D.av32) launch_row_blocked<float>(D, D.av32, D.bv32, D.btv32, c->stream);
+    refresh_values(c, D);
     io.sync();
     D.factors_ready = false;
     if (level == 0) c->coarse.ready = false;

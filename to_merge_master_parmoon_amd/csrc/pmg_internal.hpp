@@ -61,6 +61,9 @@ struct OpView {
   const T *bt;
   long bt_nnz;
   const T *av, *bv, *btv;   // row-blocked values
+  // the shared velocity block in CSR order when A11 = A22 = A33 and A_rc = 0 (r != c) on
+  // the active rows (the Galerkin Oseen form); nullptr: the nine blocks differ
+  const T *a1;
 };
 
 // Streaming kernels split their n items (rows, residual work items) into one contiguous
@@ -89,12 +92,17 @@ struct DevLevel {
   int n_udof = 0, n_active = 0, hanging_bound = 0, n_dirichlet = 0, n_pdof = 0, n_dof = 0;
   int n_cells = 0;
   long a_nnz = 0, b_nnz = 0, bt_nnz = 0;
+  long a_act_entries = 0;            // entries of the active rows, a_rowptr[n_active]
   // operator, FP64 and (mixed precision) FP32 copies
   int *a_rowptr = nullptr, *a_kcol = nullptr, *b_rowptr = nullptr, *b_kcol = nullptr;
   int *bt_rowptr = nullptr, *bt_kcol = nullptr;
   double *a = nullptr, *b = nullptr, *bt = nullptr;
   double *av64 = nullptr, *bv64 = nullptr, *btv64 = nullptr;   // row-blocked values
   float *av32 = nullptr, *bv32 = nullptr, *btv32 = nullptr;
+  // shared velocity block (OpView::a1): set by the value check after every change of the
+  // values; FP64 reads block 0 of a, FP32 its own copy
+  bool a_shared = false;
+  float *a1_32 = nullptr;
   // per color, the (slot * 32 + local row) work items of the residual kernel sorted by the
   // global row they read (velocity rows, then pressure rows)
   int *res_items = nullptr;
@@ -125,8 +133,9 @@ struct DevLevel {
   std::vector<int> h_a_rowptr, h_a_kcol, h_b_rowptr, h_b_kcol, h_bt_rowptr, h_bt_kcol;
   // algorithmic bytes (FP64 operator) of the per-color residual launches and of one defect
   // [0]: FP64 values and vectors, [1]: FP32
-  std::vector<double> res_bytes[2];
-  double defect_bytes[2] = {0, 0}, matvec_bytes = 0;
+  // [s][p]: s = 1 with the shared velocity block, p = 0 FP64, 1 FP32
+  std::vector<double> res_bytes[2][2];
+  double defect_bytes[2][2] = {{0, 0}, {0, 0}}, matvec_bytes[2] = {0, 0};
   double transfer_bytes = 0;
   // merged (batched) levels: the problems solved directly on this level, and the
   // (offset, length) pressure blocks that get their own IntoL20
@@ -146,12 +155,12 @@ struct DevLevel {
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
                           b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz,
-                          av64, bv64, btv64};
+                          av64, bv64, btv64, a_shared ? a : nullptr};
   }
   OpView<float> op32() const {
     return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, nullptr, a_nnz,
                          b_rowptr, b_kcol, nullptr, b_nnz, bt_rowptr, bt_kcol, nullptr, bt_nnz,
-                         av32, bv32, btv32};
+                         av32, bv32, btv32, a_shared ? a1_32 : nullptr};
   }
 };
 
@@ -207,6 +216,9 @@ void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const ui
 // row-blocked operator values from the structure-of-arrays FP64 values
 template <typename T>
 void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s);
+// true when the level's A blocks have the shared form of OpView::a1 (part: 256 doubles of
+// device scratch; synchronizes the stream)
+bool shared_velocity_block(const DevLevel &L, double *part, hipStream_t s);
 template <typename T, typename F>
 // ncells patches of one color: slots (may be null: 0..ncells-1) are their positions
 // relative to pos0 and their local residual slots

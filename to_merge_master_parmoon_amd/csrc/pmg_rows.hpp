@@ -88,6 +88,38 @@ __device__ inline void row_a9_v(const int *__restrict__ kc, const T *__restrict_
   }
 }
 
+// A row whose velocity coupling is one shared block (A11 = A22 = A33, A_rc = 0 for r != c):
+// s_r += A(row, col) x_r(col), one value stream for the three components
+template <typename T>
+__device__ inline void row_a1_v(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                                const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
+                                T &s1, T &s2, T &s3) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    bool ok[4];
+    const int ld = row_window(j0, n, lane, ok);
+    int c[4];
+    ld4(kc + ld, c);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = ok[u] ? c[u] : 0;
+    T a[4];
+    ld4(v + ld, a);
+    T y1[4], y2[4], y3[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      y1[u] = x1[c[u]];
+      y2[u] = x2[c[u]];
+      y3[u] = x3[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T w = ok[u] ? a[u] : T(0);
+      s1 += w * y1[u];
+      s2 += w * y2[u];
+      s3 += w * y3[u];
+    }
+  }
+}
+
 // A row of a Dirichlet dof: only the diagonal blocks A11, A22, A33
 template <typename T, int U>
 __device__ inline void row_a_diag(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
@@ -200,6 +232,42 @@ __device__ inline void row_a9_s(const int *__restrict__ kc, const T *__restrict_
   }
 }
 
+// A row with one shared velocity block (see row_a1_v)
+template <typename T, int U>
+__device__ inline void row_a1_s(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                                const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
+                                T &s1, T &s2, T &s3) {
+  for (int j0 = 0; j0 < n; j0 += 16 * U) {
+    int jj[U];
+    bool ok[U];
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + 16 * u + lane;
+      ok[u] = j < n;
+      jj[u] = ok[u] ? j : n - 1;
+      c[u] = kc[jj[u]];
+    }
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = v[jj[u]];
+    T y1[U], y2[U], y3[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      y1[u] = x1[c[u]];
+      y2[u] = x2[c[u]];
+      y3[u] = x3[c[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T w = ok[u] ? a[u] : T(0);
+      s1 += w * y1[u];
+      s2 += w * y2[u];
+      s3 += w * y3[u];
+    }
+  }
+}
+
 // BT row (velocity row, pressure columns): s_r += BT_r(row, q) p(q)
 template <typename T, int U>
 __device__ inline void row_bt_s(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
@@ -277,6 +345,15 @@ __device__ inline void row_a9(const int *__restrict__ kc, const T *__restrict__ 
     row_a9_v<T>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
   else
     row_a9_s<T, 4>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+}
+template <typename T>
+__device__ inline void row_a1(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
+                              const T *__restrict__ x1, const T *__restrict__ x2, const T *__restrict__ x3,
+                              T &s1, T &s2, T &s3) {
+  if constexpr (sizeof(T) == 8)
+    row_a1_v<T>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
+  else
+    row_a1_s<T, 4>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
 }
 template <typename T>
 __device__ inline void row_bt(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,

@@ -111,6 +111,13 @@ class NSEMultiGrid:
         self._check(self._lib.pmg_level_info(self._ctx, level, C.byref(n), C.byref(nc), C.byref(mc)))
         return dict(n_dof=n.value, n_colors=nc.value, max_cells_per_color=mc.value)
 
+    def shared_block(self, level: int) -> bool:
+        """True when the level's velocity rows stream one shared A block (A11 = A22 = A33,
+        no coupling blocks on the active rows), False when they stream all nine."""
+        sb = C.c_int()
+        self._check(self._lib.pmg_level_form(self._ctx, level, C.byref(sb)), "pmg_level_form")
+        return bool(sb.value)
+
     @property
     def n_levels(self) -> int:
         return len(self.n_dof)
