@@ -80,6 +80,13 @@ typedef struct {
                                                   form, NSE3D_FixPo.C:1919-2071), checked after every
                                                   value change, the velocity rows stream that one
                                                   block; 0: always the nine blocks */
+  int    patch_inverse;                        /* mixed precision only.  1 (default): each local
+                                                  Vanka solve multiplies by the explicit inverse of
+                                                  the patch matrix (FP32 Gauss-Jordan with the pivots
+                                                  of dgetf2, stored in the factor precision); 0: the
+                                                  stored LU factors and triangular sweeps.  FP64
+                                                  always uses the LU factors (dgetrf/dgetrs,
+                                                  LinAlg.C:106-130) */
 } pmg_params;
 
 /*

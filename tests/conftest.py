@@ -43,6 +43,14 @@ def params(**kw):
     return _abi.default_params(**base)
 
 
+def unpin_pressure(x, n_udof):
+    """x with the mean of the P1disc constants (every 4th pressure entry) removed: with
+    Dirichlet data on the whole boundary the pressure is fixed only up to a constant."""
+    y = np.array(x, dtype=np.float64, copy=True)
+    y[3 * n_udof::4] -= y[3 * n_udof::4].mean()
+    return y
+
+
 def rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)

@@ -14,7 +14,7 @@ streams one value array for the three velocity components instead of nine.  Pins
 import numpy as np
 import pytest
 
-from conftest import params, rel
+from conftest import params, rel, unpin_pressure
 
 pytestmark = pytest.mark.gpu
 
@@ -114,7 +114,10 @@ def test_forms_agree_on_oseen(oseen16, precision):
     x1, x0 = np.zeros_like(f), np.zeros_like(f)
     it1 = mg1.fgmres(x1, f)[0]
     it0 = mg0.fgmres(x0, f)[0]
-    assert abs(it1 - it0) <= 1 and rel(x1, x0) <= 1e-9
+    # both meet the relative residual 1e-10; the constant pressure is left where each
+    # preconditioner puts it
+    nu = oseen16.descs[-1].n_udof
+    assert abs(it1 - it0) <= 1 and rel(unpin_pressure(x1, nu), unpin_pressure(x0, nu)) <= 1e-7
     mg1.close()
     mg0.close()
 

@@ -44,6 +44,7 @@ class PmgParams(C.Structure):
         ("bf16_refine_steps", C.c_int),
         ("use_graph", C.c_int),
         ("shared_block", C.c_int),
+        ("patch_inverse", C.c_int),
     ]
 
 

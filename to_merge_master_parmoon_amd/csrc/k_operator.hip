@@ -459,6 +459,7 @@ void launch_l20(T *p, int NP, RedScratch &, hipStream_t s) {
 PMG_INST_VEC(double)
 PMG_INST_VEC(float)
 template void launch_fill<int>(int *, int, long, hipStream_t);
+template void launch_fill<uint16_t>(uint16_t *, uint16_t, long, hipStream_t);
 
 void launch_dot(const double *x, const double *y, long n, double *out, RedScratch &red, hipStream_t s) {
   const int g = grid_for(n, 256, 512);

@@ -69,6 +69,12 @@ __device__ __forceinline__ T patch_entry(const OpView<T> &op, const int *__restr
       li = PM_NA + j;
       blk = 3 * rr + cc;
       dir_off = rr != cc;
+      if (op.a1) {
+        // shared velocity block: A22 and A33 hold the values of A11, the coupling blocks
+        // are zero (no gathers for them)
+        if (rr != cc) mi = -1;
+        blk = 0;
+      }
     } else if (c < NLOC) {
       mi = PM_BT + j * NP_LOC + (c - 3 * NU_LOC);
       li = PM_MT + j;
@@ -107,14 +113,15 @@ struct StoreF<uint16_t> {
   }
 };
 
-// one block of 16 columns (jb = JB) of the register-tiled dgetf2 in k_patch_lu
+// Step j = 16 JB + jl of dgetf2 on the register tiles, first half: pivot search in column
+// j (first row of max |a(i, j)|, i >= j), then the swap of rows j and p over all columns.
+// Returns whether the pivot is nonzero.
 template <int JB, typename T>
-__device__ __forceinline__ void lu_columns(T (&A)[6][6], int t, int tx, int ty, int *sigma, T *lbuf, T *ubuf,
-                                           int &piv_s, int &nz_s, int &info_s) {
+__device__ __forceinline__ bool pivot_and_swap(T (&A)[6][6], int t, int tx, int ty, int jl, int *sigma, int &piv_s,
+                                               int &nz_s, int &info_s) {
   constexpr int jb = JB;
-  const int jl_end = jb == 5 ? NLOC - 80 : 16;
-  for (int jl = 0; jl < jl_end; ++jl) {
-    const int j = 16 * jb + jl;
+  const int j = 16 * jb + jl;
+  {
     // pivot: first row of max |a(i, j)|, i >= j
     if (tx == jl) {
       T bv = T(-1);
@@ -179,6 +186,19 @@ __device__ __forceinline__ void lu_columns(T (&A)[6][6], int t, int tx, int ty, 
         }
       }
     }
+    return nonzero;
+  }
+}
+
+// one block of 16 columns (jb = JB) of the register-tiled dgetf2 in k_patch_lu
+template <int JB, typename T>
+__device__ __forceinline__ void lu_columns(T (&A)[6][6], int t, int tx, int ty, int *sigma, T *lbuf, T *ubuf,
+                                           int &piv_s, int &nz_s, int &info_s) {
+  constexpr int jb = JB;
+  const int jl_end = jb == 5 ? NLOC - 80 : 16;
+  for (int jl = 0; jl < jl_end; ++jl) {
+    const int j = 16 * jb + jl;
+    const bool nonzero = pivot_and_swap<JB, T>(A, t, tx, ty, jl, sigma, piv_s, nz_s, info_s);
     // scale column j below the diagonal by the reciprocal of the pivot, publish l and u
     if (tx == jl) {
       if (nonzero) {
@@ -205,6 +225,49 @@ __device__ __forceinline__ void lu_columns(T (&A)[6][6], int t, int tx, int ty, 
 #pragma unroll
       for (int b = jb; b < 6; ++b)
         if (ty + 16 * a > j && tx + 16 * b > j) A[a][b] = fma(-lv[a], uv[b], A[a][b]);
+  }
+}
+
+// One block of 16 columns of Gauss-Jordan inversion on the same register tiles (FP32,
+// mixed precision): the pivot search and row swaps of dgetf2, then row j is scaled by the
+// reciprocal of the pivot and eliminated from every other row, with column j taking the
+// corresponding column of the inverse (in-place Gauss-Jordan).  After the last column the
+// tiles hold (P A)^{-1} for the row permutation P of dgetf2, so x = A^{-1} r = M (P r)
+// with the pivot-permuted local residual the residual kernel writes.  Column j and row j
+// are published unscaled through LDS; every thread forms the reciprocal itself.
+template <int JB>
+__device__ __forceinline__ void gj_columns(float (&A)[6][6], int t, int tx, int ty, int *sigma, float *lbuf,
+                                           float *ubuf, int &piv_s, int &nz_s, int &info_s) {
+  constexpr int jb = JB;
+  const int jl_end = jb == 5 ? NLOC - 80 : 16;
+  for (int jl = 0; jl < jl_end; ++jl) {
+    const int j = 16 * jb + jl;
+    const bool nonzero = pivot_and_swap<JB, float>(A, t, tx, ty, jl, sigma, piv_s, nz_s, info_s);
+    if (tx == jl) {
+#pragma unroll
+      for (int a = 0; a < 6; ++a) lbuf[ty + 16 * a] = A[a][jb];
+    }
+    if (ty == jl) {
+#pragma unroll
+      for (int b = 0; b < 6; ++b) ubuf[tx + 16 * b] = A[jb][b];
+    }
+    __syncthreads();
+    const float rc = nonzero ? 1.0f / lbuf[j] : 0.0f;
+    float lv[6], uv[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) lv[a] = lbuf[ty + 16 * a];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      const int c = tx + 16 * b;
+      uv[b] = c == j ? rc : ubuf[c] * rc;
+    }
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b < 6; ++b) {
+        const int r = ty + 16 * a, c = tx + 16 * b;
+        A[a][b] = r == j ? uv[b] : fmaf(-lv[a], uv[b], c == j ? 0.0f : A[a][b]);
+      }
   }
 }
 
@@ -263,15 +326,106 @@ __global__ __launch_bounds__(256) void k_patch_lu(OpView<T> op, const int *__res
   if (t == 0) info[pos] = info_s;
 }
 
+// Storage of an explicit patch inverse M (85 x MINV_LD, 85 columns used) for k_vanka_apply,
+// where lane i owns rows i and 64 + i and loads them as 16-byte vectors of PER entries:
+// vector q of row r < 64 sits at (q * 64 + r) * PER, of row r >= 64 at
+// 64 * MINV_LD + (q * 21 + r - 64) * PER, so every vector load of a wavefront reads one
+// contiguous run.
+template <typename F>
+__host__ __device__ constexpr int minv_off(int r, int c) {
+  constexpr int PER = 16 / int(sizeof(F));
+  return r < 64 ? ((c / PER) * 64 + r) * PER + c % PER
+                : 64 * MINV_LD + ((c / PER) * (NLOC - 64) + r - 64) * PER + c % PER;
+}
+
+// Patch matrix and its explicit inverse (mixed precision): the gathers of k_patch_lu, FP32
+// Gauss-Jordan with the pivots of dgetf2, M = (P A)^{-1} stored in the layout of minv_off
+// (the 3 padding columns zero) through an LDS copy of 16 rows at a time.
+template <typename F>
+__global__ __launch_bounds__(256) void k_patch_inv(OpView<float> op, const int *__restrict__ cells,
+                                                   const int *__restrict__ pmap, int ncells, F *__restrict__ minv,
+                                                   uint8_t *__restrict__ invperm, int *__restrict__ info) {
+  __shared__ int sigma[NLOC];
+  __shared__ float lbuf[96], ubuf[96];
+  __shared__ float stage[16][97];
+  __shared__ int piv_s, info_s, nz_s;
+  const int pos = blockIdx.x;
+  if (pos >= ncells) return;
+  const int t = threadIdx.x;
+  const int tx = t >> 4, ty = t & 15;
+  const int *__restrict__ mp = pmap + long(cells[pos]) * PM_STRIDE;
+  if (t < NLOC) sigma[t] = t;
+  if (t == 0) info_s = 0;
+  float A[6][6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) A[a][b] = patch_entry<float>(op, mp, ty + 16 * a, tx + 16 * b);
+  __syncthreads();
+  gj_columns<0>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  gj_columns<1>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  gj_columns<2>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  gj_columns<3>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  gj_columns<4>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  gj_columns<5>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
+  F *out = minv + long(pos) * MINV_STRIDE;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < 6; ++b) stage[ty][tx + 16 * b] = A[a][b];
+    __syncthreads();
+    // rows 16a .. 16a + 15 (those below 85), MINV_LD entries each; consecutive threads take
+    // consecutive entries of a vector, then the same vector of the next row
+    constexpr int PER = 16 / int(sizeof(F));
+    const int nr = a == 5 ? NLOC - 80 : 16;
+    for (int e = t; e < nr * MINV_LD; e += 256) {
+      const int q = e / (nr * PER), w = e - q * nr * PER;
+      const int rr = w / PER, c = q * PER + (w - rr * PER);
+      out[minv_off<F>(16 * a + rr, c)] = StoreF<F>::get(stage[rr][c]);
+    }
+  }
+  if (t < NLOC) invperm[long(pos) * PERM_STRIDE + sigma[t]] = uint8_t(t);
+  if (t == 0) info[pos] = info_s;
+}
+
 // ------------------------------------------------------------------ per-color residual
 // (slot, local row) items, 16 lanes each: rows 0..26 produce the three velocity
 // components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061).
 // items[] lists slot*32 + row sorted by the global row and each block takes one contiguous
-// run of it, so neighbouring rows share cache lines.
+// run of it, so neighbouring rows share cache lines.  Every item carries a descriptor
+// (global row, first A or B entry, first BT entry, A/B length | BT length << 16; BT length
+// 0 on Dirichlet rows) built once by k_res_desc, and a group loads the next item's
+// descriptor while it works on the current one: an item costs two dependent memory round
+// trips (row entries, then the gathers of u), not four.
+__global__ void k_res_desc(const int *__restrict__ a_rowptr, const int *__restrict__ bt_rowptr,
+                           const int *__restrict__ b_rowptr, int n_active, const int *__restrict__ items,
+                           const int *__restrict__ rows, long n, int *__restrict__ desc) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const int r = items[i] & 31, g = rows[i];
+  int k0, len, q0 = 0, m = 0;
+  if (r < NU_LOC) {
+    k0 = a_rowptr[g];
+    len = a_rowptr[g + 1] - k0;
+    q0 = bt_rowptr[g];
+    if (g < n_active) m = bt_rowptr[g + 1] - q0;
+  } else {
+    k0 = b_rowptr[g];
+    len = b_rowptr[g + 1] - k0;
+  }
+  pmg_vi4 d;
+  d.x = g;
+  d.y = k0;
+  d.z = q0;
+  d.w = int(unsigned(len) | (unsigned(m) << 16));
+  __builtin_memcpy(desc + 4 * i, &d, 16);
+}
+
 template <typename T, bool SH>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, int pos0,
                                                    const uint8_t *__restrict__ invperm,
-                                                   const int *__restrict__ items, const int *__restrict__ item_rows,
+                                                   const int *__restrict__ items, const int *__restrict__ desc,
                                                    const T *__restrict__ u, const T *__restrict__ f,
                                                    T *__restrict__ rloc) {
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
@@ -282,47 +436,54 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
   const T *__restrict__ up = u + 3 * N;
   int i0, i1;
   chunk_of_block(n_items, i0, i1);
-  for (int i = i0 + grp; i < i1; i += 16) {
-    // the item and its global row are independent loads (the row was looked up through
-    // the cell's dof map at set-up)
-    const int it = items[i], grow = item_rows[i];
+  int i = i0 + grp;
+  int it_n = 0;
+  pmg_vi4 d_n = {0, 0, 0, 0};
+  if (i < i1) {
+    it_n = items[i];
+    __builtin_memcpy(&d_n, desc + 4L * i, 16);
+  }
+  for (; i < i1; i += 16) {
+    const int it = it_n;
+    const pmg_vi4 d = d_n;
+    if (i + 16 < i1) {
+      it_n = items[i + 16];
+      __builtin_memcpy(&d_n, desc + 4L * (i + 16), 16);
+    }
     const int slot = it >> 5, r = it & 31;
+    const int grow = d.x, k0 = d.y, q0 = d.z;
+    const int n = int(unsigned(d.w) & 0xffffu), m = int(unsigned(d.w) >> 16);
     const int pos = pos0 + slot;
     const uint8_t *ip = invperm + long(pos) * PERM_STRIDE;
     T *out = rloc + long(slot) * RLOC_STRIDE;
     if (r < NU_LOC) {
       const int UDOF = grow;
+      const T f1 = f[UDOF], f2 = f[N + UDOF], f3 = f[2 * N + UDOF];
       T s1 = 0, s2 = 0, s3 = 0;
-      const int k0 = op.a_rowptr[UDOF], n = op.a_rowptr[UDOF + 1] - k0;
       const int *__restrict__ kc = op.a_kcol + k0;
-      const T *__restrict__ v = op.av + 9L * k0;
-      if constexpr (SH) {
+      if constexpr (SH)
         row_a1<T>(kc, op.a1 + k0, n, lane, u1, u2, u3, s1, s2, s3);
-      } else if (UDOF < NA) {
-        row_a9<T>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
-      } else {
-        row_a_diag<T, 1>(kc, v, n, lane, u1, u2, u3, s1, s2, s3);
-      }
-      if (UDOF < NA) {
-        const int q0 = op.bt_rowptr[UDOF], m = op.bt_rowptr[UDOF + 1] - q0;
-        row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
-      }
+      else if (UDOF < NA)
+        row_a9<T>(kc, op.av + 9L * k0, n, lane, u1, u2, u3, s1, s2, s3);
+      else
+        row_a_diag<T, 1>(kc, op.av + 9L * k0, n, lane, u1, u2, u3, s1, s2, s3);
+      if (m > 0) row_bt<T>(op.bt_kcol + q0, op.btv + 3L * q0, m, lane, up, s1, s2, s3);
       s1 = red16(s1);
       s2 = red16(s2);
       s3 = red16(s3);
       if (lane == 0) {
-        out[ip[r]] = f[UDOF] - s1;
-        out[ip[NU_LOC + r]] = f[N + UDOF] - s2;
-        out[ip[2 * NU_LOC + r]] = f[2 * N + UDOF] - s3;
+        out[ip[r]] = f1 - s1;
+        out[ip[NU_LOC + r]] = f2 - s2;
+        out[ip[2 * NU_LOC + r]] = f3 - s3;
       }
     } else {
-      const int m = r - NU_LOC;
+      const int mm = r - NU_LOC;
       const int PDOF = grow;
+      const T fp = f[3 * N + PDOF];
       T s = 0;
-      const int k0 = op.b_rowptr[PDOF], n = op.b_rowptr[PDOF + 1] - k0;
       row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
       s = red16(s);
-      if (lane == 0) out[ip[3 * NU_LOC + m]] = f[3 * N + PDOF] - s;
+      if (lane == 0) out[ip[3 * NU_LOC + mm]] = fp - s;
     }
   }
 }
@@ -543,6 +704,73 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   if (two) u[o1] += damp * x1;
 }
 
+// x = M (P r) for one patch per wavefront, M the explicit inverse of k_patch_inv: lane i
+// owns rows i and 64 + i (i < 21), loads them as 16-byte vectors up front (no dependent
+// chain: every product is independent; non-temporal, so the stream of inverses does not
+// evict the vectors the next residual gathers from L2), the local residual is broadcast
+// from the lanes holding it, then u[dof] += damp * x.
+template <typename F>
+__global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, const int *__restrict__ cells,
+                                                    int ncells, int pos0, const int *__restrict__ slots,
+                                                    const float *__restrict__ rloc, const int *__restrict__ u_global,
+                                                    const int *__restrict__ p_global, int N, float *__restrict__ u,
+                                                    float damp) {
+  constexpr int NV = MINV_LD * int(sizeof(F)) / 16;   // 16-byte vectors per row
+  constexpr int PER = 16 / int(sizeof(F));             // entries per vector
+  const int lane = threadIdx.x;
+  if (int(blockIdx.x) >= ncells) return;
+  const int slot = slots ? slots[blockIdx.x] : int(blockIdx.x);
+  const int pos = pos0 + slot;
+  const bool two = lane < NLOC - 64;
+  const F *__restrict__ mp = minv + long(pos) * MINV_STRIDE;
+  v4u m0[NV], m1[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q)
+    m0[q] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(mp + minv_off<F>(lane, q * PER)));
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    m1[q] = v4u{0u, 0u, 0u, 0u};
+    if (two) m1[q] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(mp + minv_off<F>(64 + lane, q * PER)));
+  }
+  const int cell = cells[pos];
+  int o0, o1 = 0;
+  {
+    const int c = lane / NU_LOC, j = lane - c * NU_LOC;
+    o0 = c * N + u_global[long(cell) * NU_LOC + j];
+  }
+  if (two) {
+    const int i = 64 + lane;
+    if (i < 3 * NU_LOC) {
+      const int c = i / NU_LOC, j = i - c * NU_LOC;
+      o1 = c * N + u_global[long(cell) * NU_LOC + j];
+    } else {
+      o1 = 3 * N + p_global[long(cell) * NP_LOC + (i - 3 * NU_LOC)];
+    }
+  }
+  const float *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
+  const float r0 = rs[lane];
+  const float r1 = two ? rs[64 + lane] : 0.0f;
+  float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < MINV_LD; ++j) {
+    const float rj = j < 64 ? bcast(r0, j) : bcast(r1, j - 64);   // lanes >= 21 hold r1 = 0
+    const int q = j / PER, k = j % PER;
+    const unsigned w0 = m0[q][sizeof(F) == 2 ? k / 2 : k], w1 = m1[q][sizeof(F) == 2 ? k / 2 : k];
+    float a0, a1;
+    if constexpr (sizeof(F) == 2) {
+      a0 = __uint_as_float(k % 2 ? (w0 & 0xffff0000u) : (w0 << 16));
+      a1 = __uint_as_float(k % 2 ? (w1 & 0xffff0000u) : (w1 << 16));
+    } else {
+      a0 = __uint_as_float(w0);
+      a1 = __uint_as_float(w1);
+    }
+    acc0 = fmaf(a0, rj, acc0);
+    acc1 = fmaf(a1, rj, acc1);
+  }
+  u[o0] += damp * acc0;
+  if (two) u[o1] += damp * acc1;
+}
+
 }  // namespace
 
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s) {
@@ -559,17 +787,47 @@ void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s) {
   PMG_HIP_CHECK(hipGetLastError());
 }
 
+void launch_res_desc(const DevLevel &L, const int *items, const int *rows, long n, int *desc, hipStream_t s) {
+  if (n <= 0) return;
+  k_res_desc<<<int((n + 255) / 256), 256, 0, s>>>(L.a_rowptr, L.bt_rowptr, L.b_rowptr, L.n_active, items, rows, n,
+                                                  desc);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s) {
+  if (L.n_cells <= 0) return;
+  if (L.minv32)
+    k_patch_inv<float><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv32, L.invperm,
+                                                 info);
+  else
+    k_patch_inv<uint16_t><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv16,
+                                                    L.invperm, info);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename F>
+void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
+                        const int *u_global, const int *p_global, int n_udof, float *u, float damp, hipStream_t s) {
+  if (ncells <= 0) return;
+  k_vanka_apply<F><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u, damp);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_vanka_apply<float>(const float *, const int *, int, int, const int *, const float *, const int *,
+                                        const int *, int, float *, float, hipStream_t);
+template void launch_vanka_apply<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
+                                           const int *, const int *, int, float *, float, hipStream_t);
+
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
-                           const int *items, const int *item_rows, const T *u, const T *f, T *rloc, hipStream_t s) {
+                           const int *items, const int *desc, const T *u, const T *f, T *rloc, hipStream_t s) {
   if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
   const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
   if (op.a1)
-    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, item_rows, u, f, rloc);
+    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, f, rloc);
   else
-    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, item_rows, u, f, rloc);
+    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,

@@ -342,6 +342,15 @@ void upload_operator(pmg_ctx *c, DevLevel &D, const pmg_level_desc *d, bool fp32
   D.n_dirichlet = d->n_dirichlet; D.n_pdof = d->n_pdof; D.n_dof = 3 * d->n_udof + d->n_pdof;
   D.a_nnz = d->a_nnz; D.b_nnz = d->b_nnz; D.bt_nnz = d->bt_nnz;
   D.a_act_entries = d->a_rowptr[d->n_active];
+  // the residual's work-item descriptors hold row lengths in 16 bits
+  auto longest = [](const int *rp, int n) {
+    int mx = 0;
+    for (int i = 0; i < n; ++i) mx = std::max(mx, rp[i + 1] - rp[i]);
+    return mx;
+  };
+  if (longest(d->a_rowptr, d->n_udof) > 65535 || longest(d->bt_rowptr, d->n_udof) > 65535 ||
+      longest(d->b_rowptr, d->n_pdof) > 65535)
+    throw Fail{PMG_ERR_UNSUPPORTED, "operator row with more than 65535 entries"};
   D.a_rowptr = dalloc<int>(c, D.n_udof + 1); h2d(c, D.a_rowptr, d->a_rowptr, D.n_udof + 1);
   // the streamed arrays (column indices, row-blocked values) carry ROW_PAD zeroed entries
   // after their end: the row kernels' 4-entry windows of rows shorter than 4 read past it
@@ -446,13 +455,13 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     int nc = D.ptr_colors[col + 1] - p0;
     long n_items = long(nc) * (NU_LOC + NP_LOC);
     const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
-    const int *item_rows = D.res_rows + size_t(p0) * (NU_LOC + NP_LOC);
+    const int *desc = D.res_desc + 4 * size_t(p0) * (NU_LOC + NP_LOC);
     const int *slots = nullptr;
     double part = 1.0;
     if (D.compacted) {
       n_items = D.act_items_off[col + 1] - D.act_items_off[col];
       items = D.act_items + D.act_items_off[col];
-      item_rows = D.act_rows + D.act_items_off[col];
+      desc = D.act_desc + 4 * D.act_items_off[col];
       slots = D.act_slots + D.act_slots_off[col];
       part = nc ? double(D.act_slots_off[col + 1] - D.act_slots_off[col]) / nc : 0.0;
       nc = int(D.act_slots_off[col + 1] - D.act_slots_off[col]);
@@ -461,14 +470,20 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     {
       const double rb = D.res_bytes[D.a_shared][vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0);
-      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, item_rows, u, f, rloc, c->stream);
+      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, u, f, rloc, c->stream);
     }
     Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));
     if constexpr (sizeof(T) == 8) {
       launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                          D.n_udof, u, damp, c->stream);
     } else {
-      if (c->prm.precision == PMG_PREC_FP32_BF16)
+      if (D.minv16)
+        launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
+                                     u, damp, c->stream);
+      else if (D.minv32)
+        launch_vanka_apply<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof, u,
+                                  damp, c->stream);
+      else if (c->prm.precision == PMG_PREC_FP32_BF16)
         launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                             D.n_udof, u, damp, c->stream);
       else
@@ -744,7 +759,10 @@ void setup_factors(pmg_ctx *c) {
   for (size_t l = 1; l < c->L.size(); ++l) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
-    launch_patch_lu(D, D.patch_info, c->stream);
+    if (D.minv32 || D.minv16)
+      launch_patch_inv(D, D.patch_info, c->stream);
+    else
+      launch_patch_lu(D, D.patch_info, c->stream);
     infos[l].resize(D.n_cells);
     PMG_HIP_CHECK(hipMemcpyAsync(infos[l].data(), D.patch_info, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost,
                                  c->stream));
@@ -989,6 +1007,7 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
     if (!D.act_items) {
       D.act_items = dalloc<int>(c, D.h_res_items.size());
       D.act_rows = dalloc<int>(c, D.h_res_items.size());
+      D.act_desc = dalloc<int>(c, 4 * D.h_res_items.size());
       D.act_slots = dalloc<int>(c, D.n_cells);
     }
     std::vector<int> items, rows, slots;
@@ -1012,6 +1031,7 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
     }
     h2d(c, D.act_items, items.data(), items.size());
     h2d(c, D.act_rows, rows.data(), rows.size());
+    launch_res_desc(D, D.act_items, D.act_rows, long(items.size()), D.act_desc, c->stream);
     h2d(c, D.act_slots, slots.data(), slots.size());
     PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
     D.compacted = true;
@@ -1271,6 +1291,7 @@ void pmg_params_default(pmg_params *p) {
   p->bf16_refine_steps = 0;
   p->use_graph = 1;
   p->shared_block = 1;
+  p->patch_inverse = 1;
 }
 
 const char *pmg_status_string(int s) {
@@ -1476,6 +1497,8 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
       h2d(c, D.res_items, items.data(), items.size());
       D.res_rows = dalloc<int>(c, rows.size());
       h2d(c, D.res_rows, rows.data(), rows.size());
+      D.res_desc = dalloc<int>(c, 4 * rows.size());
+      launch_res_desc(D, D.res_items, D.res_rows, long(rows.size()), D.res_desc, c->stream);
       if (merged) {
         D.h_res_items = items;
         D.h_res_rows = rows;
@@ -1526,9 +1549,16 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     }
     if (lvl > 0) {
       const size_t lun = size_t(D.n_cells) * LU_STRIDE + LU_PAD;
+      const size_t minvn = size_t(D.n_cells) * MINV_STRIDE + LU_PAD;
       if (c->prm.precision == PMG_PREC_FP64) {
         D.lu = dalloc<double>(c, lun);
         launch_fill<double>(D.lu, 0.0, long(lun), c->stream);
+      } else if (c->prm.patch_inverse && c->prm.precision == PMG_PREC_FP32) {
+        D.minv32 = dalloc<float>(c, minvn);
+        launch_fill<float>(D.minv32, 0.0f, long(minvn), c->stream);
+      } else if (c->prm.patch_inverse) {
+        D.minv16 = dalloc<uint16_t>(c, minvn);
+        launch_fill<uint16_t>(D.minv16, uint16_t(0), long(minvn), c->stream);
       } else if (c->prm.precision == PMG_PREC_FP32) {
         D.lu32 = dalloc<float>(c, lun);
         launch_fill<float>(D.lu32, 0.0f, long(lun), c->stream);

@@ -23,6 +23,10 @@ constexpr int LU_PAD = 512;                   // tail entries after the last pat
 __host__ __device__ constexpr int lu_loff(int k) { return k * (NLOC - 1) - k * (k - 1) / 2; }  // L column k
 __host__ __device__ constexpr int lu_uoff(int k) { return k * (k + 1) / 2; }                   // U column k
 static_assert(LU_UOFF + LU_U <= LU_STRIDE, "packed factors exceed the patch stride");
+// explicit patch inverse (mixed precision): 85 rows of MINV_LD entries (85 used), row major
+constexpr int MINV_LD = 88;
+constexpr int MINV_STRIDE = 7488;   // 85 * 88 = 7480 padded to a multiple of 16 entries
+static_assert(NLOC * MINV_LD <= MINV_STRIDE, "inverse exceeds the patch stride");
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
@@ -70,6 +74,8 @@ struct OpView {
 // chunk per workgroup, chunk g = [n*g/G, n*(g+1)/G).  The hardware spreads consecutive
 // workgroups round-robin over the 8 XCDs, so every XCD gets an even mix of the cheap and
 // expensive rows (Dirichlet, pressure and full velocity rows differ up to ~50x in bytes).
+// (Measured: chunks of equal bytes, and one contiguous eighth of the rows per XCD for L2
+// locality of the gathers, were both slower at 32^3.)
 inline int chunk_grid(long n_items, long per_block, int cap) {
   long g = (n_items + per_block - 1) / per_block;
   if (g < 8) g = 8;
@@ -107,6 +113,7 @@ struct DevLevel {
   // global row they read (velocity rows, then pressure rows)
   int *res_items = nullptr;
   int *res_rows = nullptr;           // global row of every work item
+  int *res_desc = nullptr;           // its descriptor (row, A/B start, BT start, lengths)
   // cells
   int *u_global = nullptr;           // n_cells * 27 (UGlobalNumbers + UBeginIndex[cell])
   int *p_global = nullptr;           // n_cells * 4
@@ -118,6 +125,9 @@ struct DevLevel {
   double *lu = nullptr;
   float *lu32 = nullptr;
   uint16_t *lu16 = nullptr;          // bf16 factors
+  // or (mixed precision, pmg_params.patch_inverse) the explicit inverses, MINV_STRIDE each
+  float *minv32 = nullptr;
+  uint16_t *minv16 = nullptr;
   uint8_t *invperm = nullptr;        // n_cells * PERM_STRIDE
   int *pmap = nullptr;               // n_cells * PM_STRIDE patch gather map
   int *patch_info = nullptr;         // n_cells
@@ -149,7 +159,7 @@ struct DevLevel {
   std::vector<int> h_res_rows;
   std::vector<int> h_pos_problem;    // problem of the cell at each color position
   std::vector<long> act_items_off, act_slots_off;   // per color (n_colors + 1)
-  int *act_items = nullptr, *act_rows = nullptr, *act_slots = nullptr;
+  int *act_items = nullptr, *act_rows = nullptr, *act_slots = nullptr, *act_desc = nullptr;
   double *geom = nullptr;            // n_cells * 8 corners * 3 (device re-assembly)
 
   OpView<double> op64() const {
@@ -207,12 +217,19 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
 // patch LU of every cell of a level into the level's factor array: FP64 (lu), or, in the
 // mixed path, computed in FP32 from the FP32 operator copy into lu32 or bf16 lu16
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
+// mixed precision, explicit inverses: FP32 Gauss-Jordan into minv32 or bf16 minv16
+void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s);
+template <typename F>
+void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
+                        const int *u_global, const int *p_global, int n_udof, float *u, float damp, hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
-                           const int *items, const int *item_rows, const T *u, const T *f, T *rloc,
+                           const int *items, const int *desc, const T *u, const T *f, T *rloc,
                            hipStream_t s);
+// descriptors of residual work items (4 ints each) from their items and global rows
+void launch_res_desc(const DevLevel &L, const int *items, const int *rows, long n, int *desc, hipStream_t s);
 // row-blocked operator values from the structure-of-arrays FP64 values
 template <typename T>
 void launch_row_blocked(const DevLevel &L, T *av, T *bv, T *btv, hipStream_t s);
