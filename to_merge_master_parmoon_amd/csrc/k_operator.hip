@@ -195,6 +195,24 @@ __global__ void k_dir(T *__restrict__ u, const T *__restrict__ f, int N, int sta
   const long o = long(c) * N + start + j;
   u[o] = f ? f[o] : T(0);
 }
+// Start of an FP32 sweep with interleaved velocity gathers: the Dirichlet slices of u take
+// f (as k_dir does), and ua[i] = (u1, u2, u3, 0) of every velocity dof i
+__global__ void k_sweep_start(float *__restrict__ u, const float *__restrict__ f, int N, int start, int count,
+                              float4 *__restrict__ ua) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float v1 = u[i], v2 = u[N + i], v3 = u[2 * N + i];
+  if (i >= start && i < start + count) {
+    v1 = f[i];
+    v2 = f[N + i];
+    v3 = f[2 * N + i];
+    u[i] = v1;
+    u[N + i] = v2;
+    u[2 * N + i] = v3;
+  }
+  ua[i] = make_float4(v1, v2, v3, 0.0f);
+}
+
 // IntoL20 of the P1disc constants in two passes without atomics: every block sums its
 // chunk of constants (k_l20_part), then every block adds all partials in the same order
 // and subtracts the mean from its chunk (k_l20_sub)
@@ -442,6 +460,12 @@ void launch_dirichlet_slices(T *u, const T *f, int N, int start, int count, hipS
   k_dir<T><<<(3 * count + 255) / 256, 256, 0, s>>>(u, f, N, start, count);
   PMG_HIP_CHECK(hipGetLastError());
 }
+void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s) {
+  if (N <= 0) return;
+  k_sweep_start<<<(N + 255) / 256, 256, 0, s>>>(u, f, N, start, count, reinterpret_cast<float4 *>(ua));
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
 void launch_l20(T *p, int NP, RedScratch &, hipStream_t s) {
   if (NP <= 0) return;

@@ -426,8 +426,8 @@ template <typename T, bool SH>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, int pos0,
                                                    const uint8_t *__restrict__ invperm,
                                                    const int *__restrict__ items, const int *__restrict__ desc,
-                                                   const T *__restrict__ u, const T *__restrict__ f,
-                                                   T *__restrict__ rloc) {
+                                                   const T *__restrict__ u, const float *__restrict__ ua,
+                                                   const T *__restrict__ f, T *__restrict__ rloc) {
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active;
   const T *__restrict__ u1 = u;
@@ -461,7 +461,12 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
       const T f1 = f[UDOF], f2 = f[N + UDOF], f3 = f[2 * N + UDOF];
       T s1 = 0, s2 = 0, s3 = 0;
       const int *__restrict__ kc = op.a_kcol + k0;
-      if constexpr (SH)
+      if constexpr (SH && sizeof(T) == 4) {
+        if (ua)
+          row_a1_aos<4>(kc, op.a1 + k0, n, lane, ua, s1, s2, s3);
+        else
+          row_a1<T>(kc, op.a1 + k0, n, lane, u1, u2, u3, s1, s2, s3);
+      } else if constexpr (SH)
         row_a1<T>(kc, op.a1 + k0, n, lane, u1, u2, u3, s1, s2, s3);
       else if (UDOF < NA)
         row_a9<T>(kc, op.av + 9L * k0, n, lane, u1, u2, u3, s1, s2, s3);
@@ -648,7 +653,7 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
     // unit lower sweep (dtrsm L, column oriented); L(r,k) at lu_loff(k) + r - k - 1.  Row
     // 64+lane is below every k < 64, row lane is above every k >= 64, so per step only one
     // of the two rows needs the "below the pivot" test.
-  #pragma unroll
+#pragma unroll
     for (int k = 0; k < NLOC - 1; ++k) {
       const int o = lu_loff(k) - k - 1;
       if (k < 64) {
@@ -665,7 +670,7 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
       }
     }
     __syncthreads();
-  #pragma unroll
+#pragma unroll
     for (int q = 0; q < PS::RU; ++q) s4[lane + 64 * q] = rU[q];
     __syncthreads();
     // upper sweep (dtrsm U): x_k /= U(k,k), then eliminate column k above the diagonal;
@@ -678,7 +683,7 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
     const T dg0 = Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(lane) + lane);
     const T dg1 = two ? Ld<T, F>::get(reinterpret_cast<const F *>(s4), lu_uoff(64 + lane) + 64 + lane) : T(1);
     const T rc0 = T(1) / dg0, rc1 = T(1) / dg1;
-  #pragma unroll
+#pragma unroll
     for (int k = NLOC - 1; k >= 0; --k) {
       const int o = lu_uoff(k);
       if (k >= 64) {
@@ -714,7 +719,7 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
                                                     int ncells, int pos0, const int *__restrict__ slots,
                                                     const float *__restrict__ rloc, const int *__restrict__ u_global,
                                                     const int *__restrict__ p_global, int N, float *__restrict__ u,
-                                                    float damp) {
+                                                    float *__restrict__ ua, float damp) {
   constexpr int NV = MINV_LD * int(sizeof(F)) / 16;   // 16-byte vectors per row
   constexpr int PER = 16 / int(sizeof(F));             // entries per vector
   const int lane = threadIdx.x;
@@ -733,16 +738,21 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
     if (two) m1[q] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(mp + minv_off<F>(64 + lane, q * PER)));
   }
   const int cell = cells[pos];
-  int o0, o1 = 0;
+  // offsets in u (and in ua, -1: a pressure row) of this lane's rows
+  int o0, o1 = 0, a0i, a1i = -1;
   {
     const int c = lane / NU_LOC, j = lane - c * NU_LOC;
-    o0 = c * N + u_global[long(cell) * NU_LOC + j];
+    const int g = u_global[long(cell) * NU_LOC + j];
+    o0 = c * N + g;
+    a0i = 4 * g + c;
   }
   if (two) {
     const int i = 64 + lane;
     if (i < 3 * NU_LOC) {
       const int c = i / NU_LOC, j = i - c * NU_LOC;
-      o1 = c * N + u_global[long(cell) * NU_LOC + j];
+      const int g = u_global[long(cell) * NU_LOC + j];
+      o1 = c * N + g;
+      a1i = 4 * g + c;
     } else {
       o1 = 3 * N + p_global[long(cell) * NP_LOC + (i - 3 * NU_LOC)];
     }
@@ -767,8 +777,14 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
     acc0 = fmaf(a0, rj, acc0);
     acc1 = fmaf(a1, rj, acc1);
   }
-  u[o0] += damp * acc0;
-  if (two) u[o1] += damp * acc1;
+  const float n0 = u[o0] + damp * acc0;
+  u[o0] = n0;
+  if (ua) ua[a0i] = n0;
+  if (two) {
+    const float n1 = u[o1] + damp * acc1;
+    u[o1] = n1;
+    if (ua && a1i >= 0) ua[a1i] = n1;
+  }
 }
 
 }  // namespace
@@ -807,33 +823,38 @@ void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s) {
 
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
-                        const int *u_global, const int *p_global, int n_udof, float *u, float damp, hipStream_t s) {
+                        const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
+                        hipStream_t s) {
   if (ncells <= 0) return;
-  k_vanka_apply<F><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u, damp);
+  k_vanka_apply<F><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u, ua,
+                                         damp);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_apply<float>(const float *, const int *, int, int, const int *, const float *, const int *,
-                                        const int *, int, float *, float, hipStream_t);
+                                        const int *, int, float *, float *, float, hipStream_t);
 template void launch_vanka_apply<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
-                                           const int *, const int *, int, float *, float, hipStream_t);
+                                           const int *, const int *, int, float *, float *, float, hipStream_t);
 
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
-                           const int *items, const int *desc, const T *u, const T *f, T *rloc, hipStream_t s) {
+                           const int *items, const int *desc, const T *u, const float *ua, const T *f,
+                           T *rloc, hipStream_t s) {
   if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
   const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
   if (op.a1)
-    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, f, rloc);
+    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, ua, f, rloc);
   else
-    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, f, rloc);
+    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, ua, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,
-                                            const int *, const double *, const double *, double *, hipStream_t);
+                                            const int *, const double *, const float *, const double *, double *,
+                                            hipStream_t);
 template void launch_vanka_residual<float>(const OpView<float> &, long, int, const uint8_t *, const int *,
-                                           const int *, const float *, const float *, float *, hipStream_t);
+                                           const int *, const float *, const float *, const float *, float *,
+                                           hipStream_t);
 
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,

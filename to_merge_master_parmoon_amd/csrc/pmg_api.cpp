@@ -446,7 +446,18 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
   const bool relax = std::fabs(1 - alpha) > 1e-3;
   const int vs = sizeof(T);
   const int fb = sizeof(T) == 8 ? 8 : (c->prm.precision == PMG_PREC_FP32_BF16 ? 2 : 4);
-  set_dirichlet_t<T>(c, D, u, f);
+  // FP32 with inverses and the shared block: the residual gathers the velocity from the
+  // interleaved copy ua, built here and kept current by the local updates
+  float *ua = nullptr;
+  if constexpr (sizeof(T) == 4) {
+    if (D.ua32 && D.a_shared && (D.minv16 || D.minv32)) ua = D.ua32;
+    if (ua)
+      launch_sweep_start(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, ua, c->stream);
+    else
+      set_dirichlet_t<T>(c, D, u, f);
+  } else {
+    set_dirichlet_t<T>(c, D, u, f);
+  }
   if (relax) launch_copy<T>(aux, u, D.n_dof, c->stream);
   const OpView<T> op = opview<T>(D);
   for (int col = 0; col < D.n_colors; ++col) {
@@ -470,7 +481,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     {
       const double rb = D.res_bytes[D.a_shared][vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0);
-      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, u, f, rloc, c->stream);
+      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, u, ua, f, rloc, c->stream);
     }
     Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));
     if constexpr (sizeof(T) == 8) {
@@ -479,10 +490,10 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     } else {
       if (D.minv16)
         launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
-                                     u, damp, c->stream);
+                                     u, ua, damp, c->stream);
       else if (D.minv32)
         launch_vanka_apply<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof, u,
-                                  damp, c->stream);
+                                  ua, damp, c->stream);
       else if (c->prm.precision == PMG_PREC_FP32_BF16)
         launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                             D.n_udof, u, damp, c->stream);
@@ -1559,7 +1570,8 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
       } else if (c->prm.patch_inverse) {
         D.minv16 = dalloc<uint16_t>(c, minvn);
         launch_fill<uint16_t>(D.minv16, uint16_t(0), long(minvn), c->stream);
-      } else if (c->prm.precision == PMG_PREC_FP32) {
+      }
+      if (D.minv32 || D.minv16) D.ua32 = dalloc<float>(c, 4 * size_t(D.n_udof)); else if (c->prm.precision == PMG_PREC_FP32) {
         D.lu32 = dalloc<float>(c, lun);
         launch_fill<float>(D.lu32, 0.0f, long(lun), c->stream);
       } else {

@@ -128,6 +128,9 @@ struct DevLevel {
   // or (mixed precision, pmg_params.patch_inverse) the explicit inverses, MINV_STRIDE each
   float *minv32 = nullptr;
   uint16_t *minv16 = nullptr;
+  // FP32 sweeps with inverses and the shared block: interleaved copy (u1, u2, u3, 0) of the
+  // velocity the residual gathers, built at the sweep start and kept by the local updates
+  float *ua32 = nullptr;
   uint8_t *invperm = nullptr;        // n_cells * PERM_STRIDE
   int *pmap = nullptr;               // n_cells * PM_STRIDE patch gather map
   int *patch_info = nullptr;         // n_cells
@@ -221,13 +224,16 @@ void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s);
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
-                        const int *u_global, const int *p_global, int n_udof, float *u, float damp, hipStream_t s);
+                        const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
+                        hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
-                           const int *items, const int *desc, const T *u, const T *f, T *rloc,
-                           hipStream_t s);
+                           const int *items, const int *desc, const T *u, const float *ua, const T *f,
+                           T *rloc, hipStream_t s);
+// FP32 sweep start: Dirichlet slices of u from f, ua[4 i + c] = u_c(i) (c < 3)
+void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s);
 // descriptors of residual work items (4 ints each) from their items and global rows
 void launch_res_desc(const DevLevel &L, const int *items, const int *rows, long n, int *desc, hipStream_t s);
 // row-blocked operator values from the structure-of-arrays FP64 values

@@ -268,6 +268,39 @@ __device__ inline void row_a1_s(const int *__restrict__ kc, const T *__restrict_
   }
 }
 
+// row_a1_s with the velocity gathered from the interleaved copy xa (x_c(col) at 4 col + c):
+// one 16-byte gather per entry instead of three
+template <int U>
+__device__ inline void row_a1_aos(const int *__restrict__ kc, const float *__restrict__ v, int n, int lane,
+                                  const float *__restrict__ xa, float &s1, float &s2, float &s3) {
+  typedef float vf4 __attribute__((ext_vector_type(4)));
+  for (int j0 = 0; j0 < n; j0 += 16 * U) {
+    int jj[U];
+    bool ok[U];
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + 16 * u + lane;
+      ok[u] = j < n;
+      jj[u] = ok[u] ? j : n - 1;
+      c[u] = kc[jj[u]];
+    }
+    float a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = v[jj[u]];
+    vf4 y[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) __builtin_memcpy(&y[u], xa + 4L * c[u], 16);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float w = ok[u] ? a[u] : 0.0f;
+      s1 += w * y[u].x;
+      s2 += w * y[u].y;
+      s3 += w * y[u].z;
+    }
+  }
+}
+
 // BT row (velocity row, pressure columns): s_r += BT_r(row, q) p(q)
 template <typename T, int U>
 __device__ inline void row_bt_s(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
