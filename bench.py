@@ -36,6 +36,7 @@ FP64_PEAK_TFLOPS = 78.6
 
 USE_GRAPH = 1
 SHARED_BLOCK = 1
+PATCH_INVERSE = 1
 
 
 def solver_params(precision: int):
@@ -45,7 +46,7 @@ def solver_params(precision: int):
     return _abi.default_params(sc_gmres_restart=30, sc_lin_red_factor_saddle=1e-10,
                                sc_lin_res_norm_min_saddle=0.0, sc_lin_maxit_saddle=1000,
                                sc_pre_smooth_saddle=2, sc_post_smooth_saddle=2, precision=precision,
-                               use_graph=USE_GRAPH, shared_block=SHARED_BLOCK)
+                               use_graph=USE_GRAPH, shared_block=SHARED_BLOCK, patch_inverse=PATCH_INVERSE)
 
 
 def run_gpu(prob, precision: int, steps: int, warmup: int, log):
@@ -318,12 +319,15 @@ def main():
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-maxit", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true", help="launch every V-cycle kernel by kernel")
+    ap.add_argument("--lu-sweeps", action="store_true",
+                    help="mixed precision: stored LU factors and triangular sweeps instead of patch inverses")
     ap.add_argument("--nine-blocks", action="store_true",
                     help="stream all nine A blocks even where A11 = A22 = A33 and the coupling blocks vanish")
     args = ap.parse_args()
-    global USE_GRAPH, SHARED_BLOCK
+    global USE_GRAPH, SHARED_BLOCK, PATCH_INVERSE
     USE_GRAPH = 0 if args.no_graph else 1
     SHARED_BLOCK = 0 if args.nine_blocks else 1
+    PATCH_INVERSE = 0 if args.lu_sweeps else 1
     log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition "
