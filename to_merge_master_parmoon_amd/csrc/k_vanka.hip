@@ -341,8 +341,11 @@ __host__ __device__ constexpr int minv_off(int r, int c) {
 // Patch matrix and its explicit inverse (mixed precision): the gathers of k_patch_lu, FP32
 // Gauss-Jordan with the pivots of dgetf2, M = (P A)^{-1} stored in the layout of minv_off
 // (the 3 padding columns zero) through an LDS copy of 16 rows at a time.
+// (6 resident wavefronts per SIMD instead of the 4 that 99 VGPRs allow, at the price of a
+// small spill: measured faster, 5.0 -> 4.0 ms for the three levels, and it leaves the
+// coarse LU beside it less time to wait)
 template <typename F>
-__global__ __launch_bounds__(256) void k_patch_inv(OpView<float> op, const int *__restrict__ cells,
+__global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const int *__restrict__ cells,
                                                    const int *__restrict__ pmap, int ncells, F *__restrict__ minv,
                                                    uint8_t *__restrict__ invperm, int *__restrict__ info) {
   __shared__ int sigma[NLOC];
