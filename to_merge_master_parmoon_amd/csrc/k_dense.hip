@@ -463,11 +463,19 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const double *__restrict__ f
         }
       }
     } else {
+      // U(k,k) sits in lane k's row_v[k].  Every lane forms the reciprocal of its own
+      // diagonal entry before the chain; a step then takes the corrected quotient
+      // q = x rc, e = fma(-q, u, x), q' = fma(e, rc, q), which with rc = RN(1/u) is the
+      // correctly rounded x / u (Markstein): the value of the division, in three dependent
+      // operations instead of the division's sequence.
+      const double dg = r < kb ? f[long(R0 + r) * n + R0 + r] : 1.0;
+      const double rc = 1.0 / dg;
 #pragma unroll
       for (int k = TB - 1; k >= 0; --k) {
         if (k < kb) {
-          // U(k,k) sits in lane k's row_v[k]
-          const double xk = bcast(v, k) / bcast(row_v[k], k);
+          const double xv = bcast(v, k), uk = bcast(dg, k), rk = bcast(rc, k);
+          const double q = xv * rk;
+          const double xk = fma(fma(-q, uk, xv), rk, q);
           if (r == k) v = xk;
           if (r < k) v = fma(-xk, row_v[k], v);
         }
