@@ -649,6 +649,9 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
   const T *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
   T x0 = rs[lane];
   T x1 = two ? rs[64 + lane] : T(0);
+  // the values to update, loaded before the sweeps (no round trip after the chain)
+  const T v0 = u[o0];
+  const T v1 = two ? u[o1] : T(0);
   __syncthreads();
   if constexpr (REG) {
     sweeps_in_registers<T, F>(s4, sl0, sl1, rU, lane, two, x0, x1);
@@ -708,8 +711,8 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
     }
   }
   // u[UDOF] += damp * delta, p[PDOF] += damp * delta
-  u[o0] += damp * x0;
-  if (two) u[o1] += damp * x1;
+  u[o0] = v0 + damp * x0;
+  if (two) u[o1] = v1 + damp * x1;
 }
 
 // x = M (P r) for one patch per wavefront, M the explicit inverse of k_patch_inv: lane i
@@ -763,6 +766,9 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
   const float *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
   const float r0 = rs[lane];
   const float r1 = two ? rs[64 + lane] : 0.0f;
+  // the values to update, loaded before the products (no round trip after them)
+  const float v0 = u[o0];
+  const float v1 = two ? u[o1] : 0.0f;
   float acc0 = 0.0f, acc1 = 0.0f;
 #pragma unroll
   for (int j = 0; j < MINV_LD; ++j) {
@@ -780,11 +786,11 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
     acc0 = fmaf(a0, rj, acc0);
     acc1 = fmaf(a1, rj, acc1);
   }
-  const float n0 = u[o0] + damp * acc0;
+  const float n0 = v0 + damp * acc0;
   u[o0] = n0;
   if (ua) ua[a0i] = n0;
   if (two) {
-    const float n1 = u[o1] + damp * acc1;
+    const float n1 = v1 + damp * acc1;
     u[o1] = n1;
     if (ua && a1i >= 0) ua[a1i] = n1;
   }
