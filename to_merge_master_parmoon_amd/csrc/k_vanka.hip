@@ -489,7 +489,14 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
       const int PDOF = grow;
       const T fp = f[3 * N + PDOF];
       T s = 0;
-      row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
+      if constexpr (sizeof(T) == 4) {
+        if (ua)
+          row_b_aos<2>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, ua, s);
+        else
+          row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
+      } else {
+        row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
+      }
       s = red16(s);
       if (lane == 0) out[ip[3 * NU_LOC + mm]] = fp - s;
     }

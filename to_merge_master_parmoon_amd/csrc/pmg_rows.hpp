@@ -301,6 +301,38 @@ __device__ inline void row_a1_aos(const int *__restrict__ kc, const float *__res
   }
 }
 
+// row_b_s with the velocity gathered from the interleaved copy xa (x_c(col) at 4 col + c)
+template <int U>
+__device__ inline void row_b_aos(const int *__restrict__ kc, const float *__restrict__ v, int n, int lane,
+                                 const float *__restrict__ xa, float &s) {
+  typedef float vf4 __attribute__((ext_vector_type(4)));
+  for (int j0 = 0; j0 < n; j0 += 16 * U) {
+    int jj[U];
+    bool ok[U];
+    int c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + 16 * u + lane;
+      ok[u] = j < n;
+      jj[u] = ok[u] ? j : n - 1;
+      c[u] = kc[jj[u]];
+    }
+    float b[3][U];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int u = 0; u < U; ++u) b[r][u] = v[r * n + jj[u]];
+    vf4 y[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) __builtin_memcpy(&y[u], xa + 4L * c[u], 16);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float z1 = ok[u] ? y[u].x : 0.0f, z2 = ok[u] ? y[u].y : 0.0f, z3 = ok[u] ? y[u].z : 0.0f;
+      s += b[0][u] * z1 + b[1][u] * z2 + b[2][u] * z3;
+    }
+  }
+}
+
 // BT row (velocity row, pressure columns): s_r += BT_r(row, q) p(q)
 template <typename T, int U>
 __device__ inline void row_bt_s(const int *__restrict__ kc, const T *__restrict__ w, int m, int lane,
