@@ -106,7 +106,9 @@ def test_nine_blocks_vcycle_and_fgmres(coupled16):
 def test_forms_agree_on_oseen(oseen16, precision):
     mg1 = _hier(oseen16, precision=precision)
     mg0 = _hier(oseen16, precision=precision, shared_block=0)
-    tol = 1e-12 if precision == 0 else 1e-5
+    # FP64: same arithmetic up to the summation order; BF16: the two forms store different
+    # local data (block form vs full patch inverse) in BF16, so they agree to its accuracy
+    tol = 1e-12 if precision == 0 else 2e-3
     for k in range(2):
         v = _rand(mg1.n_dof[-1], 40 + k)
         assert rel(mg1.cycle(v), mg0.cycle(v)) <= tol

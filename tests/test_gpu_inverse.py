@@ -1,8 +1,11 @@
 """Explicit patch inverses of the mixed-precision smoother, through the C ABI.
 
-With patch_inverse = 1 (the mixed-precision default) every 85 x 85 Vanka patch matrix is
-inverted once per operator by FP32 Gauss-Jordan with the row pivots of dgetf2, and each
-local solve is one product with the stored inverse; patch_inverse = 0 keeps the LU factors
+With patch_inverse = 1 (the mixed-precision default) every Vanka patch is inverted once per
+operator and each local solve is a product with the stored data: with the shared velocity
+block (the Oseen problems here) in the block form (A^{-1} of the 27 x 27 velocity block,
+W_c = A^{-1} BT_c, B_c and the inverse of the 4 x 4 Schur complement), otherwise the full
+85 x 85 inverse by FP32 Gauss-Jordan with the row pivots of dgetf2 (covered on the
+nine-block operator by tests/test_gpu_blocks.py); patch_inverse = 0 keeps the LU factors
 and triangular sweeps.  Pins:
   * one FP32 sweep with inverses against the FP64 oracle's sweep (dgetrf/dgetrs per patch)
     to FP32 accuracy, and with BF16-stored inverses to BF16 accuracy, on two levels; a

@@ -392,6 +392,156 @@ __global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const in
   if (t == 0) info[pos] = info_s;
 }
 
+// Block form of a patch with the shared velocity block (mixed precision).  The patch matrix
+// is [A 0 0 BT1; 0 A 0 BT2; 0 0 A BT3; B1 B2 B3 0] with one 27 x 27 block A (patch_entry
+// gives exactly these entries: zero coupling blocks, Dirichlet rows with the diagonal only
+// and no BT).  A is inverted by FP32 Gauss-Jordan with partial pivoting (the column swaps
+// undone at the end, so A^{-1} itself), then W_c = A^{-1} BT_c, the Schur complement
+// S = -sum_c B_c W_c and S^{-1} (4 x 4, pivoted); A^{-1}, W, B and S^{-1} are stored in the
+// factor precision (SCH_* layout).  The local residual is then used in its natural order
+// (invperm = identity).
+template <typename F>
+__global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int *__restrict__ cells,
+                                                     const int *__restrict__ pmap, int ncells, F *__restrict__ sch,
+                                                     uint8_t *__restrict__ invperm, int *__restrict__ info) {
+  __shared__ float A[NU_LOC][NU_LOC + 1];
+  __shared__ float BT[3][NU_LOC][NP_LOC];
+  __shared__ float Bm[3][NP_LOC][NU_LOC];
+  __shared__ float W[3][NU_LOC][NP_LOC];
+  __shared__ float S[NP_LOC][NP_LOC];
+  __shared__ float colk[NU_LOC], rowk[NU_LOC];
+  __shared__ int piv[NU_LOC];
+  __shared__ int info_s;
+  const int pos = blockIdx.x;
+  if (pos >= ncells) return;
+  const int t = threadIdx.x;
+  const int *__restrict__ mp = pmap + long(cells[pos]) * PM_STRIDE;
+  if (t == 0) info_s = 0;
+  for (int e = t; e < NU_LOC * NU_LOC; e += 128) {
+    const int j = e / NU_LOC, m = e - NU_LOC * j;
+    A[j][m] = patch_entry<float>(op, mp, j, m);
+  }
+  for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
+    const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC, j = r / NP_LOC, k = r - NP_LOC * j;
+    BT[c][j][k] = patch_entry<float>(op, mp, NU_LOC * c + j, 3 * NU_LOC + k);
+    Bm[c][k][j] = patch_entry<float>(op, mp, 3 * NU_LOC + k, NU_LOC * c + j);
+  }
+  __syncthreads();
+  for (int k = 0; k < NU_LOC; ++k) {
+    if (t < 64) {
+      float bv = -1.0f;
+      int bi = NU_LOC;
+      if (t >= k && t < NU_LOC) {
+        bv = fabsf(A[t][k]);
+        bi = t;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      if (t == 0) {
+        piv[k] = bi;
+        if (bv == 0.0f && info_s == 0) info_s = k + 1;
+      }
+    }
+    __syncthreads();
+    const int p = piv[k];
+    if (p != k && t < NU_LOC) {
+      const float x = A[k][t];
+      A[k][t] = A[p][t];
+      A[p][t] = x;
+    }
+    __syncthreads();
+    if (t < NU_LOC) {
+      colk[t] = A[t][k];
+      rowk[t] = A[k][t];
+    }
+    __syncthreads();
+    const float pv = rowk[k];
+    const float rc = pv != 0.0f ? 1.0f / pv : 0.0f;
+    for (int e = t; e < NU_LOC * NU_LOC; e += 128) {
+      const int i = e / NU_LOC, c = e - NU_LOC * i;
+      const float uk = c == k ? rc : rowk[c] * rc;
+      A[i][c] = i == k ? uk : fmaf(-colk[i], uk, c == k ? 0.0f : A[i][c]);
+    }
+    __syncthreads();
+  }
+  // (P A)^{-1} -> A^{-1}: undo the row interchanges as column swaps, last first
+  for (int k = NU_LOC - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k && t < NU_LOC) {
+      const float x = A[t][k];
+      A[t][k] = A[t][p];
+      A[t][p] = x;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
+    const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC, j = r / NP_LOC, k = r - NP_LOC * j;
+    float acc = 0.0f;
+    for (int m = 0; m < NU_LOC; ++m) acc = fmaf(A[j][m], BT[c][m][k], acc);
+    W[c][j][k] = acc;
+  }
+  __syncthreads();
+  if (t < NP_LOC * NP_LOC) {
+    const int k = t / NP_LOC, l = t - NP_LOC * k;
+    float acc = 0.0f;
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < NU_LOC; ++j) acc = fmaf(Bm[c][k][j], W[c][j][l], acc);
+    S[k][l] = -acc;
+  }
+  __syncthreads();
+  if (t == 0) {
+    // 4 x 4 Gauss-Jordan with partial pivoting, column swaps undone
+    int sp[NP_LOC];
+    for (int k = 0; k < NP_LOC; ++k) {
+      int p = k;
+      for (int i = k + 1; i < NP_LOC; ++i)
+        if (fabsf(S[i][k]) > fabsf(S[p][k])) p = i;
+      sp[k] = p;
+      if (S[p][k] == 0.0f && info_s == 0) info_s = NU_LOC + k + 1;
+      if (p != k)
+        for (int c = 0; c < NP_LOC; ++c) {
+          const float x = S[k][c];
+          S[k][c] = S[p][c];
+          S[p][c] = x;
+        }
+      const float rc = S[k][k] != 0.0f ? 1.0f / S[k][k] : 0.0f;
+      float rowv[NP_LOC], colv[NP_LOC];
+      for (int c = 0; c < NP_LOC; ++c) {
+        rowv[c] = c == k ? rc : S[k][c] * rc;
+        colv[c] = S[c][k];
+      }
+      for (int i = 0; i < NP_LOC; ++i)
+        for (int c = 0; c < NP_LOC; ++c) S[i][c] = i == k ? rowv[c] : fmaf(-colv[i], rowv[c], c == k ? 0.0f : S[i][c]);
+    }
+    for (int k = NP_LOC - 1; k >= 0; --k)
+      if (sp[k] != k)
+        for (int i = 0; i < NP_LOC; ++i) {
+          const float x = S[i][k];
+          S[i][k] = S[i][sp[k]];
+          S[i][sp[k]] = x;
+        }
+  }
+  __syncthreads();
+  F *out = sch + long(pos) * SCH_STRIDE;
+  for (int e = t; e < NU_LOC * NU_LOC; e += 128) out[SCH_AINV + e] = StoreF<F>::get(A[e / NU_LOC][e % NU_LOC]);
+  for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
+    const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC;
+    out[SCH_W + e] = StoreF<F>::get(W[c][r / NP_LOC][r % NP_LOC]);
+    out[SCH_B + e] = StoreF<F>::get(Bm[c][r / NU_LOC][r % NU_LOC]);
+  }
+  if (t < NP_LOC * NP_LOC) out[SCH_SINV + t] = StoreF<F>::get(S[t / NP_LOC][t % NP_LOC]);
+  for (int e = SCH_SINV + NP_LOC * NP_LOC + t; e < SCH_STRIDE; e += 128) out[e] = StoreF<F>::get(0.0f);
+  if (t < NLOC) invperm[long(pos) * PERM_STRIDE + t] = uint8_t(t);
+  if (t == 0) info[pos] = info_s;
+}
+
 // ------------------------------------------------------------------ per-color residual
 // (slot, local row) items, 16 lanes each: rows 0..26 produce the three velocity
 // components of local dof j, rows 27..30 the pressure rows (NSE_MGLevel4.C:902-1061).
@@ -803,6 +953,110 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
   }
 }
 
+// Local solve in the block form, one wavefront per patch: the patch's 1408 entries are
+// staged in LDS (non-temporal 16-byte loads), then y_c = A^{-1} r_c (lanes 0..26, three
+// components), g = r_p - sum_c B_c y_c, p = S^{-1} g (lanes 0..3), x_c = y_c - W_c p, and
+// u += damp * (x, p) (with the interleaved velocity copy when ua is given).
+template <typename F>
+__global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ sch, const int *__restrict__ cells,
+                                                          int ncells, int pos0, const int *__restrict__ slots,
+                                                          const float *__restrict__ rloc,
+                                                          const int *__restrict__ u_global,
+                                                          const int *__restrict__ p_global, int N,
+                                                          float *__restrict__ u, float *__restrict__ ua, float damp) {
+  constexpr int PER = 16 / int(sizeof(F));
+  constexpr int NVEC = SCH_STRIDE / PER;
+  __shared__ float M[SCH_STRIDE];
+  __shared__ float r[RLOC_STRIDE], y[3][NU_LOC], g[NP_LOC], pp[NP_LOC];
+  const int lane = threadIdx.x;
+  if (int(blockIdx.x) >= ncells) return;
+  const int slot = slots ? slots[blockIdx.x] : int(blockIdx.x);
+  const int pos = pos0 + slot;
+  const F *__restrict__ src = sch + long(pos) * SCH_STRIDE;
+  for (int q = lane; q < NVEC; q += 64) {
+    const v4u w = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(src + q * PER));
+    if constexpr (sizeof(F) == 2) {
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        M[q * PER + 2 * h] = __uint_as_float(w[h] << 16);
+        M[q * PER + 2 * h + 1] = __uint_as_float(w[h] & 0xffff0000u);
+      }
+    } else {
+#pragma unroll
+      for (int h = 0; h < 4; ++h) M[q * PER + h] = __uint_as_float(w[h]);
+    }
+  }
+  const float *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
+  r[lane] = rs[lane];
+  if (lane < NLOC - 64) r[64 + lane] = rs[64 + lane];
+  const int cell = cells[pos];
+  int g0 = 0, op = 0;
+  float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, vp = 0.0f;
+  if (lane < NU_LOC) {
+    g0 = u_global[long(cell) * NU_LOC + lane];
+    v0 = u[g0];
+    v1 = u[N + g0];
+    v2 = u[2 * N + g0];
+  } else if (lane < NU_LOC + NP_LOC) {
+    op = 3 * N + p_global[long(cell) * NP_LOC + (lane - NU_LOC)];
+    vp = u[op];
+  }
+  __syncthreads();
+  if (lane < NU_LOC) {
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int m = 0; m < NU_LOC; ++m) {
+      const float w = M[SCH_AINV + lane * NU_LOC + m];
+      a0 = fmaf(w, r[m], a0);
+      a1 = fmaf(w, r[NU_LOC + m], a1);
+      a2 = fmaf(w, r[2 * NU_LOC + m], a2);
+    }
+    y[0][lane] = a0;
+    y[1][lane] = a1;
+    y[2][lane] = a2;
+  }
+  __syncthreads();
+  {
+    // g = r_p - sum_c B_c y_c: the 16 lanes of group k share pressure row k's 81 terms
+    const int k = lane >> 4, part = lane & 15;
+    float acc = 0.0f;
+    for (int q = part; q < 3 * NU_LOC; q += 16) {
+      const int c = q / NU_LOC, j = q - NU_LOC * c;
+      acc = fmaf(M[SCH_B + (c * NP_LOC + k) * NU_LOC + j], y[c][j], acc);
+    }
+    acc += __shfl_xor(acc, 8, 16);
+    acc += __shfl_xor(acc, 4, 16);
+    acc += __shfl_xor(acc, 2, 16);
+    acc += __shfl_xor(acc, 1, 16);
+    if (part == 0) g[k] = r[3 * NU_LOC + k] - acc;
+  }
+  __syncthreads();
+  if (lane < NP_LOC) {
+    float acc = 0.0f;
+    for (int l = 0; l < NP_LOC; ++l) acc = fmaf(M[SCH_SINV + lane * NP_LOC + l], g[l], acc);
+    pp[lane] = acc;
+  }
+  __syncthreads();
+  if (lane < NU_LOC) {
+    float x[3];
+    for (int c = 0; c < 3; ++c) {
+      float acc = y[c][lane];
+      for (int k = 0; k < NP_LOC; ++k) acc = fmaf(-M[SCH_W + (c * NU_LOC + lane) * NP_LOC + k], pp[k], acc);
+      x[c] = acc;
+    }
+    const float n0 = v0 + damp * x[0], n1 = v1 + damp * x[1], n2 = v2 + damp * x[2];
+    u[g0] = n0;
+    u[N + g0] = n1;
+    u[2 * N + g0] = n2;
+    if (ua) {
+      ua[4 * g0] = n0;
+      ua[4 * g0 + 1] = n1;
+      ua[4 * g0 + 2] = n2;
+    }
+  } else if (lane < NU_LOC + NP_LOC) {
+    u[op] = vp + damp * pp[lane - NU_LOC];
+  }
+}
+
 }  // namespace
 
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s) {
@@ -836,6 +1090,31 @@ void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s) {
                                                     L.invperm, info);
   PMG_HIP_CHECK(hipGetLastError());
 }
+
+void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s) {
+  if (L.n_cells <= 0) return;
+  if (L.minv32)
+    k_patch_schur<float><<<L.n_cells, 128, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv32, L.invperm,
+                                                   info);
+  else
+    k_patch_schur<uint16_t><<<L.n_cells, 128, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv16,
+                                                      L.invperm, info);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename F>
+void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
+                              const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
+                              float *ua, float damp, hipStream_t s) {
+  if (ncells <= 0) return;
+  k_vanka_apply_schur<F><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u,
+                                               ua, damp);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_vanka_apply_schur<float>(const float *, const int *, int, int, const int *, const float *,
+                                              const int *, const int *, int, float *, float *, float, hipStream_t);
+template void launch_vanka_apply_schur<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
+                                                 const int *, const int *, int, float *, float *, float, hipStream_t);
 
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,

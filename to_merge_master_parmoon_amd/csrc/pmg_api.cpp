@@ -310,10 +310,15 @@ void check_coarse(pmg_ctx *c, Coarse &C) {
 // the streamed forms of new operator values (CSR arrays D.a, D.b, D.bt on the device):
 // row-blocked FP64 (and FP32) copies and, when the A blocks have it, the shared velocity
 // block form (FP64: block 0 of D.a read in place; FP32: its own copy)
+void drop_graph(pmg_ctx *c);
+
 void refresh_values(pmg_ctx *c, DevLevel &D) {
   launch_row_blocked<double>(D, D.av64, D.bv64, D.btv64, c->stream);
   if (D.av32) launch_row_blocked<float>(D, D.av32, D.bv32, D.btv32, c->stream);
+  const bool was = D.a_shared;
   D.a_shared = c->prm.shared_block && shared_velocity_block(D, c->red.partials, c->stream);
+  // the captured V-cycle holds the kernels and operator views of the previous form
+  if (D.a_shared != was) drop_graph(c);
   if (D.a_shared && D.av32) {
     if (!D.a1_32) {
       D.a1_32 = dalloc<float>(c, size_t(D.a_nnz) + ROW_PAD);
@@ -488,7 +493,13 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
       launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                          D.n_udof, u, damp, c->stream);
     } else {
-      if (D.minv16)
+      if (D.schur && D.minv16)
+        launch_vanka_apply_schur<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
+                                           D.n_udof, u, ua, damp, c->stream);
+      else if (D.schur && D.minv32)
+        launch_vanka_apply_schur<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
+                                        D.n_udof, u, ua, damp, c->stream);
+      else if (D.minv16)
         launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
                                      u, ua, damp, c->stream);
       else if (D.minv32)
@@ -770,10 +781,21 @@ void setup_factors(pmg_ctx *c) {
   for (size_t l = 1; l < c->L.size(); ++l) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
-    if (D.minv32 || D.minv16)
-      launch_patch_inv(D, D.patch_info, c->stream);
-    else
+    if (D.minv32 || D.minv16) {
+      // the block form needs the shared velocity block; the captured V-cycle launches the
+      // local solves of the form it was captured with
+      // (levels with up to 2048 patches per color: the full inverse, whose single-phase
+      // product is faster at the latency floor of small launches)
+      const bool schur = D.a_shared && D.max_cells_per_color > 2048;
+      if (D.schur != schur) drop_graph(c);
+      D.schur = schur;
+      if (D.schur)
+        launch_patch_schur(D, D.patch_info, c->stream);
+      else
+        launch_patch_inv(D, D.patch_info, c->stream);
+    } else {
       launch_patch_lu(D, D.patch_info, c->stream);
+    }
     infos[l].resize(D.n_cells);
     PMG_HIP_CHECK(hipMemcpyAsync(infos[l].data(), D.patch_info, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost,
                                  c->stream));

@@ -27,6 +27,13 @@ static_assert(LU_UOFF + LU_U <= LU_STRIDE, "packed factors exceed the patch stri
 constexpr int MINV_LD = 88;
 constexpr int MINV_STRIDE = 7488;   // 85 * 88 = 7480 padded to a multiple of 16 entries
 static_assert(NLOC * MINV_LD <= MINV_STRIDE, "inverse exceeds the patch stride");
+// block form of a patch with the shared velocity block (mixed precision): the patch matrix
+// is [A 0 0 BT1; 0 A 0 BT2; 0 0 A BT3; B1 B2 B3 0] with one 27 x 27 block A, so its solve
+// needs A^{-1} (729), W_c = A^{-1} BT_c (3 x 27 x 4), B_c (3 x 4 x 27) and the inverse of the
+// 4 x 4 Schur complement S = -sum_c B_c W_c (16): 1393 entries per patch
+constexpr int SCH_AINV = 0, SCH_W = 729, SCH_B = SCH_W + 324, SCH_SINV = SCH_B + 324;
+constexpr int SCH_STRIDE = 1408;    // 1393 padded to a multiple of 16 entries
+static_assert(SCH_SINV + 16 <= SCH_STRIDE && SCH_STRIDE <= MINV_STRIDE, "block-form patch stride");
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
@@ -131,6 +138,9 @@ struct DevLevel {
   // FP32 sweeps with inverses and the shared block: interleaved copy (u1, u2, u3, 0) of the
   // velocity the residual gathers, built at the sweep start and kept by the local updates
   float *ua32 = nullptr;
+  // the patch inverses are stored in the block form (set at every factorization: shared
+  // velocity block present)
+  bool schur = false;
   uint8_t *invperm = nullptr;        // n_cells * PERM_STRIDE
   int *pmap = nullptr;               // n_cells * PM_STRIDE patch gather map
   int *patch_info = nullptr;         // n_cells
@@ -226,6 +236,13 @@ template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
                         const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
                         hipStream_t s);
+// block form (shared velocity block): set-up into minv32 / minv16 with SCH_STRIDE per patch,
+// and its local solves
+void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s);
+template <typename F>
+void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
+                              const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
+                              float *ua, float damp, hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
