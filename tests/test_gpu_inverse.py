@@ -68,3 +68,26 @@ def test_inverse_and_lu_forms_agree(oseen16, precision):
     assert rel(unpin_pressure(xi, nu), unpin_pressure(xl, nu)) <= 1e-7
     mi.close()
     ml.close()
+
+
+def test_block_form_on_large_colors():
+    """Colors of more than 2048 patches (the 32^3 level) take the block form: one FP32 sweep
+    there equals the sweep with LU factors (patch_inverse = 0, pinned against the oracle
+    above) and the one with full 85 x 85 inverses on the nine-block kernels
+    (shared_block = 0) to FP32 accuracy."""
+    from to_merge_master_parmoon_amd import problem
+    P = problem.oseen3d(32)
+    lvl = 3
+    res = {}
+    for name, kw in (("block", dict(patch_inverse=1)), ("lu", dict(patch_inverse=0)),
+                     ("full", dict(patch_inverse=1, shared_block=0))):
+        mg = _hier(P, precision=1, **kw)
+        assert mg.level_info(lvl)["max_cells_per_color"] > 2048
+        n = mg.n_dof[lvl]
+        rng = np.random.default_rng(80)
+        u0, f = rng.standard_normal(n), rng.standard_normal(n)
+        res[name] = mg.cell_vanka(lvl, u0.copy(), f, 1) - u0
+        mg.close()
+    e_lu, e_full = rel(res["block"], res["lu"]), rel(res["block"], res["full"])
+    print("32^3 sweep: block vs LU", e_lu, "block vs full inverse", e_full, flush=True)
+    assert e_lu <= 2e-5 and e_full <= 2e-5
