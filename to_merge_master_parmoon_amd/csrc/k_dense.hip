@@ -84,21 +84,13 @@ __global__ void k_fill_d(double *x, double v, long n) {
 template <int PR>
 __global__ __launch_bounds__(PNT) void k_lu_panel(double *__restrict__ a, int n, int j0, int j1,
                                                   int *__restrict__ ipiv, int *__restrict__ info,
-                                                  const double *__restrict__ ubuf, int pj0, int pj1,
-                                                  int *__restrict__ sigma) {
+                                                  int *__restrict__ swp, int *__restrict__ sigma) {
   __shared__ double sv[2][PNT / 64];
   __shared__ int si[2][PNT / 64];
   __shared__ double bufP[2][NBR], bufJ[2][NBR];
   __shared__ int pivs[NBR];
-  __shared__ int rows_s[2 * NBR], src_s[2 * NBR];
-  __shared__ int nk_s;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int w = j1 - j0;
-  if (pj1 > pj0) {
-    const int pw = pj1 - pj0;
-    for (int kk = 0; kk < pw; ++kk)
-      for (int c = pj1 + t; c < n; c += PNT) a[long(pj0 + kk) * n + c] = ubuf[long(kk) * n + c];
-  }
   double v[PR][NBR];
 #pragma unroll
   for (int q = 0; q < PR; ++q) {
@@ -223,27 +215,48 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(double *__restrict__ a, int n,
       if (lane == k) src = sf;
       if (lane == f) src = sk;
     }
+    // for k_lu_swap: the rows involved and their sources, then their count
     if (lane < 2 * NBR) {
-      rows_s[lane] = row;
-      src_s[lane] = src;
+      swp[lane] = row;
+      swp[2 * NBR + lane] = src;
     }
-    if (lane == 0) nk_s = nk;
+    if (lane == 0) swp[4 * NBR] = nk;
     // the permutation vector (sigma[i] = original row now at row i) moves like a column
     const int sg = lane < nk ? sigma[src] : 0;
     if (lane < nk && src != row) sigma[row] = sg;
   }
-  __syncthreads();
-  const int nk = nk_s;
-  for (int c = t; c < n - w; c += PNT) {
-    const int col = c < j0 ? c : c + w;
-    double *cp = a + col;
-    double x[2 * NBR];
-#pragma unroll
-    for (int k = 0; k < 2 * NBR; ++k) x[k] = k < nk ? cp[long(src_s[k]) * n] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 2 * NBR; ++k)
-      if (k < nk && src_s[k] != rows_s[k]) cp[long(rows_s[k]) * n] = x[k];
+}
+
+// After a panel: its row interchanges applied to every column outside it (one thread per
+// column, the <= 2*NBR rows involved permuted at once, as dlaswp's sequential swaps would
+// leave them), and the U block row of the previous step (rows pj0..pj1-1, kept in ubuf by
+// k_lu_update) copied into place for the columns >= pj1.  The two touch disjoint rows (the
+// interchanges involve rows >= j0 = pj1 only).
+__global__ __launch_bounds__(256) void k_lu_swap(double *__restrict__ a, int n, int j0, int j1,
+                                                 const int *__restrict__ swp, const double *__restrict__ ubuf,
+                                                 int pj0, int pj1) {
+  __shared__ int rows_s[2 * NBR], src_s[2 * NBR];
+  __shared__ int nk_s;
+  const int t = threadIdx.x;
+  if (t < 2 * NBR) {
+    rows_s[t] = swp[t];
+    src_s[t] = swp[2 * NBR + t];
   }
+  if (t == 0) nk_s = swp[4 * NBR];
+  __syncthreads();
+  const int c = blockIdx.x * 256 + t;
+  if (c >= n) return;
+  if (pj1 > pj0 && c >= pj1)
+    for (int kk = 0; kk < pj1 - pj0; ++kk) a[long(pj0 + kk) * n + c] = ubuf[long(kk) * n + c];
+  if (c >= j0 && c < j1) return;
+  const int nk = nk_s;
+  double *cp = a + c;
+  double x[2 * NBR];
+#pragma unroll
+  for (int k = 0; k < 2 * NBR; ++k) x[k] = k < nk ? cp[long(src_s[k]) * n] : 0.0;
+#pragma unroll
+  for (int k = 0; k < 2 * NBR; ++k)
+    if (k < nk && src_s[k] != rows_s[k]) cp[long(rows_s[k]) * n] = x[k];
 }
 
 // Trailing update of one 64x64 tile of A22 = rows/columns >= j1: the block first solves its
@@ -546,14 +559,16 @@ void coarse_lu_factor(Coarse &C, hipStream_t s) {
     const int j1 = std::min(n, j0 + NBR);
     const int pr = (n - j0 + PNT - 1) / PNT;
     switch (pr) {
-      case 1: k_lu_panel<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      case 2: k_lu_panel<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      case 3: k_lu_panel<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      case 4: k_lu_panel<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      case 5: k_lu_panel<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      case 6: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
-      default: k_lu_panel<7><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.ubuf, pj0, pj1, C.sigma); break;
+      case 1: k_lu_panel<1><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
+      case 2: k_lu_panel<2><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
+      case 3: k_lu_panel<3><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
+      case 4: k_lu_panel<4><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
+      case 5: k_lu_panel<5><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
+      case 6: k_lu_panel<6><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
+      default: k_lu_panel<7><<<1, PNT, 0, s>>>(C.a, n, j0, j1, C.ipiv, C.info, C.swp, C.sigma); break;
     }
+    PMG_HIP_CHECK(hipGetLastError());
+    k_lu_swap<<<(n + 255) / 256, 256, 0, s>>>(C.a, n, j0, j1, C.swp, C.ubuf, pj0, pj1);
     PMG_HIP_CHECK(hipGetLastError());
     if (j1 < n) {
       dim3 g((n - j1 + 63) / 64, (n - j1 + 63) / 64);

@@ -709,7 +709,7 @@ void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
 void coarse_alloc(pmg_ctx *c, Coarse &C, int nr) {
   if (C.a && C.n != nr) {
     for (void *p : {(void *)C.a, (void *)C.f, (void *)C.ipiv, (void *)C.sigma, (void *)C.b, (void *)C.y,
-                    (void *)C.x, (void *)C.ubuf, (void *)C.sync, (void *)C.flags})
+                    (void *)C.x, (void *)C.ubuf, (void *)C.swp, (void *)C.sync, (void *)C.flags})
       dfree(c, p);
     C.a = nullptr;
   }
@@ -723,6 +723,7 @@ void coarse_alloc(pmg_ctx *c, Coarse &C, int nr) {
     C.y = dalloc<double>(c, C.n);
     C.x = dalloc<double>(c, C.n);
     C.ubuf = dalloc<double>(c, size_t(C.n) * 16);
+    C.swp = dalloc<int>(c, 4 * 16 + 1);
     const int nblk = (C.n + 63) / 64;
     C.sync = dalloc<unsigned>(c, 4);
     C.flags = dalloc<unsigned>(c, 2 * size_t(nblk));

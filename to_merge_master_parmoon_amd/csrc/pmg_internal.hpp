@@ -198,6 +198,7 @@ struct Coarse {
   int *ipiv = nullptr;     // device
   int *sigma = nullptr;    // b'[i] = b[sigma[i]]
   double *ubuf = nullptr;  // U block row of the last LU panel step (n * 16)
+  int *swp = nullptr;      // row interchanges of the last LU panel (rows, sources, count)
   double *b = nullptr;     // work vectors of the two sweeps
   double *y = nullptr;
   int *info = nullptr;
