@@ -330,8 +330,8 @@ def main():
     PATCH_INVERSE = 0 if args.lu_sweeps else 1
     log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU decomposition "
-              "of this solver is described in DESIGN.md and not launched.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this solver has no multi-GPU "
+              "path (one problem, or one batch of independent problems, per GPU).", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
         print(json.dumps(run_batched(args, log)), flush=True)
