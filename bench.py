@@ -378,7 +378,7 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "fp32": "f32 smoother+level operator / f64 FGMRES+coarse",
-                  "bf16": "bf16 patch factors / f32 smoother / f64 FGMRES+coarse"}[args.precision],
+                  "bf16": "bf16 patch data / f32 smoother / f64 FGMRES+coarse"}[args.precision],
         "data": "synthetic",
         "config": cfg,
         "roofline": roof,
