@@ -81,10 +81,14 @@ typedef struct {
                                                   value change, the velocity rows stream that one
                                                   block; 0: always the nine blocks */
   int    patch_inverse;                        /* mixed precision only.  1 (default): each local
-                                                  Vanka solve multiplies by the explicit inverse of
-                                                  the patch matrix (FP32 Gauss-Jordan with the pivots
-                                                  of dgetf2, stored in the factor precision); 0: the
-                                                  stored LU factors and triangular sweeps.  FP64
+                                                  Vanka solve is a product with data stored once per
+                                                  operator in the factor precision: with the shared
+                                                  velocity block and colors of more than 2048
+                                                  patches A^{-1} of the 27 x 27 block, A^{-1} BT_c,
+                                                  B_c and the inverse of the 4 x 4 Schur complement,
+                                                  otherwise the explicit inverse of the patch matrix
+                                                  (FP32 Gauss-Jordan with the pivots of dgetf2); 0:
+                                                  the stored LU factors and triangular sweeps.  FP64
                                                   always uses the LU factors (dgetrf/dgetrs,
                                                   LinAlg.C:106-130) */
 } pmg_params;
