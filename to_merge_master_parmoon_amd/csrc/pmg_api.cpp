@@ -443,7 +443,9 @@ inline double solve_bytes(int fbytes, int vs) {
 
 // TNSE_MGLevel4::CellVanka, colored (NSE_MGLevel4.C:603-1649), one sweep
 template <typename T>
-void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
+// last = false: another sweep with the same f follows at once and sets the Dirichlet values
+// again before anything reads them, so this sweep's closing SetDirichletNodes is left out
+void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, bool last = true) {
   const bool prof = c->profiling;
   auto t0 = std::chrono::steady_clock::now();
   const double alpha = c->prm.sc_smooth_damp_factor_saddle;
@@ -514,7 +516,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc) {
     }
   }
   if (relax) launch_relax<T>(u, aux, T(alpha), D.n_dof, c->stream);
-  set_dirichlet_t<T>(c, D, u, f);
+  if (last) set_dirichlet_t<T>(c, D, u, f);
   if (prof) {
     c->tm.sweep_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->tm.sweep_calls += 1;
@@ -602,7 +604,8 @@ template <typename T>
 void cycle(pmg_ctx *c, int i) {
   DevLevel &D = c->L[i];
   T *u = Vec<T>::sol(D), *f = Vec<T>::rhs(D), *d = Vec<T>::aux0(D), *aux = Vec<T>::aux1(D);
-  correct_nodes_t<T>(c, D, u);
+  // CorrectNodes then SetDirichletNodes (NSE_MultiGrid.C:501-502): both act on the same
+  // slice of u and the second overwrites what the first writes, so only the second runs
   set_dirichlet_t<T>(c, D, u, f);
   if (i == 0) {
     if (c->batch)
@@ -616,7 +619,8 @@ void cycle(pmg_ctx *c, int i) {
   // them unchanged (their prolongated correction rows are empty)
   if (!D.coarse_insts.empty()) batch_coarse<T>(c, D, u, f);
   DevLevel &C = c->L[i - 1];
-  for (int j = 0; j < c->prm.sc_pre_smooth_saddle; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D));
+  for (int j = 0; j < c->prm.sc_pre_smooth_saddle; ++j)
+    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == c->prm.sc_pre_smooth_saddle);
   correct_nodes_t<T>(c, D, u);
   mg_defect<T>(c, D, u, f, d, nullptr);
   restrict_defect<T>(c, D, C, d, Vec<T>::rhs(C));
@@ -627,7 +631,8 @@ void cycle(pmg_ctx *c, int i) {
   correct_nodes_t<T>(c, D, d);
   launch_axpy<T>(u, T(c->prm.sc_gmg_damp_factor_saddle), d, D.n_dof, c->stream);
   correct_nodes_t<T>(c, D, u);
-  for (int j = 0; j < c->prm.sc_post_smooth_saddle; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D));
+  for (int j = 0; j < c->prm.sc_post_smooth_saddle; ++j)
+    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == c->prm.sc_post_smooth_saddle);
 }
 
 // the main stream waits for coarse factorizations still running on the second stream
