@@ -628,9 +628,13 @@ void cycle(pmg_ctx *c, int i) {
   launch_fill<T>(Vec<T>::sol(C), T(0), C.n_dof, c->stream);
   cycle<T>(c, i - 1);
   prolongate<T>(c, D, C, Vec<T>::sol(C), d);
-  correct_nodes_t<T>(c, D, d);
+  // CorrectNodes of the correction and of u (NSE_MultiGrid.C:1548, 1580) write only the
+  // Dirichlet slice of u, which the first post-smoothing sweep sets from f before reading
+  // it: with post-smoothing they are left out (same results)
+  const bool post = c->prm.sc_post_smooth_saddle > 0;
+  if (!post) correct_nodes_t<T>(c, D, d);
   launch_axpy<T>(u, T(c->prm.sc_gmg_damp_factor_saddle), d, D.n_dof, c->stream);
-  correct_nodes_t<T>(c, D, u);
+  if (!post) correct_nodes_t<T>(c, D, u);
   for (int j = 0; j < c->prm.sc_post_smooth_saddle; ++j)
     vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == c->prm.sc_post_smooth_saddle);
 }
