@@ -11,9 +11,10 @@
  * mkl.h unconditionally), so this restatement is pinned by the closed-form and brute-force
  * checks in tests/test_oracle_*.py.
  *
- * Parity unpinned: the step sequence of Cycle for V(nu1, nu2) with nu > 0 (smoothing inside
- * the recursion) is checked only through its parts and the V(0,0) closed form; the greedy
- * color order of oracle_color_cells is the reference's by reading, checked for validity.
+ * The step sequence of Cycle for V(nu1, nu2) with smoothing is pinned against a
+ * matrix-algebra statement of NSE_MultiGrid.C:434-1779 built from the separately pinned
+ * sweep, transfers and coarse solve.  The greedy color order of oracle_color_cells is the
+ * reference's by reading, checked for validity and against an O(N^2) restatement.
  */
 #ifndef PMG_ORACLE_H
 #define PMG_ORACLE_H

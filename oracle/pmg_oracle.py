@@ -190,6 +190,9 @@ class Oracle:
     def correct_defect(self, lvl, d):
         self._lib.oracle_correct_defect(self._h, lvl, dp(d))
 
+    def update(self, lvl, u, d):
+        self._lib.oracle_update(self._h, lvl, dp(u), dp(np.ascontiguousarray(d)))
+
     def restrict_defect(self, lvl, d_fine):
         d = d_fine.copy()
         out = np.empty(self.n_dof(lvl - 1))

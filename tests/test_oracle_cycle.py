@@ -3,6 +3,9 @@
 * two-level V-cycle without smoothing (V(0,0)) and an exact coarse solve is the coarse
   correction z = Z_D P Kc^-1 C R Z_D L20(v) in dense algebra (Z_D zeroes Dirichlet
   rows, C = CorrectDefect, Kc with the pressure row pinned);
+* V(nu1, nu2) with smoothing (damp, alpha, beta != 1; two and three levels) equals a
+  matrix-algebra statement of Cycle composed of the separately pinned sweep, transfers
+  and coarse solve;
 * FGMRES equals an independent dense statement of flexible GMRES (Saad 1993: Arnoldi
   with modified Gram-Schmidt on the right-preconditioned directions, Givens rotations,
   restarts) using the oracle's V-cycle as the black-box preconditioner, including the
@@ -150,4 +153,166 @@ def test_restart_rule_is_relative_to_cycle_start(oseen8):
     it, res, st = orc.fgmres(x, b)
     assert it == it_cycle == 4
     assert it_global == 3
+    orc.close()
+
+
+# ---------------------------------------------------------------------------------------
+# V(nu1, nu2) with smoothing inside the recursion, against a dense statement of
+# TNSE_MultiGrid::Cycle (NSE_MultiGrid.C:434-1779) composed from pieces pinned elsewhere:
+# the colored sweep (dense restatement, tests/test_oracle_vanka.py), the transfers
+# (R = P^T, exact reproduction, tests/test_oracle_transfer.py) and the coarse solve (dense
+# solve with the pressure row pinned).  A CorrectNodes or SetDirichletNodes out of place, a
+# wrong or missing beta in Update (NSE_MGLevel.C:121-129), the alpha relaxation applied per
+# color instead of per sweep, or the coarse correction added before pre-smoothing all
+# change the result by far more than the bound.
+
+def _sparse_operator(prob, lvl):
+    import scipy.sparse as sp
+    d = prob.descs[lvl]
+    a = prob.arrays(lvl)
+    N, NA, NP = d.n_udof, d.n_active, d.n_pdof
+    n = 3 * N + NP
+    rows, cols, vals = [], [], []
+    rows_a = np.repeat(np.arange(N), np.diff(a["a_rowptr"]))
+    cols_a = a["a_kcol"]
+    act = rows_a < NA
+    for r in range(3):
+        for c in range(3):
+            v = a["a"][3 * r + c]
+            m = np.ones_like(act) if r == c else act
+            rows.append(r * N + rows_a[m]); cols.append(c * N + cols_a[m]); vals.append(v[m])
+    rows_bt = np.repeat(np.arange(N), np.diff(a["bt_rowptr"]))
+    act_bt = rows_bt < NA
+    for r in range(3):
+        rows.append(r * N + rows_bt[act_bt]); cols.append(3 * N + a["bt_kcol"][act_bt])
+        vals.append(a["bt"][r][act_bt])
+    rows_b = np.repeat(np.arange(NP), np.diff(a["b_rowptr"]))
+    for c in range(3):
+        rows.append(3 * N + rows_b); cols.append(c * N + a["b_kcol"]); vals.append(a["b"][c])
+    K = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    K.sum_duplicates()
+    return K
+
+
+class _DenseCycle:
+    """The reference's Cycle restated in matrix algebra (one object per hierarchy)."""
+
+    def __init__(self, prob, orc, damp, alpha, beta, nu1, nu2):
+        import pmg_oracle
+        from scipy.linalg import lu_factor, lu_solve
+        self.prob, self.damp, self.alpha, self.beta, self.nu1, self.nu2 = prob, damp, alpha, beta, nu1, nu2
+        self.L = prob.n_levels
+        self.K = [_sparse_operator(prob, l) for l in range(self.L)]
+        self.colors = []
+        for l in range(self.L):
+            nc, ptr, reo = pmg_oracle.color_cells(prob.arrays(l)["u_global"])
+            self.colors.append([reo[ptr[k]:ptr[k + 1]] for k in range(nc)])
+        # transfers as matrices: P (prolongation of unit vectors through the pinned oracle
+        # transfer), R = P^T
+        self.P = [None]
+        for l in range(1, self.L):
+            nc_dof = orc.n_dof(l - 1)
+            cols = [orc.prolongate(l, np.eye(1, nc_dof, k).ravel()) for k in range(nc_dof)]
+            self.P.append(np.array(cols).T)
+        d0 = prob.descs[0]
+        Kc = self.K[0].toarray()
+        N3 = 3 * d0.n_udof
+        Kc[N3, :] = 0.0
+        Kc[N3, N3] = 1.0
+        self.coarse = lu_factor(Kc)
+        self._lu_solve = lu_solve
+        self.patch = {}
+
+    def _dir(self, l):
+        d = self.prob.descs[l]
+        N, NA = d.n_udof, d.n_active
+        return np.concatenate([np.arange(c * N + NA, (c + 1) * N) for c in range(3)])
+
+    def _sweep(self, l, u, f):
+        from dense_ref import patch_indices
+        K = self.K[l]
+        D = self._dir(l)
+        u = u.copy()
+        u[D] = f[D]
+        uold = u.copy()
+        for cells in self.colors[l]:
+            r = f - K @ u
+            upd = []
+            for cell in cells:
+                idx = patch_indices(self.prob, l, cell)
+                key = (l, int(cell))
+                if key not in self.patch:
+                    self.patch[key] = K[idx][:, idx].toarray()
+                upd.append((idx, np.linalg.solve(self.patch[key], r[idx])))
+            for idx, dl in upd:
+                u[idx] += self.damp * dl
+        if abs(1 - self.alpha) > 1e-3:
+            u = uold + self.alpha * (u - uold)
+        u[D] = f[D]
+        return u
+
+    def _defect(self, l, u, f):
+        d = self.prob.descs[l]
+        N3 = 3 * d.n_udof
+        r = f - self.K[l] @ u
+        r[N3:] = into_l20(r[N3:])
+        r[self._dir(l)] = 0.0
+        return r
+
+    def cycle(self, l, u, f):
+        D = self._dir(l)
+        u = u.copy()
+        u[D] = 0.0                      # CorrectNodes (:501)
+        u[D] = f[D]                     # SetDirichletNodes (:502)
+        if l == 0:                      # SolveExact (:546-557)
+            N3 = 3 * self.prob.descs[0].n_udof
+            fp = f.copy()
+            fp[N3] = 0.0
+            return self._lu_solve(self.coarse, fp)
+        for _ in range(self.nu1):       # pre-smoothing (:1195-1203)
+            u = self._sweep(l, u, f)
+        u[D] = 0.0                      # CorrectNodes (:1340)
+        dfine = self._defect(l, u, f)   # Defect (:1341)
+        fc = self.P[l].T @ dfine        # DefectRestriction (:1407-1408)
+        dc = self.prob.descs[l - 1]
+        fc[self._dir(l - 1)] = 0.0      # CorrectDefect (:1449)
+        fc[3 * dc.n_udof:] = into_l20(fc[3 * dc.n_udof:])
+        uc = self.cycle(l - 1, np.zeros_like(fc), fc)   # Reset + Cycle (:1450-1460)
+        e = self.P[l] @ uc              # Prolongate (:1510-1514)
+        e[D] = 0.0                      # CorrectNodes of the correction (:1548)
+        u = u + self.beta * e           # Update (:1549)
+        u[D] = 0.0                      # CorrectNodes (:1580)
+        for _ in range(self.nu2):       # post-smoothing (:1610-1618)
+            u = self._sweep(l, u, f)
+        return u
+
+    def vcycle(self, v):
+        """TMultiGridIte::Iterate with zero start (MultiGridIte.C:76-130)."""
+        top = self.L - 1
+        z = np.zeros_like(v)
+        z[self._dir(top)] = v[self._dir(top)]
+        return self.cycle(top, z, v)
+
+
+@pytest.mark.parametrize("coarsest,n,nus", [(4, 8, (1, 1)), (4, 8, (2, 2)), (4, 8, (0, 2)), (4, 8, (2, 0)),
+                                            (2, 8, (2, 1))])
+def test_vcycle_with_smoothing_matches_dense_cycle(coarsest, n, nus):
+    from to_merge_master_parmoon_amd import problem
+    prob = problem.oseen3d(n, coarsest=coarsest)
+    damp, alpha, beta = 0.8, 0.9, 0.7
+    prm = params(sc_pre_smooth_saddle=nus[0], sc_post_smooth_saddle=nus[1], sc_smooth_damp_factor_coarse_saddle=damp,
+                 sc_smooth_damp_factor_saddle=alpha, sc_gmg_damp_factor_saddle=beta)
+    orc = pmg_oracle.Oracle(prm, prob.descs)
+    dense = _DenseCycle(prob, orc, damp, alpha, beta, *nus)
+    v = np.random.default_rng(11 + nus[0] + 3 * nus[1]).standard_normal(orc.n_dof(prob.n_levels - 1))
+    z, st = orc.vcycle(v)
+    assert st == 0
+    zd = dense.vcycle(v)
+    e = rel(z, zd)
+    print(f"levels {prob.n_levels}, V{nus}: oracle vs dense cycle {e:.2e}")
+    assert e <= 1e-12
+    # the bound separates the reference's sequence from plausible misorderings: without the
+    # beta of Update, or with the relaxation alpha left out, the cycle moves by O(1e-1)
+    dense.beta = 1.0
+    assert rel(z, dense.vcycle(v)) > 1e-3
     orc.close()
