@@ -1,0 +1,66 @@
+"""The coarse solve of the mixed-precision forms, through the C ABI.
+
+With the shared velocity block the mixed forms solve the coarsest level in block form:
+FP64 LU factors of the velocity block A and of the pressure Schur complement
+S = -sum_b B_b A^{-1} BT_b (one workgroup each), the explicit FP64 inverses A^{-1}, S^{-1}
+and the products W_b = A^{-1} BT_b, -S^{-1} B_b A^{-1} (matrix-core GEMMs and blocked
+substitutions), and per visit two matrix-vector phases.  It is the same solution as the
+LU of the whole reduced system up to round-off amplified by the conditioning, so it is
+checked against the oracle's SolveExact (dgetrf/dgetrs on the dense system with the pinned
+pressure row) to 1e-10 relative, on the 4^3 level (343 + 255 unknowns after the pin) and
+on a 2^3 level, in FP32 and BF16 smoother configurations (the coarse solve is FP64 in
+both).  The FP64 form keeps the LU of the whole system (bit-identical factors).
+"""
+import numpy as np
+import pytest
+
+from conftest import params, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _hier(prob, **kw):
+    from to_merge_master_parmoon_amd.multigrid import build_hierarchy
+    return build_hierarchy(prob, params(**kw))
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("coarsest", [4, 2])
+def test_block_coarse_solve_matches_oracle(precision, coarsest):
+    import pmg_oracle
+    from to_merge_master_parmoon_amd import problem
+    prob = problem.oseen3d(2 * coarsest, coarsest=coarsest)
+    mg = _hier(prob, precision=precision)
+    assert mg.shared_block(0)
+    orc = pmg_oracle.Oracle(params(), prob.descs, refactor=0)
+    for seed in (0, 1):
+        f = np.random.default_rng(500 + seed).standard_normal(mg.n_dof[0])
+        u_g = mg.solve_exact(f)
+        u_c, info = orc.solve_exact(f)
+        assert info == 0
+        e = rel(u_g, u_c)
+        print(f"coarsest {coarsest}^3 precision {precision}: block coarse solve vs SolveExact {e:.2e}", flush=True)
+        assert e <= 1e-10
+    mg.close()
+    orc.close()
+
+
+def test_block_coarse_after_new_values():
+    """New operator values (scaled convection) and pmg_refactor: the block form is rebuilt
+    and matches the oracle built on the same values."""
+    import pmg_oracle
+    from to_merge_master_parmoon_amd import problem
+    prob = problem.oseen3d(8)
+    mg = _hier(prob, precision=2)
+    a = prob.arrays(0)
+    for arr in a["a"]:
+        arr *= 1.5
+    mg.update_values(0, prob.descs[0])
+    mg.refactor()
+    orc = pmg_oracle.Oracle(params(), prob.descs, refactor=0)
+    f = np.random.default_rng(510).standard_normal(mg.n_dof[0])
+    e = rel(mg.solve_exact(f), orc.solve_exact(f)[0])
+    print(f"after new values: {e:.2e}", flush=True)
+    assert e <= 1e-10
+    mg.close()
+    orc.close()

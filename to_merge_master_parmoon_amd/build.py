@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["pmg_api.cpp", "batch_merge.cpp", "coloring.cpp", "k_operator.hip", "k_vanka.hip", "k_dense.hip",
-               "k_batch.hip", "k_assemble.hip"]
+               "k_batch.hip", "k_assemble.hip", "k_coarse.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     # no packed FP32 instructions anywhere in the device code
@@ -43,11 +43,34 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
 
 
 def build_pmg() -> Path:
+    """One hipcc per source file in parallel (each file holds its kernels and their
+    launchers, so no relocatable device code is needed), then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
     LIB.mkdir(exist_ok=True)
+    obj_dir = PKG / "_build"
+    obj_dir.mkdir(exist_ok=True)
     out = LIB / "libpmg.so"
     tmp = LIB / "libpmg.so.tmp"
-    srcs = [CSRC / s for s in HIP_SOURCES]
-    _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", *srcs, "-o", tmp])
+    compile_flags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [obj_dir / (Path(s).stem + ".o") for s in HIP_SOURCES]
+
+    # PMG_INCREMENTAL=1 (development loop only): keep objects newer than their source and
+    # every header; build() always recompiles everything
+    incremental = os.environ.get("PMG_INCREMENTAL") == "1"
+    headers = list(CSRC.glob("*.hpp")) + list(INCLUDE.glob("*.h"))
+    newest_header = max((h.stat().st_mtime for h in headers), default=0.0)
+
+    def one(src_obj):
+        src, obj = src_obj
+        if incremental and obj.exists() and obj.stat().st_mtime > max((CSRC / src).stat().st_mtime, newest_header):
+            return
+        _run([HIPCC, *compile_flags, "-c", f"-I{INCLUDE}", f"-I{CSRC}", CSRC / src, "-o", obj])
+
+    jobs = int(os.environ.get("MAX_JOBS", "0")) or min(len(HIP_SOURCES), os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(one, zip(HIP_SOURCES, objs)))
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -1,0 +1,550 @@
+// k_coarse.hip - block-form FP64 coarse solve of the mixed-precision forms (gfx950).
+//
+// TNSE_MGLevel4::SolveExact (src/FE/NSE_MGLevel4.C:5728-5846) solves the coarsest level
+// directly.  With the shared velocity block (A11 = A22 = A33, no coupling blocks on the
+// active rows, the Galerkin Oseen form) the reduced coarse system (Dirichlet rows and the
+// pinned pressure row eliminated, see k_dense.hip) is
+//
+//     [ A  0  0  BT1 ] [x1]   [r1]
+//     [ 0  A  0  BT2 ] [x2] = [r2]        A: m x m (active velocity dofs of one component)
+//     [ 0  0  A  BT3 ] [x3]   [r3]        BTb: m x q, Bb: q x m (q free pressure dofs)
+//     [ B1 B2 B3  0  ] [p ]   [rp]
+//
+// and block elimination gives, with Ainv = A^{-1}, Wb = Ainv BTb, S = -sum_b Bb Wb:
+//     p  = S^{-1} rp - sum_b Qb rb,       Qb = S^{-1} Bb Ainv
+//     xb = Ainv rb - Wb p.
+// Set-up (once per operator, FP64, on the second stream): LU with partial pivoting of A and
+// of S, each by one workgroup (panels of 16 columns in registers, row interchanges, the U
+// block row, and the trailing update on the matrix cores: v_mfma_f64_16x16x4f64); the
+// solves with many right-hand sides [I | BT1 BT2 BT3] and I by blocked substitution (one
+// workgroup per 16 columns, diagonal blocks in one wavefront, updates on the matrix
+// cores); the products S, Bb Ainv and S^{-1}(...) by an FP64 MFMA GEMM.  Each visit is then
+// two matrix-vector phases over explicit FP64 data (Ainv, [-Q | S^{-1}] and W: 5.6 MB at
+// the 4^3 level) instead of two chains of dependent triangular sweeps; Ainv and W are the
+// columns of X = A^{-1} [I | BT1 BT2 BT3], read in place.
+//
+// Lane layout of v_mfma_f64_16x16x4f64 (D = A B + C, A 16 x 4, B 4 x 16): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and holds D[4 i + l / 16][l % 16] in its i-th
+// result register.
+#include "pmg_internal.hpp"
+#include "pmg_rows.hpp"
+
+namespace pmg {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline d4 mfma16(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+constexpr int LU_NB = 16;          // panel width
+constexpr int LU_T = 512;          // threads of the single-workgroup LU (one panel row each)
+constexpr int TRSM_T = 256;        // threads of the substitution kernel
+constexpr int TRSM_NR = 16;        // right-hand sides per workgroup
+
+// ------------------------------------------------------------------ dense blocks
+// A (m x m), R = [I | BT1 BT2 BT3] (m x (m + 3q)), B = [B1 B2 B3] (q x 3m), all row-major,
+// zero-filled beforehand.  Reduced numbering as in k_dense.hip: active velocity dof r,
+// free pressure dof i - pin.
+__global__ __launch_bounds__(256) void k_cb_scatter(OpView<double> op, double *__restrict__ A, double *__restrict__ R,
+                                                    double *__restrict__ B, int m, int q, int pin) {
+  const int lane = threadIdx.x & 15;
+  const int r = (blockIdx.x * 256 + threadIdx.x) >> 4;   // 16 lanes per operator row
+  const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
+  const long ldr = long(m) + 3L * q;
+  if (r < NA) {
+    for (int k = op.a_rowptr[r] + lane; k < op.a_rowptr[r + 1]; k += 16) {
+      const int c = op.a_kcol[k];
+      if (c < NA) A[long(r) * m + c] = op.a[k];   // block A11 (= A22 = A33)
+    }
+    if (lane == 0) R[long(r) * ldr + r] = 1.0;
+    for (int k = op.bt_rowptr[r] + lane; k < op.bt_rowptr[r + 1]; k += 16) {
+      const int i = op.bt_kcol[k];
+      if (pin && i == 0) continue;
+      for (int b = 0; b < 3; ++b) R[long(r) * ldr + m + long(b) * q + (i - pin)] = op.bt[b * op.bt_nnz + k];
+    }
+  } else if (r >= N && r < N + NP) {
+    const int i = r - N;
+    if (pin && i == 0) return;
+    for (int k = op.b_rowptr[i] + lane; k < op.b_rowptr[i + 1]; k += 16) {
+      const int c = op.b_kcol[k];
+      if (c >= NA) continue;
+      for (int b = 0; b < 3; ++b) B[long(i - pin) * (3L * m) + long(b) * m + c] = op.b[b * op.b_nnz + k];
+    }
+  }
+}
+
+__global__ void k_cb_zero(double *x, long n) {
+  for (long i = blockIdx.x * long(blockDim.x) + threadIdx.x; i < n; i += long(gridDim.x) * blockDim.x) x[i] = 0.0;
+}
+
+// ------------------------------------------------------------------ single-workgroup LU
+// a (n x n, row-major, leading dimension ld) := L U with the row interchanges of dgetf2
+// (pivot = first row of largest |a(i, j)|, i >= j); sigma[i] = original row now at row i;
+// *info = 1 + the first column with a zero pivot (0: none).  n <= LU_T.  The panel (rows
+// j0.., 16 columns) is factored in LDS, one row per thread.
+__global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, int ld, int *__restrict__ sigma,
+                                                 int *__restrict__ info) {
+  __shared__ double Lp[LU_T][LU_NB + 1];     // the panel
+  __shared__ double Us[LU_NB][LU_T];         // the U block row of the panel (columns j0 + 16 ..)
+  __shared__ double wv[2][LU_T / 64];
+  __shared__ int wi[2][LU_T / 64];
+  __shared__ int pivs[LU_NB];
+  __shared__ int sig[LU_T];
+  __shared__ int rows_s[2 * LU_NB], src_s[2 * LU_NB];
+  __shared__ int nk_s, info_s;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  if (t < n) sig[t] = t;
+  if (t == 0) info_s = 0;
+  for (int j0 = 0; j0 < n; j0 += LU_NB) {
+    const int w = min(LU_NB, n - j0);
+    const int r = j0 + t;
+    const bool own = r < n;
+#pragma unroll
+    for (int cc = 0; cc < LU_NB; ++cc) Lp[t][cc] = (own && cc < w) ? a[long(r) * ld + j0 + cc] : 0.0;
+    __syncthreads();
+#pragma unroll 1
+    for (int jj = 0; jj < w; ++jj) {
+      const int j = j0 + jj, par = jj & 1;
+      double bv = (own && r >= j) ? fabs(Lp[t][jj]) : -1.0;
+      int bi = (own && r >= j) ? r : n;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      if (lane == 0) {
+        wv[par][wave] = bv;
+        wi[par][wave] = bi;
+      }
+      __syncthreads();
+      bv = wv[par][0];
+      bi = wi[par][0];
+#pragma unroll
+      for (int w2 = 1; w2 < LU_T / 64; ++w2) {
+        const double ov = wv[par][w2];
+        const int oi = wi[par][w2];
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      const int p = bi;
+      if (t == 0) {
+        pivs[jj] = p;
+        if (bv == 0.0 && info_s == 0) info_s = j + 1;
+      }
+      if (bv == 0.0) continue;   // uniform: dgetf2 records the zero pivot and moves on
+      if (p != j && t < LU_NB) {
+        const double x = Lp[jj][t];
+        Lp[jj][t] = Lp[p - j0][t];
+        Lp[p - j0][t] = x;
+      }
+      __syncthreads();
+      if (own && r > j) {
+        const double l = Lp[t][jj] / Lp[jj][jj];
+        Lp[t][jj] = l;
+        for (int cc = jj + 1; cc < LU_NB; ++cc) Lp[t][cc] = fma(-l, Lp[jj][cc], Lp[t][cc]);
+      }
+    }
+    __syncthreads();
+    // the factored panel to memory
+    if (own)
+#pragma unroll
+      for (int cc = 0; cc < LU_NB; ++cc)
+        if (cc < w) a[long(r) * ld + j0 + cc] = Lp[t][cc];
+    if (wave == 0) {
+      // rows touched by the panel's interchanges (slots 0..w-1: j0..j0+w-1, then the pivot
+      // rows below the panel in order of first use) and the source row whose original
+      // values end in each after the sequential interchanges
+      int row = lane < w ? j0 + lane : -1;
+      int nk = w;
+      for (int k = 0; k < w; ++k) {
+        const int p = pivs[k];
+        if (p >= j0 + w && __ballot(row == p) == 0) {
+          if (lane == nk) row = p;
+          ++nk;
+        }
+      }
+      int src = row;
+      for (int k = 0; k < w; ++k) {
+        const int p = pivs[k];
+        if (p == j0 + k) continue;
+        const unsigned long long msk = __ballot(row == p);
+        const int f = __ffsll((long long)msk) - 1;
+        const int sk = __shfl(src, k, 64), sf = __shfl(src, f, 64);
+        if (lane == k) src = sf;
+        if (lane == f) src = sk;
+      }
+      if (lane < 2 * LU_NB) {
+        rows_s[lane] = row;
+        src_s[lane] = src;
+      }
+      if (lane == 0) nk_s = nk;
+      const int sg = (lane < nk) ? sig[src] : 0;
+      __builtin_amdgcn_wave_barrier();
+      if (lane < nk && src != row) sig[row] = sg;
+    }
+    __syncthreads();
+    // interchanges on every column outside the panel: the <= 32 rows involved, staged
+    // through LDS (the U block row buffer) 256 columns at a time, read before written
+    {
+      const int nk = nk_s;
+      double *stage = &Us[0][0];   // 32 x 256
+      for (int cb = 0; cb < n; cb += 256) {
+        for (int idx = t; idx < nk * 256; idx += LU_T) {
+          const int k = idx >> 8, c = cb + (idx & 255);
+          stage[idx] = c < n ? a[long(src_s[k]) * ld + c] : 0.0;
+        }
+        __syncthreads();
+        for (int idx = t; idx < nk * 256; idx += LU_T) {
+          const int k = idx >> 8, c = cb + (idx & 255);
+          if (c < n && (c < j0 || c >= j0 + w) && src_s[k] != rows_s[k]) a[long(rows_s[k]) * ld + c] = stage[idx];
+        }
+        __syncthreads();
+      }
+    }
+    const int j1 = j0 + w, nt = n - j1;
+    if (nt <= 0) break;
+    // U block row: U12 = L11^{-1} A12 (unit lower), columns j1.. (one column per thread,
+    // nt < LU_T), solved in place in LDS
+    if (t < nt) {
+      const int c = j1 + t;
+#pragma unroll
+      for (int k = 0; k < LU_NB; ++k) Us[k][t] = k < w ? a[long(j0 + k) * ld + c] : 0.0;
+#pragma unroll 1
+      for (int k = 0; k < LU_NB - 1; ++k) {
+        const double uk = Us[k][t];
+        for (int i = k + 1; i < LU_NB; ++i) Us[i][t] = fma(-Lp[i][k], uk, Us[i][t]);
+      }
+#pragma unroll
+      for (int k = 0; k < LU_NB; ++k)
+        if (k < w) a[long(j0 + k) * ld + c] = Us[k][t];
+    } else {
+#pragma unroll
+      for (int k = 0; k < LU_NB; ++k) Us[k][t] = 0.0;
+    }
+    __syncthreads();
+    // trailing update A22 -= L21 U12 on 16 x 16 tiles, one wavefront per tile
+    const int nb = (nt + 15) / 16;
+    const int col = lane & 15, rq = lane >> 4;
+    for (int tile = wave; tile < nb * nb; tile += LU_T / 64) {
+      const int ti = tile / nb, tc = tile - ti * nb;
+      const int i0 = j1 + ti * 16, c0 = j1 + tc * 16;
+      d4 acc;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int i = i0 + 4 * qq + rq, c = c0 + col;
+        acc[qq] = (i < n && c < n) ? a[long(i) * ld + c] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const double av = -Lp[i0 - j0 + col][4 * s + rq];
+        const double bv = Us[4 * s + rq][c0 - j1 + col];
+        acc = mfma16(av, bv, acc);
+      }
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int i = i0 + 4 * qq + rq, c = c0 + col;
+        if (i < n && c < n) a[long(i) * ld + c] = acc[qq];
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (t < n) sigma[t] = sig[t];
+  if (t == 0) *info = info_s;
+}
+
+// ------------------------------------------------------------------ many right-hand sides
+// X[:, c0 .. c0+15] = (L U)^{-1} P R[:, c0 ..] for the factors of k_lu_wg (sigma: the row
+// permutation P), one workgroup per 16 columns; R == nullptr: R = I (nrhs = n).  The slab
+// lives in LDS; per 16-row block the diagonal solve runs in one wavefront and the rows
+// below (forward) or above (backward) are updated on the matrix cores.  n <= 512.
+__global__ __launch_bounds__(TRSM_T) void k_trsm_wg(const double *__restrict__ lu, int n, int ld,
+                                                     const int *__restrict__ sigma, const double *__restrict__ R,
+                                                     int ldr, int nrhs, double *__restrict__ X, int ldx) {
+  __shared__ double xs[512][TRSM_NR + 1];
+  __shared__ double dg[16][17];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int c0 = blockIdx.x * TRSM_NR;
+  for (int idx = t; idx < n * TRSM_NR; idx += TRSM_T) {
+    const int i = idx / TRSM_NR, cc = idx - i * TRSM_NR, c = c0 + cc;
+    double v = 0.0;
+    if (c < nrhs) v = R ? R[long(sigma[i]) * ldr + c] : (sigma[i] == c ? 1.0 : 0.0);
+    xs[i][cc] = v;
+  }
+  const int nb = (n + 15) / 16;
+  const int col = lane & 15, rq = lane >> 4;
+  // forward: unit lower L
+  for (int kb = 0; kb < nb; ++kb) {
+    const int k0 = kb * 16, kw = min(16, n - k0);
+    __syncthreads();
+    {
+      const int i = t >> 4, k = t & 15;
+      dg[i][k] = (i < kw && k < kw) ? lu[long(k0 + i) * ld + k0 + k] : 0.0;
+    }
+    __syncthreads();
+    if (t < TRSM_NR) {
+      double x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = i < kw ? xs[k0 + i][t] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+#pragma unroll
+        for (int i = k + 1; i < 16; ++i) x[i] = fma(-dg[i][k], x[k], x[i]);
+        __asm__ __volatile__("" ::: "memory");
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < kw) xs[k0 + i][t] = x[i];
+    }
+    __syncthreads();
+    for (int tb = kb + 1 + wave; tb < nb; tb += TRSM_T / 64) {
+      const int i0 = tb * 16;
+      d4 acc;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int i = i0 + 4 * qq + rq;
+        acc[qq] = i < n ? xs[i][col] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int i = i0 + col, k = k0 + 4 * s + rq;
+        const double av = (i < n && k < k0 + kw) ? -lu[long(i) * ld + k] : 0.0;
+        const double bv = (4 * s + rq < kw) ? xs[k][col] : 0.0;
+        acc = mfma16(av, bv, acc);
+      }
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int i = i0 + 4 * qq + rq;
+        if (i < n) xs[i][col] = acc[qq];
+      }
+    }
+  }
+  // backward: upper U with its diagonal
+  for (int kb = nb - 1; kb >= 0; --kb) {
+    const int k0 = kb * 16, kw = min(16, n - k0);
+    __syncthreads();
+    {
+      const int i = t >> 4, k = t & 15;
+      dg[i][k] = (i < kw && k < kw) ? lu[long(k0 + i) * ld + k0 + k] : 1.0;
+    }
+    __syncthreads();
+    if (t < TRSM_NR) {
+      double x[16];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) x[i] = i < kw ? xs[k0 + i][t] : 0.0;
+#pragma unroll
+      for (int k = 15; k >= 0; --k) {
+        x[k] = x[k] / dg[k][k];
+#pragma unroll
+        for (int i = 0; i < k; ++i) x[i] = fma(-dg[i][k], x[k], x[i]);
+        __asm__ __volatile__("" ::: "memory");
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i)
+        if (i < kw) xs[k0 + i][t] = x[i];
+    }
+    __syncthreads();
+    for (int tb = wave; tb < kb; tb += TRSM_T / 64) {
+      const int i0 = tb * 16;
+      d4 acc;
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) acc[qq] = xs[i0 + 4 * qq + rq][col];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int i = i0 + col, k = k0 + 4 * s + rq;
+        const double av = (4 * s + rq < kw) ? -lu[long(i) * ld + k] : 0.0;
+        const double bv = (4 * s + rq < kw) ? xs[k][col] : 0.0;
+        acc = mfma16(av, bv, acc);
+      }
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) xs[i0 + 4 * qq + rq][col] = acc[qq];
+    }
+  }
+  __syncthreads();
+  for (int idx = t; idx < n * TRSM_NR; idx += TRSM_T) {
+    const int i = idx / TRSM_NR, cc = idx - i * TRSM_NR, c = c0 + cc;
+    if (c < nrhs) X[long(i) * ldx + c] = xs[i][cc];
+  }
+}
+
+// ------------------------------------------------------------------ FP64 GEMM
+// C = alpha A B + beta C (row-major; A M x K, B K x N); one wavefront per 16 x 16 tile of C,
+// four per workgroup, K in steps of 4 on the matrix cores.
+__global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, double alpha, const double *__restrict__ A,
+                                                  int lda, const double *__restrict__ B, int ldb, double beta,
+                                                  double *__restrict__ C, int ldc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tn = (N + 15) / 16;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= ((M + 15) / 16) * tn) return;
+  const int i0 = (tile / tn) * 16, c0 = (tile % tn) * 16;
+  const int col = lane & 15, rq = lane >> 4;
+  const int ia = i0 + col, cb = c0 + col;
+  d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < K; k += 16) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kk = k + 4 * s + rq;
+      av[s] = (ia < M && kk < K) ? A[long(ia) * lda + kk] : 0.0;
+      bv[s] = (cb < N && kk < K) ? B[long(kk) * ldb + cb] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc = mfma16(av[s], bv[s], acc);
+  }
+#pragma unroll
+  for (int qq = 0; qq < 4; ++qq) {
+    const int i = i0 + 4 * qq + rq;
+    if (i < M && cb < N) {
+      double *cp = C + long(i) * ldc + cb;
+      *cp = beta == 0.0 ? alpha * acc[qq] : fma(alpha, acc[qq], beta * *cp);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ per-visit phases
+// phase 1: y_b = Ainv r_b (b = 0..2, one wavefront per row r of Ainv) and
+// p = [-Q | S^{-1}] r (one wavefront per pressure row)
+__global__ __launch_bounds__(256) void k_cb_phase1(const double *__restrict__ Ainv, int lda,
+                                                   const double *__restrict__ P1, const double *__restrict__ rr,
+                                                   double *__restrict__ y, double *__restrict__ p, int m, int q) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row < m) {
+    const double *ar = Ainv + long(row) * lda;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int c = lane; c < m; c += 64) {
+      const double v = ar[c];
+      s0 = fma(v, rr[c], s0);
+      s1 = fma(v, rr[m + c], s1);
+      s2 = fma(v, rr[2 * m + c], s2);
+    }
+    s0 = red64(s0);
+    s1 = red64(s1);
+    s2 = red64(s2);
+    if (lane == 0) {
+      y[row] = s0;
+      y[m + row] = s1;
+      y[2 * m + row] = s2;
+    }
+  } else if (row < m + q) {
+    const int i = row - m;
+    const long nr = 3L * m + q;
+    const double *pr = P1 + long(i) * nr;
+    double s = 0.0;
+    for (long k = lane; k < nr; k += 64) s = fma(pr[k], rr[k], s);
+    s = red64(s);
+    if (lane == 0) p[i] = s;
+  }
+}
+
+// phase 2: x_b = y_b - W_b p into the level vector u (active velocity dofs, one wavefront per
+// row of W), the Dirichlet velocity dofs (f_D / diagonal) and the pressure (0 at the pinned
+// dof) by the workgroups past the velocity rows
+__global__ __launch_bounds__(256) void k_cb_phase2(OpView<double> op, const double *__restrict__ W, int ldw,
+                                                   const double *__restrict__ y, const double *__restrict__ p,
+                                                   const double *__restrict__ f, double *__restrict__ u, int m,
+                                                   int q, int pin, int vel_blocks) {
+  const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
+  if (int(blockIdx.x) < vel_blocks) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= m) return;
+    const double *wr = W + long(r) * ldw;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = lane; i < q; i += 64) {
+      const double pi = p[i];
+      s0 = fma(wr[i], pi, s0);
+      s1 = fma(wr[q + i], pi, s1);
+      s2 = fma(wr[2 * q + i], pi, s2);
+    }
+    s0 = red64(s0);
+    s1 = red64(s1);
+    s2 = red64(s2);
+    if (lane == 0) {
+      u[r] = y[r] - s0;
+      u[long(N) + r] = y[m + r] - s1;
+      u[2L * N + r] = y[2 * m + r] - s2;
+    }
+    return;
+  }
+  const long idx = long(blockIdx.x - vel_blocks) * 256 + threadIdx.x;
+  const long nd = 3L * (N - NA);
+  if (idx < nd) {
+    const int b = int(idx / (N - NA)), c = NA + int(idx - long(b) * (N - NA));
+    const int k = op.a_rowptr[c];   // Dirichlet rows hold only their diagonal
+    u[long(b) * N + c] = f[long(b) * N + c] / op.a[(4 * b) * op.a_nnz + k];
+  } else if (idx < nd + NP) {
+    const int i = int(idx - nd);
+    u[3L * N + i] = (pin && i == 0) ? 0.0 : p[i - pin];
+  }
+}
+
+}  // namespace
+
+bool coarse_block_fits(int m, int q) { return m > 0 && q > 0 && m <= LU_T && q <= LU_T; }
+
+void launch_cb_build(const DevLevel &L, CoarseBlock &B, hipStream_t s) {
+  const long m = B.m, q = B.q;
+  k_cb_zero<<<1024, 256, 0, s>>>(B.A, m * m);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_cb_zero<<<1024, 256, 0, s>>>(B.X, m * (m + 3 * q));   // R, overwritten by the solve
+  PMG_HIP_CHECK(hipGetLastError());
+  k_cb_zero<<<1024, 256, 0, s>>>(B.B, q * 3 * m);
+  PMG_HIP_CHECK(hipGetLastError());
+  const int rows = L.n_udof + L.n_pdof;
+  k_cb_scatter<<<(rows * 16 + 255) / 256, 256, 0, s>>>(L.op64(), B.A, B.X, B.B, B.m, B.q, B.pin);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+namespace {
+void gemm(int M, int N, int K, double alpha, const double *A, int lda, const double *B, int ldb, double beta,
+          double *C, int ldc, hipStream_t s) {
+  const long tiles = long((M + 15) / 16) * ((N + 15) / 16);
+  k_gemm_f64<<<int((tiles + 3) / 4), 256, 0, s>>>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void coarse_block_factor(CoarseBlock &B, hipStream_t s) {
+  const int m = B.m, q = B.q;
+  const int ldx = m + 3 * q;          // X = [Ainv | W1 W2 W3]
+  const int ldp = 3 * m + q;          // P1 = [-Q1 -Q2 -Q3 | S^{-1}]
+  // A = P^T L U; X = A^{-1} [I | BT1 BT2 BT3] (in place: the solve reads its slab first)
+  k_lu_wg<<<1, LU_T, 0, s>>>(B.A, m, m, B.sigA, B.info);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_trsm_wg<<<(ldx + TRSM_NR - 1) / TRSM_NR, TRSM_T, 0, s>>>(B.A, m, m, B.sigA, B.X, ldx, ldx, B.X, ldx);
+  PMG_HIP_CHECK(hipGetLastError());
+  // S = -sum_b B_b W_b
+  for (int b = 0; b < 3; ++b)
+    gemm(q, q, m, -1.0, B.B + long(b) * m, 3 * m, B.X + m + long(b) * q, ldx, b == 0 ? 0.0 : 1.0, B.S, q, s);
+  // V_b = B_b Ainv
+  for (int b = 0; b < 3; ++b) gemm(q, m, m, 1.0, B.B + long(b) * m, 3 * m, B.X, ldx, 0.0, B.V + long(b) * m, 3 * m, s);
+  // S = P^T L U; S^{-1} into the last q columns of P1; -Q = -S^{-1} V into the first 3m
+  k_lu_wg<<<1, LU_T, 0, s>>>(B.S, q, q, B.sigS, B.info + 1);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_trsm_wg<<<(q + TRSM_NR - 1) / TRSM_NR, TRSM_T, 0, s>>>(B.S, q, q, B.sigS, nullptr, 0, q, B.P1 + 3L * m, ldp);
+  PMG_HIP_CHECK(hipGetLastError());
+  gemm(q, 3 * m, q, -1.0, B.P1 + 3L * m, ldp, B.V, 3 * m, 0.0, B.P1, ldp, s);
+}
+
+void coarse_block_solve(CoarseBlock &B, const OpView<double> &op, const double *rr, const double *f, double *u,
+                        hipStream_t s) {
+  const int m = B.m, q = B.q;
+  k_cb_phase1<<<(m + q + 3) / 4, 256, 0, s>>>(B.X, m + 3 * q, B.P1, rr, B.y, B.p, m, q);
+  PMG_HIP_CHECK(hipGetLastError());
+  const int vel_blocks = (m + 3) / 4;
+  const long rest = 3L * (op.n_udof - op.n_active) + op.n_pdof;
+  k_cb_phase2<<<vel_blocks + int((rest + 255) / 256), 256, 0, s>>>(op, B.X + m, m + 3 * q, B.y, B.p, f, u, m, q,
+                                                                    B.pin, vel_blocks);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace pmg

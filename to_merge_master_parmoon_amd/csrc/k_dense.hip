@@ -603,6 +603,12 @@ void coarse_lu_factor(Coarse &C, hipStream_t s) {
   PMG_HIP_CHECK(hipGetLastError());
 }
 
+void launch_coarse_rhs(const OpView<double> &op, const double *f, const int *sigma, double *b, int n, int pin,
+                       hipStream_t s) {
+  k_coarse_rhs<<<(n + 15) / 16, 256, 0, s>>>(op, f, sigma, b, n, pin);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
 void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s) {
   const int n = C.n;
   const int nblk = (n + TB - 1) / TB;

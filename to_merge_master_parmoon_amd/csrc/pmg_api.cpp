@@ -293,8 +293,22 @@ void check_coarse(pmg_ctx *c, Coarse &C) {
     PMG_HIP_CHECK(hipEventElapsedTime(&ms, C.ev_start, C.ev_done));
     c->tm.coarse_lu.ms += ms;
     c->tm.coarse_lu.calls += 1;
-    c->tm.coarse_lu.bytes += double(C.n) * C.n * 16.0;
-    c->tm.coarse_lu.flops += (2.0 / 3.0) * double(C.n) * C.n * C.n;
+    if (C.block) {
+      int bi[2] = {0, 0};
+      PMG_HIP_CHECK(hipMemcpy(bi, C.cb.info, sizeof(bi), hipMemcpyDeviceToHost));
+      const double m = C.cb.m, q = C.cb.q;
+      c->tm.coarse_lu.bytes += 8.0 * (m * m * 2 + 2 * m * (m + 3 * q) + 3 * q * m * 3 + q * q * 3 + 2 * q * (3 * m + q));
+      c->tm.coarse_lu.flops += (2.0 / 3.0) * (m * m * m + q * q * q) + 2.0 * m * m * (m + 3 * q) +
+                               6.0 * q * q * m + 6.0 * q * m * m + 2.0 * q * q * q + 6.0 * q * q * m;
+      if (bi[0] || bi[1]) {
+        C.ready = false;
+        throw Fail{PMG_ERR_SINGULAR_COARSE, std::string("zero pivot in the coarse ") +
+                                                (bi[0] ? "velocity block" : "Schur complement") + " LU"};
+      }
+    } else {
+      c->tm.coarse_lu.bytes += double(C.n) * C.n * 16.0;
+      c->tm.coarse_lu.flops += (2.0 / 3.0) * double(C.n) * C.n * C.n;
+    }
     if (cinfo) {
       C.ready = false;
       throw Fail{PMG_ERR_SINGULAR_COARSE, "zero pivot " + std::to_string(cinfo) + " in the coarse LU"};
@@ -304,7 +318,13 @@ void check_coarse(pmg_ctx *c, Coarse &C) {
   int e = 0;
   PMG_HIP_CHECK(hipMemcpyAsync(&e, C.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
-  if (e) throw Fail{PMG_ERR_DEVICE, "coarse triangular sweep: wait bound exceeded"};
+  if (e) {
+    // reported once: cleared (outside any capture, the stream is idle here) so that later
+    // calls on the context are not failed by an old timeout
+    launch_fill<int>(C.err, 0, 1, c->stream);
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    throw Fail{PMG_ERR_DEVICE, "coarse triangular sweep: wait bound exceeded"};
+  }
 }
 
 // the streamed forms of new operator values (CSR arrays D.a, D.b, D.bt on the device):
@@ -544,6 +564,26 @@ void prolongate(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *uc, T
   launch_csr_spmv<T>(F.pp, 1, uc + 3L * C.n_udof, 0, d + 3L * F.n_udof, 0, c->stream);
 }
 
+// one coarse visit: the block form (explicit inverses of the blocks) or the LU sweeps
+void coarse_solve_any(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s) {
+  if (C.block) {
+    launch_coarse_rhs(op, f, C.cb.iota, C.cb.rr, 3 * C.cb.m + C.cb.q, C.cb.pin, s);
+    coarse_block_solve(C.cb, op, C.cb.rr, f, u, s);
+  } else {
+    coarse_lu_solve(C, op, f, u, s);
+  }
+}
+double coarse_visit_bytes(const Coarse &C) {
+  if (!C.block) return double(C.n) * C.n * 8.0;
+  const double m = C.cb.m, q = C.cb.q;
+  return 8.0 * (m * m + q * (3 * m + q) + 3 * m * q);
+}
+double coarse_visit_flops(const Coarse &C) {
+  if (!C.block) return 2.0 * double(C.n) * C.n;
+  const double m = C.cb.m, q = C.cb.q;
+  return 2.0 * (3 * m * m + q * (3 * m + q) + 3 * m * q);
+}
+
 // coarse solve in FP64 (vectors of type T converted at the boundary)
 template <typename T>
 void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
@@ -551,13 +591,12 @@ void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
     PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, c->coarse.ev_done, 0));
     c->coarse.pending = false;
   }
-  const double n = c->coarse.n;
-  Timed tm(c, &c->tm.coarse_solve, n * n * 8.0, 2.0 * n * n);
+  Timed tm(c, &c->tm.coarse_solve, coarse_visit_bytes(c->coarse), coarse_visit_flops(c->coarse));
   if constexpr (sizeof(T) == 8) {
-    coarse_lu_solve(c->coarse, D.op64(), f, u, c->stream);
+    coarse_solve_any(c->coarse, D.op64(), f, u, c->stream);
   } else {
     launch_f2d(D.aux0, f, D.n_dof, c->stream);
-    coarse_lu_solve(c->coarse, D.op64(), D.aux0, D.sol, c->stream);
+    coarse_solve_any(c->coarse, D.op64(), D.aux0, D.sol, c->stream);
     launch_d2f(u, D.sol, D.n_dof, c->stream);
   }
 }
@@ -572,10 +611,9 @@ void batch_coarse(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
       PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, b.co.ev_done, 0));
       b.co.pending = false;
     }
-    const double n = b.co.n;
-    Timed tm(c, &c->tm.coarse_solve, n * n * 8.0, 2.0 * n * n);
+    Timed tm(c, &c->tm.coarse_solve, coarse_visit_bytes(b.co), coarse_visit_flops(b.co));
     launch_gather_d<T>(b.cf, f, b.cidx, b.n_dof_coarse, c->stream);
-    coarse_lu_solve(b.co, b.cl.op64(), b.cf, b.cu, c->stream);
+    coarse_solve_any(b.co, b.cl.op64(), b.cf, b.cu, c->stream);
     launch_scatter_d<T>(u, b.cu, b.cidx, b.n_dof_coarse, c->stream);
   }
 }
@@ -750,13 +788,70 @@ void coarse_alloc(pmg_ctx *c, Coarse &C, int nr) {
   }
 }
 
-// assemble and factor the reduced coarse system of level D on the second stream
+// buffers of the block form for m velocity and q pressure unknowns (reallocated when they change)
+void coarse_block_alloc(pmg_ctx *c, CoarseBlock &B, int m, int q) {
+  if (B.A && (B.m != m || B.q != q)) {
+    for (void *p : {(void *)B.A, (void *)B.X, (void *)B.B, (void *)B.V, (void *)B.S, (void *)B.P1, (void *)B.rr,
+                    (void *)B.y, (void *)B.p, (void *)B.sigA, (void *)B.sigS, (void *)B.iota})
+      dfree(c, p);
+    B.A = nullptr;
+  }
+  if (B.A) return;
+  B.m = m;
+  B.q = q;
+  const size_t M = size_t(m), Q = size_t(q);
+  B.A = dalloc<double>(c, M * M);
+  B.X = dalloc<double>(c, M * (M + 3 * Q));
+  B.B = dalloc<double>(c, Q * 3 * M);
+  B.V = dalloc<double>(c, Q * 3 * M);
+  B.S = dalloc<double>(c, Q * Q);
+  B.P1 = dalloc<double>(c, Q * (3 * M + Q));
+  B.rr = dalloc<double>(c, 3 * M + Q);
+  B.y = dalloc<double>(c, 3 * M);
+  B.p = dalloc<double>(c, Q);
+  B.sigA = dalloc<int>(c, M);
+  B.sigS = dalloc<int>(c, Q);
+  B.iota = dalloc<int>(c, 3 * M + Q);
+  std::vector<int> id(3 * M + Q);
+  std::iota(id.begin(), id.end(), 0);
+  h2d(c, B.iota, id.data(), id.size());
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  if (!B.info) B.info = dalloc<int>(c, 2);
+}
+
+// assemble and factor the reduced coarse system of level D on the second stream: in the
+// block form when the smoother runs in mixed precision on the shared velocity block
+// (FP64 inverses of the blocks, each visit two matrix-vector phases), otherwise as one
+// dense LU whose factors equal dgetf2's (the FP64 form, parity with SolveExact)
 void start_coarse_factor(pmg_ctx *c, const DevLevel &D, Coarse &C, int pin) {
-  coarse_alloc(c, C, 3 * D.n_active + D.n_pdof - pin);
+  const int m = D.n_active, q = D.n_pdof - pin;
+  const bool block = mixed(c) && D.a_shared && coarse_block_fits(m, q);
+  if (block) {
+    if (!C.ev_start) {
+      PMG_HIP_CHECK(hipEventCreate(&C.ev_start));
+      PMG_HIP_CHECK(hipEventCreate(&C.ev_done));
+    }
+    if (!C.info) C.info = dalloc<int>(c, 1);
+    coarse_block_alloc(c, C.cb, m, q);
+    if (!C.block) drop_graph(c);   // the captured V-cycle holds the other form's kernels
+    C.block = true;
+    C.n = 3 * m + q;
+    C.cb.pin = pin;
+  } else {
+    if (C.block) drop_graph(c);
+    C.block = false;
+    coarse_alloc(c, C, 3 * D.n_active + D.n_pdof - pin);
+  }
   C.N = D.n_udof; C.na = D.n_active; C.np = D.n_pdof; C.pin = pin;
   PMG_HIP_CHECK(hipEventRecord(C.ev_start, c->stream2));
-  launch_coarse_build(D, C, c->stream2);
-  coarse_lu_factor(C, c->stream2);
+  if (block) {
+    PMG_HIP_CHECK(hipMemsetAsync(C.info, 0, sizeof(int), c->stream2));
+    launch_cb_build(D, C.cb, c->stream2);
+    coarse_block_factor(C.cb, c->stream2);
+  } else {
+    launch_coarse_build(D, C, c->stream2);
+    coarse_lu_factor(C, c->stream2);
+  }
   PMG_HIP_CHECK(hipEventRecord(C.ev_done, c->stream2));
   C.pending = true;
   C.info_pending = true;
@@ -1427,11 +1522,25 @@ int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
   if (!c || !p) return PMG_ERR_INVALID_ARGUMENT;
   std::string why;
   if (int st = check_params(p, why)) { c->err = why; return st; }
-  const bool prec_change = p->precision != c->prm.precision;
   const bool pin_change = p->internal_project_pressure != c->prm.internal_project_pressure;
-  if (prec_change && !c->L.empty()) { c->err = "precision is fixed once levels exist"; return PMG_ERR_UNSUPPORTED; }
+  // the factor and inverse buffers and the streamed operator forms are chosen when a level
+  // is added: these knobs are fixed from then on
+  if (!c->L.empty() && (p->precision != c->prm.precision || p->patch_inverse != c->prm.patch_inverse ||
+                        p->shared_block != c->prm.shared_block)) {
+    c->err = "precision, patch_inverse and shared_block are fixed once levels exist";
+    return PMG_ERR_UNSUPPORTED;
+  }
+  if (pin_change && c->batch) {
+    c->err = "the problems of a batch carry their own pressure projection";
+    return PMG_ERR_UNSUPPORTED;
+  }
   c->prm = *p;
-  if (pin_change) c->coarse.ready = false;
+  if (pin_change) {
+    // the coarse system pins (or frees) its pressure row: solves report PMG_ERR_NOT_READY
+    // until pmg_setup / pmg_refactor builds the coarse factors again
+    c->coarse.ready = false;
+    c->setup_done = false;
+  }
   drop_graph(c);
   return PMG_OK;
 }

@@ -187,6 +187,20 @@ struct DevLevel {
   }
 };
 
+// Block form of the reduced coarse system when the level has the shared velocity block
+// (mixed-precision forms): m active velocity dofs per component, q free pressure dofs.
+// Row-major FP64 arrays: A (m x m, then its LU), X = A^{-1} [I | BT1 BT2 BT3]
+// (m x (m + 3q): A^{-1} then W_b), B = [B1 B2 B3] (q x 3m), V = [B_b A^{-1}] (q x 3m), S
+// (q x q, then its LU), P1 = [-S^{-1} V | S^{-1}] (q x (3m + q)); work vectors rr (reduced
+// right-hand side, natural order), y (3m), p (q); sigA / sigS the row permutations of the
+// LUs, iota the identity permutation, info the two zero-pivot flags.
+struct CoarseBlock {
+  int m = 0, q = 0, pin = 0;
+  double *A = nullptr, *X = nullptr, *B = nullptr, *V = nullptr, *S = nullptr, *P1 = nullptr;
+  double *rr = nullptr, *y = nullptr, *p = nullptr;
+  int *sigA = nullptr, *sigS = nullptr, *iota = nullptr, *info = nullptr;
+};
+
 // Coarse level: the dense system of SolveExact with the Dirichlet rows (identity) and the
 // pinned pressure row eliminated; n = 3*na + np - pin unknowns remain.
 struct Coarse {
@@ -211,6 +225,10 @@ struct Coarse {
   hipEvent_t ev_start = nullptr, ev_done = nullptr;
   bool pending = false, info_pending = false;
   bool ready = false;
+  // block form (mixed precision with the shared velocity block): explicit FP64 inverses of
+  // the blocks instead of the LU of the whole reduced system
+  bool block = false;
+  CoarseBlock cb;
 };
 
 // reduction scratch: partial sums + a completion counter per reduction slot
@@ -290,6 +308,17 @@ void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long x_stride, T *y, l
 
 // dense coarse LU / solve
 void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s);
+// reduced right-hand side b[i] = (f_I - K_ID x_D)[sigma[i]] of the coarse system (n unknowns)
+void launch_coarse_rhs(const OpView<double> &op, const double *f, const int *sigma, double *b, int n, int pin,
+                       hipStream_t s);
+// block form (k_coarse.hip): sizes it supports, dense blocks from the level operator, set-up
+// (asynchronous; cb.info gets the zero-pivot flags of the two LUs) and one visit from the
+// reduced right-hand side rr into the level vector u
+bool coarse_block_fits(int m, int q);
+void launch_cb_build(const DevLevel &L, CoarseBlock &B, hipStream_t s);
+void coarse_block_factor(CoarseBlock &B, hipStream_t s);
+void coarse_block_solve(CoarseBlock &B, const OpView<double> &op, const double *rr, const double *f, double *u,
+                        hipStream_t s);
 void coarse_lu_factor(Coarse &C, hipStream_t s);   // asynchronous; C.info gets the dgetrf info
 void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s);
 

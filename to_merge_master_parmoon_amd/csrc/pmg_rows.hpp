@@ -21,6 +21,14 @@ __device__ inline T red16(T v) {
   return v;
 }
 
+// sum over the 64 lanes of a wavefront (every lane gets the total)
+template <typename T>
+__device__ inline T red64(T v) {
+  v += __shfl_xor(v, 32, 64);
+  v += __shfl_xor(v, 16, 64);
+  return red16(v);
+}
+
 // 16-byte vector loads of 4 consecutive entries (the addresses are 4-byte aligned; the
 // hardware runs in unaligned mode, so one dwordx4 per float4/int4 and two per double4)
 typedef float pmg_vf4 __attribute__((ext_vector_type(4)));
