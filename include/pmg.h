@@ -177,6 +177,16 @@ int  pmg_defect_nse4(pmg_ctx *ctx, int level, const double *x, const double *b, 
    called nsweeps times.  u is updated in place; f is the level right-hand side. */
 int  pmg_vanka_smooth(pmg_ctx *ctx, int level, double *u, const double *f, int nsweeps,
                       int mem);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the FP64
+   colored Vanka sweeps of pmg_vanka_smooth on device vectors u and f, with the local
+   residuals and the relaxation copy of u in a file-scope device array of 8 MiB per GPU
+   instead of library-allocated buffers; the call allocates and frees nothing.  Levels whose
+   scratch (max patches per color x 96 doubles, plus n_dof doubles when the global
+   relaxation sc_smooth_damp_factor_saddle != 1 is on) exceeds 8 MiB get
+   PMG_ERR_UNSUPPORTED; mixed-precision contexts get PMG_ERR_UNSUPPORTED.  Every call on a
+   GPU shares that GPU's array: calls that overlap on one GPU race. */
+int  pmg_vanka_smooth_noalloc(pmg_ctx *ctx, int level, double *u, const double *f, int nsweeps);
 /* Replaces: DefectRestriction for U (3 functions, MGComponents3D.C:996-1240) and P
    (MGComponents3D.C:711-993): f_coarse = R d_fine.  'level' is the fine level. */
 int  pmg_restrict_defect(pmg_ctx *ctx, int level, const double *d_fine, double *f_coarse,

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -1403,6 +1404,40 @@ struct HostIO {
 
 }  // namespace
 
+// ===================================================================== static scratch
+// pmg_vanka_smooth_noalloc keeps its scratch (the local residuals of one color and the
+// pre-sweep copy of u for the global relaxation) in this file-scope device array instead
+// of buffers the library allocates: 8 MiB per GPU, held until the process ends.  Host
+// code uses only the per-device address from hipGetSymbolAddress (scratch_dev), never the
+// array's name.  Every call on a GPU shares that GPU's array: calls that overlap on one
+// GPU (two contexts, two streams) race.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex scratch_mu;
+std::vector<double *> scratch_dev;   // device -> address of that device's copy (nullptr: failed)
+std::vector<int> scratch_state;      // device -> 0 not looked up, 1 ok, 2 failed
+
+// once per device, with the device current (called from pmg_ctx_create)
+void scratch_lookup(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (int(scratch_dev.size()) <= device) {
+    scratch_dev.resize(device + 1, nullptr);
+    scratch_state.resize(device + 1, 0);
+  }
+  if (scratch_state[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  scratch_dev[device] = e == hipSuccess ? static_cast<double *>(p) : nullptr;
+  scratch_state[device] = e == hipSuccess ? 1 : 2;
+}
+double *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  return device < int(scratch_dev.size()) ? scratch_dev[device] : nullptr;
+}
+}  // namespace
+
 // ===================================================================== C ABI
 extern "C" {
 
@@ -1475,6 +1510,7 @@ int pmg_ctx_create(int device, const pmg_params *params, pmg_ctx **out) {
     PMG_HIP_CHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) throw Fail{PMG_ERR_INVALID_ARGUMENT, "no such device"};
     PMG_HIP_CHECK(hipSetDevice(device));
+    scratch_lookup(device);
     PMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     PMG_HIP_CHECK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     PMG_HIP_CHECK(hipEventCreateWithFlags(&c->evs, hipEventDisableTiming));
@@ -1877,6 +1913,36 @@ int pmg_vanka_smooth(pmg_ctx *c, int level, double *u, const double *f, int nswe
     }
     io.out(0, u);
     io.sync();
+  });
+}
+
+int pmg_vanka_smooth_noalloc(pmg_ctx *c, int level, double *u, const double *f, int nsweeps) {
+  if (!c || !u || !f || nsweeps < 0) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    PMG_NEED_READY();
+    if (level < 1) throw Fail{PMG_ERR_INVALID_ARGUMENT, "the coarsest level is solved directly"};
+    if (mixed(c))
+      throw Fail{PMG_ERR_UNSUPPORTED, "pmg_vanka_smooth_noalloc runs the FP64 form only (the mixed forms convert "
+                                      "through level-owned FP32 vectors)"};
+    DevLevel &D = level_of(c, level);
+    const bool relax = std::fabs(1 - c->prm.sc_smooth_damp_factor_saddle) > 1e-3;
+    const size_t n_rloc = size_t(D.max_cells_per_color) * RLOC_STRIDE;
+    const size_t n_aux = relax ? size_t(D.n_dof) : 0;
+    const size_t off_aux = (n_rloc + 31) / 32 * 32;   // 256-byte aligned
+    if (off_aux + n_aux > NOALLOC_SCRATCH_DOUBLES)
+      throw Fail{PMG_ERR_UNSUPPORTED, "level " + std::to_string(level) + " needs " +
+                                          std::to_string((off_aux + n_aux) * 8) + " bytes of scratch (" +
+                                          std::to_string(D.max_cells_per_color) + " patches per color" +
+                                          (relax ? ", relaxation copy of u" : "") + "); the static scratch holds " +
+                                          std::to_string(NOALLOC_SCRATCH_DOUBLES * 8)};
+    double *base = scratch_of(c->device);
+    if (!base) throw Fail{PMG_ERR_DEVICE, "hipGetSymbolAddress of the static scratch failed on this device"};
+    double *rloc = base, *aux = base + off_aux;
+    // the array keeps the previous call's contents: the local residual slots start at zero
+    // (their padding is read by vector loads); aux is written by the sweep before it is read
+    launch_fill<double>(rloc, 0.0, long(n_rloc), c->stream);
+    for (int k = 0; k < nsweeps; ++k) vanka_sweep<double>(c, D, u, f, aux, rloc);
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
   });
 }
 
