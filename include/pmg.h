@@ -314,6 +314,7 @@ typedef struct {
   pmg_kstat patch_lu;      /* batched patch LU (set-up)                                   */
   pmg_kstat coarse_lu;     /* coarse dense LU (set-up)                                    */
   double sweep_ms; long long sweep_calls;   /* whole colored sweeps                       */
+  pmg_kstat vanka_res_finest;  /* the per-color local residuals of the finest level alone */
 } pmg_timers;
 int  pmg_timers_get(const pmg_ctx *ctx, pmg_timers *t);
 int  pmg_timers_reset(pmg_ctx *ctx);

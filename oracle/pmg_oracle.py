@@ -18,16 +18,41 @@ LIB = HERE / "libpmg_oracle.so"
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
 _lib = None
+_lib_path = None
+CXXFLAGS_PREBUILT = "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"   # oracle/Makefile
+CXXFLAGS_NATIVE = "-O3 -march=native -fopenmp -ffp-contract=off"
+
+
+def build_native(out_dir) -> str | None:
+    """Compile the 3D oracle with -march=native for the CPU this process runs on (the
+    CPU baseline; the prebuilt library targets x86-64-v3 so that it runs on any host).
+    Returns the path of the library, or None when the compiler fails."""
+    out = Path(out_dir) / "libpmg_oracle_native.so"
+    cmd = ["g++", *CXXFLAGS_NATIVE.split(), "-fPIC", "-std=c++17", f"-I{HERE.parent / 'include'}", "-shared",
+           str(HERE / "pmg_oracle.cpp"), str(HERE / "pmg_oracle2d.cpp"), "-o", str(out)]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    return str(out)
+
+
+def use_library(path: str):
+    """Bind the oracle functions from `path` (a build of oracle/pmg_oracle.cpp) instead of
+    the prebuilt library; must be called before the first oracle object is made."""
+    global _lib, _lib_path
+    _lib = None
+    _lib_path = path
 
 
 def _load():
     global _lib
     if _lib is not None:
         return _lib
-    if not LIB.exists():
+    if _lib_path is None and not LIB.exists():
         subprocess.run(["make", "-C", str(HERE)], check=True)
     from to_merge_master_parmoon_amd._abi import PmgLevelDesc, PmgParams
-    lib = C.CDLL(str(LIB))
+    lib = C.CDLL(str(_lib_path or LIB))
     sig = {
         "oracle_create": (C.c_int, [C.POINTER(PmgParams), C.POINTER(PmgLevelDesc), C.c_int, C.c_int,
                                     C.POINTER(C.c_void_p)]),

@@ -75,7 +75,8 @@ KSTATS = ["defect", "matvec", "vanka_res", "vanka_solve", "transfer", "coarse_so
 
 
 class PmgTimers(C.Structure):
-    _fields_ = [(k, PmgKstat) for k in KSTATS] + [("sweep_ms", C.c_double), ("sweep_calls", C.c_longlong)]
+    _fields_ = [(k, PmgKstat) for k in KSTATS] + [("sweep_ms", C.c_double), ("sweep_calls", C.c_longlong),
+                                                   ("vanka_res_finest", PmgKstat)]
 
 
 class PmgpConfig(C.Structure):

@@ -118,9 +118,10 @@ int guard(pmg_ctx *c, const std::function<void()> &f);
 // active while profiling is on, because it waits for the launch to finish.
 struct Timed {
   pmg_ctx *c;
-  pmg_kstat *k;
+  pmg_kstat *k, *k2;   // k2 (may be null): a second statistic the launch also counts in
   double bytes, flops;
-  Timed(pmg_ctx *c_, pmg_kstat *k_, double b = 0, double f = 0) : c(c_), k(k_), bytes(b), flops(f) {
+  Timed(pmg_ctx *c_, pmg_kstat *k_, double b = 0, double f = 0, pmg_kstat *k2_ = nullptr)
+      : c(c_), k(k_), k2(k2_), bytes(b), flops(f) {
     if (c->profiling) PMG_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
   }
   ~Timed() noexcept(false) {
@@ -129,10 +130,13 @@ struct Timed {
     PMG_HIP_CHECK(hipEventSynchronize(c->ev1));
     float ms = 0;
     PMG_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    k->ms += ms;
-    k->calls += 1;
-    k->bytes += bytes;
-    k->flops += flops;
+    for (pmg_kstat *q : {k, k2}) {
+      if (!q) continue;
+      q->ms += ms;
+      q->calls += 1;
+      q->bytes += bytes;
+      q->flops += flops;
+    }
   }
 };
 
@@ -508,7 +512,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
     }
     {
       const double rb = D.res_bytes[D.a_shared][vs == 8 ? 0 : 1][col] * part;
-      Timed tm(c, &c->tm.vanka_res, rb, 0.0);
+      Timed tm(c, &c->tm.vanka_res, rb, 0.0, &D == &c->L.back() ? &c->tm.vanka_res_finest : nullptr);
       launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, u, ua, f, rloc, c->stream);
     }
     Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));

@@ -229,6 +229,8 @@ class NSEMultiGrid:
             ks = getattr(t, k)
             out[k] = dict(ms=ks.ms, calls=ks.calls, bytes=ks.bytes, flops=ks.flops)
         out["sweep_ms"], out["sweep_calls"] = t.sweep_ms, t.sweep_calls
+        ks = t.vanka_res_finest
+        out["vanka_res_finest"] = dict(ms=ks.ms, calls=ks.calls, bytes=ks.bytes, flops=ks.flops)
         return out
 
     def reset_timers(self):
