@@ -56,7 +56,12 @@ typedef enum { PMG_MEM_HOST = 0, PMG_MEM_DEVICE = 1 } pmg_memspace;
  * pmg_params_default() returns the reference defaults.
  */
 typedef struct {
-  int    sc_smoother_saddle;                   /* 2 = colored cell Vanka (1: same, GS order) */
+  int    sc_smoother_saddle;                   /* 2: colored cell Vanka (_HYBRID order); 1: cell
+                                                  Vanka in the Gauss-Seidel order (cells in
+                                                  collection order, run as the level sets of
+                                                  pmg_sequential_schedule; any coloring in the
+                                                  level descriptors is then ignored).  Fixed once
+                                                  levels exist */
   int    sc_pre_smooth_saddle;                 /* nu1 */
   int    sc_post_smooth_saddle;                /* nu2 */
   double sc_smooth_damp_factor_saddle;         /* global post-sweep relaxation alpha      */
@@ -173,8 +178,9 @@ int  pmg_matvec(pmg_ctx *ctx, int level, const double *x, double *y, int mem);
 /* Replaces: Defect_NSE4 (src/FE/MGComponents3D.C:3210-3231): r = b - K x, IntoL20 on p. */
 int  pmg_defect_nse4(pmg_ctx *ctx, int level, const double *x, const double *b, double *r,
                      int mem);
-/* Replaces: TNSE_MGLevel4::CellVanka, colored variant (src/FE/NSE_MGLevel4.C:603-1649),
-   called nsweeps times.  u is updated in place; f is the level right-hand side. */
+/* Replaces: TNSE_MGLevel4::CellVanka (src/FE/NSE_MGLevel4.C:603-1649 colored, smoother 2;
+   :1653-2085 Gauss-Seidel, smoother 1), called nsweeps times.  u is updated in place; f
+   is the level right-hand side. */
 int  pmg_vanka_smooth(pmg_ctx *ctx, int level, double *u, const double *f, int nsweeps,
                       int mem);
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code
@@ -195,6 +201,15 @@ int  pmg_restrict_defect(pmg_ctx *ctx, int level, const double *d_fine, double *
    d_fine = P u_coarse.  'level' is the fine level. */
 int  pmg_prolongate(pmg_ctx *ctx, int level, const double *u_coarse, double *d_fine,
                     int mem);
+/* Replaces: RestrictFunction (src/FE/MGComponents3D.C:1509-1755) for the velocity (3
+   components): on every coarse cell the L2 projection onto Q2 of the fine velocity on its
+   children, averaged over the cells sharing a coarse dof.  'level' is the fine level;
+   the pressure part of u_coarse is left unchanged. */
+int  pmg_restrict_function(pmg_ctx *ctx, int level, const double *u_fine, double *u_coarse, int mem);
+/* Replaces: TNSE_MultiGrid::RestrictToAllGrids (src/FE/NSE_MultiGrid.C:391-431): the
+   finest level's velocity restricted level by level; u_levels[l] (l = 0 .. n_levels-2)
+   receives level l's velocity (vectors of that level's n_dof, pressure untouched). */
+int  pmg_restrict_to_all_grids(pmg_ctx *ctx, const double *u_finest, double *const *u_levels, int mem);
 /* Replaces: TNSE_MGLevel4::CorrectNodes (NSE_MGLevel4.C:542-601): u_D := 0. */
 int  pmg_correct_nodes(pmg_ctx *ctx, int level, double *u, int mem);
 /* Replaces: TNSE_MultiGrid::SetDirichletNodes (NSE_MultiGrid.C:1781-1802): u_D := f_D. */
@@ -287,6 +302,17 @@ int  pmg_batch_size(const pmg_ctx *ctx);
    hold n_cells+1 ints, cell_reorder n_cells ints. */
 int  pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_global,
                      int n_udof, int *n_colors, int *ptr_colors, int *cell_reorder);
+
+/* Replaces: the cell order of TNSE_MGLevel4::CellVanka, Gauss-Seidel variant
+   (src/FE/NSE_MGLevel4.C:1653-2085, smoother 1): the cells in level sets such that every
+   pair of cells whose patch rows read each other's dofs stays in collection order and
+   each set holds only cells that do not; a colored sweep over these sets is the
+   Gauss-Seidel sweep.  u_global / p_global: 27 / 4 dofs per cell; the CSR patterns of
+   A, BT and B as in pmg_level_desc.  ptr_sets must hold n_cells+1 ints. */
+int  pmg_sequential_schedule(int n_cells, const int *u_global, const int *p_global, int n_udof, int n_pdof,
+                             const int *a_rowptr, const int *a_kcol, const int *bt_rowptr, const int *bt_kcol,
+                             const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets,
+                             int *cell_reorder);
 
 /* ----------------------------------------------------------------------------------- */
 /* device memory helpers for callers without a device allocator                          */

@@ -550,6 +550,96 @@ void ProlongateSpace(const OLevel &C, const OLevel &F, int N_Functions, bool vel
   }
 }
 
+// TFEDatabase3D::GetRestrictionMatrix3D for Q2 hexahedra and the regular refinement
+// (FEDatabase3D.C:1791-1970): G = coarse basis mass on the reference cube, R_j[l2][l1] =
+// sum over the quadrature points of child j of w |det| phi_coarse_l1(X) phi_fine_l2(xi),
+// QQ_j[l1][l2] = (G^{-1} R_j[l2])[l1].  Quadrature: 3 x 3 x 3 Gauss points (exact for
+// these degree-4 products).  Child j covers the octant of the reference cube with centre
+// ((j & 1) ? 0.5 : -0.5, (j & 2) ? 0.5 : -0.5, (j & 4) ? 0.5 : -0.5), as the children of
+// the level descriptors are numbered (the prolongation tables fix the same octants).
+void Q2Values(double x, double y, double z, double *v) {
+  auto l = [](double t, double *o) { o[0] = 0.5 * t * (t - 1); o[1] = 1 - t * t; o[2] = 0.5 * t * (t + 1); };
+  double a[3], b[3], c[3];
+  l(x, a); l(y, b); l(z, c);
+  for (int k = 0; k < 27; ++k) v[k] = a[k % 3] * b[(k / 3) % 3] * c[k / 9];
+}
+const std::vector<double> &Q2RestrictionMatrices() {
+  static std::vector<double> qq;
+  if (!qq.empty()) return qq;
+  const double gp[3] = {-std::sqrt(0.6), 0.0, std::sqrt(0.6)}, gw[3] = {5.0 / 9, 8.0 / 9, 5.0 / 9};
+  std::vector<double> G(27 * 27, 0.0);
+  double v[27], vc[27];
+  for (int q = 0; q < 27; ++q) {
+    const double w = gw[q % 3] * gw[(q / 3) % 3] * gw[q / 9];
+    Q2Values(gp[q % 3], gp[(q / 3) % 3], gp[q / 9], v);
+    for (int i = 0; i < 27; ++i)
+      for (int j = 0; j < 27; ++j) G[i * 27 + j] += w * v[i] * v[j];
+  }
+  std::vector<int> piv(27);
+  std::vector<double> Gf(G);   // symmetric: row- and column-major agree
+  lu_factor(27, Gf.data(), piv.data());
+  qq.assign(8 * 27 * 27, 0.0);
+  for (int ch = 0; ch < 8; ++ch) {
+    const double s[3] = {(ch & 1) ? 0.5 : -0.5, (ch & 2) ? 0.5 : -0.5, (ch & 4) ? 0.5 : -0.5};
+    std::vector<double> R(27 * 27, 0.0);   // R[l2][l1]
+    for (int q = 0; q < 27; ++q) {
+      const double w = gw[q % 3] * gw[(q / 3) % 3] * gw[q / 9] / 8.0;   // |det| of the child map
+      const double xi = gp[q % 3], eta = gp[(q / 3) % 3], zeta = gp[q / 9];
+      Q2Values(xi, eta, zeta, v);                                                // fine basis
+      Q2Values(s[0] + 0.5 * xi, s[1] + 0.5 * eta, s[2] + 0.5 * zeta, vc);        // coarse basis
+      for (int l1 = 0; l1 < 27; ++l1)
+        for (int l2 = 0; l2 < 27; ++l2) R[l2 * 27 + l1] += w * vc[l1] * v[l2];
+    }
+    for (int l2 = 0; l2 < 27; ++l2) {
+      lu_solve(27, Gf.data(), piv.data(), &R[l2 * 27]);
+      for (int l1 = 0; l1 < 27; ++l1) qq[size_t(ch) * 729 + l1 * 27 + l2] = R[l2 * 27 + l1];
+    }
+  }
+  return qq;
+}
+
+// RestrictFunction, N_Functions = 3 (velocity): MGComponents3D.C:1509-1755.  Parents are
+// visited when the first of their children is met in fine collection order; a coarse
+// dof's value is the sum over its cells divided by the number of cells (aux).
+void RestrictFunctionU(const OLevel &C, const OLevel &F, const double *fine, double *coarse, double *aux) {
+  const std::vector<double> &QQ = Q2RestrictionMatrices();
+  const int Nc = C.N_UDOF, Nf = F.N_UDOF;
+  std::memset(aux, 0, sizeof(double) * Nc);
+  std::memset(coarse, 0, sizeof(double) * 3 * Nc);
+  std::vector<int> kids;
+  ChildrenOf(F, C.d.n_cells, kids);
+  std::vector<char> done(C.d.n_cells, 0);
+  double Val[27], Val2[3 * 27];
+  for (int i = 0; i < F.d.n_cells; i++) {
+    const int pc = F.d.parent[i];
+    if (done[pc]) continue;
+    done[pc] = 1;
+    std::memset(Val2, 0, sizeof(Val2));
+    for (int j = 0; j < F.d.n_children; j++) {
+      const int fc = kids[size_t(pc) * F.d.n_children + j];
+      const int *FineDOF = F.d.u_global + F.d.u_begin[fc];
+      for (int IFunct = 0; IFunct < 3; IFunct++) {
+        for (int l = 0; l < 27; l++) Val[l] = fine[IFunct * Nf + FineDOF[l]];
+        for (int k = 0; k < 27; k++) {
+          double s = 0;
+          for (int l = 0; l < 27; l++) s += QQ[size_t(j) * 729 + k * 27 + l] * Val[l];
+          Val2[IFunct * 27 + k] += s;
+        }
+      }
+    }
+    const int *CoarseDOF = C.d.u_global + C.d.u_begin[pc];
+    for (int IFunct = 0; IFunct < 3; IFunct++)
+      for (int k = 0; k < 27; k++) {
+        const int l = CoarseDOF[k];
+        aux[l] += 1;
+        coarse[IFunct * Nc + l] += Val2[IFunct * 27 + k];
+      }
+  }
+  Dscal(Nc, 1.0 / 3, aux);
+  for (int IFunct = 0; IFunct < 3; IFunct++)
+    for (int i = 0; i < Nc; i++) coarse[IFunct * Nc + i] /= aux[i];
+}
+
 // DefectRestriction: MGComponents3D.C:996-1240 (U, 3 functions) and :711-993 (P)
 void RestrictSpace(const OLevel &C, const OLevel &F, int N_Functions, bool velocity,
                    double *CoarseFunction, double *FineFunction, double *aux) {
@@ -956,6 +1046,9 @@ void oracle_prolongate(oracle_mg *mg, int fine_level, const double *u_coarse, do
   ProlongateSpace(C, F, 1, false, u_coarse + GEO_DIM * C.N_UDOF, d_fine + GEO_DIM * F.N_UDOF, aux);
 }
 int oracle_solve_exact(oracle_mg *mg, double *u, const double *f) { return SolveExact(*mg, mg->L[0], u, f); }
+void oracle_restrict_function(oracle_mg *mg, int fine_level, const double *u_fine, double *u_coarse, double *aux) {
+  RestrictFunctionU(mg->L[fine_level - 1], mg->L[fine_level], u_fine, u_coarse, aux);
+}
 int oracle_vcycle(oracle_mg *mg, const double *v, double *z) { return MGIterate(*mg, z, v); }
 int oracle_fgmres(oracle_mg *mg, double *sol, const double *rhs, int *iters, double *final_res) {
   return Fgmres(*mg, sol, rhs, iters, final_res);

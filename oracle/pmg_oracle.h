@@ -51,6 +51,9 @@ void oracle_update(oracle_mg *mg, int level, double *u, const double *d);
 void oracle_restrict_defect(oracle_mg *mg, int fine_level, double *d_fine, double *f_coarse, double *aux);
 void oracle_prolongate(oracle_mg *mg, int fine_level, const double *u_coarse, double *d_fine, double *aux);
 int  oracle_solve_exact(oracle_mg *mg, double *u, const double *f);
+/* RestrictFunction of the velocity (MGComponents3D.C:1509-1755), fine_level >= 1: the
+   velocity part of u_coarse is overwritten, aux holds n_udof(coarse) doubles */
+void oracle_restrict_function(oracle_mg *mg, int fine_level, const double *u_fine, double *u_coarse, double *aux);
 
 /* drivers */
 int  oracle_vcycle(oracle_mg *mg, const double *v, double *z);        /* TMultiGridIte::Iterate */

@@ -71,6 +71,7 @@ def _load():
         "oracle_restrict_defect": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
         "oracle_prolongate": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
         "oracle_solve_exact": (C.c_int, [C.c_void_p, _dp, _dp]),
+        "oracle_restrict_function": (None, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
         "oracle_vcycle": (C.c_int, [C.c_void_p, _dp, _dp]),
         "oracle_fgmres": (C.c_int, [C.c_void_p, _dp, _dp, _ip, _dp]),
         "oracle_fgmres_history": (C.c_int, [C.c_void_p, _dp, C.c_int]),
@@ -231,6 +232,13 @@ class Oracle:
         self._lib.oracle_prolongate(self._h, lvl, dp(np.ascontiguousarray(u_coarse)), dp(out), dp(aux))
         return out
 
+    def restrict_function(self, lvl, u_fine):
+        """RestrictFunction of the velocity from level lvl to lvl - 1 (pressure part zero)."""
+        out = np.zeros(self.n_dof(lvl - 1))
+        aux = np.empty(self.n_dof(lvl - 1))
+        self._lib.oracle_restrict_function(self._h, lvl, dp(np.ascontiguousarray(u_fine)), dp(out), dp(aux))
+        return out
+
     def solve_exact(self, f):
         u = np.empty_like(f)
         info = self._lib.oracle_solve_exact(self._h, dp(u), dp(f))
@@ -356,6 +364,13 @@ class Oracle2D:
         out = np.empty(self.n_dof(fine - 1))
         d = np.ascontiguousarray(d).copy()
         self._lib.oracle2d_restrict_defect(self._h, fine, dp(d), dp(out))
+        return out
+
+    def restrict_function(self, lvl, u_fine):
+        """RestrictFunction of the velocity from level lvl to lvl - 1 (pressure part zero)."""
+        out = np.zeros(self.n_dof(lvl - 1))
+        aux = np.empty(self.n_dof(lvl - 1))
+        self._lib.oracle_restrict_function(self._h, lvl, dp(np.ascontiguousarray(u_fine)), dp(out), dp(aux))
         return out
 
     def solve_exact(self, f):

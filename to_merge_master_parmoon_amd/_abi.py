@@ -116,6 +116,8 @@ PMG_FUNCS = {
     "pmg_vanka_smooth_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_restrict_defect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_prolongate": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "pmg_restrict_function": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "pmg_restrict_to_all_grids": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_int]),
     "pmg_correct_nodes": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "pmg_set_dirichlet": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_correct_defect": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
@@ -133,6 +135,8 @@ PMG_FUNCS = {
     "pmg_eval_velocity": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_assemble_oseen": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int]),
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
+    "pmg_sequential_schedule": (C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, C.c_int, c_int_p, c_int_p, c_int_p,
+                                          c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
     "pmg_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "pmg_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pmg_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),

@@ -229,6 +229,94 @@ void build_transfer(pmg_ctx *c, int nloc_f, int nloc_c, int n_fine_cells, const 
   PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
 }
 
+// RestrictFunction for Q2 (MGComponents3D.C:1509-1755): on every coarse cell the L2
+// projection onto Q2 of the fine function on its children, Val2 = sum_j QQr_j f_j with the
+// restriction matrix of child j (TFEDatabase3D::GetRestrictionMatrix3D,
+// FEDatabase3D.C:1791-1970: G^{-1} R_j, G the coarse basis' mass matrix on the reference
+// cube, R_j the mixed mass of coarse and fine basis on the child); a coarse dof's value is
+// the mean of its cells' values.  The fine basis on child j is the coarse basis restricted
+// there with the prolongation table P_j (fine functional k of coarse basis l), so
+// R_j = P_j^T G / 8 and QQr_j = G^{-1} P_j^T G / 8, with G = g x g x g, g the 1D quadratic
+// Lagrange mass on [-1, 1] (nodes -1, 0, 1, local index a + 3b + 9c).
+void build_function_restriction(pmg_ctx *c, int n_fine_cells, const int *parent, const int *child, const double *qq,
+                                const std::vector<int> &fglob, const std::vector<int> &cglob, int n_fdof, int n_cdof,
+                                Csr &RF) {
+  const double g1[3][3] = {{4 / 15.0, 2 / 15.0, -1 / 15.0}, {2 / 15.0, 16 / 15.0, 2 / 15.0},
+                           {-1 / 15.0, 2 / 15.0, 4 / 15.0}};
+  const double gi1[3][3] = {{9 / 2.0, -3 / 4.0, 3 / 2.0}, {-3 / 4.0, 9 / 8.0, -3 / 4.0}, {3 / 2.0, -3 / 4.0, 9 / 2.0}};
+  auto tens = [](const double (&m)[3][3], int i, int j) {
+    return m[i % 3][j % 3] * m[(i / 3) % 3][(j / 3) % 3] * m[i / 9][j / 9];
+  };
+  // QQr_j[k][l] = sum_{a, b} Ginv[k][a] P_j[b][a] G[b][l] / 8 (coarse k, fine l)
+  std::vector<double> qr(size_t(8) * 729, 0.0);
+  for (int ch = 0; ch < 8; ++ch) {
+    const double *P = qq + size_t(ch) * 729;
+    std::vector<double> t(729, 0.0);   // t[a][l] = sum_b P[b][a] G[b][l]
+    for (int a = 0; a < 27; ++a)
+      for (int l = 0; l < 27; ++l) {
+        double s = 0;
+        for (int b = 0; b < 27; ++b) s += P[b * 27 + a] * tens(g1, b, l);
+        t[a * 27 + l] = s;
+      }
+    for (int k = 0; k < 27; ++k)
+      for (int l = 0; l < 27; ++l) {
+        double s = 0;
+        for (int a = 0; a < 27; ++a) s += tens(gi1, k, a) * t[a * 27 + l];
+        qr[size_t(ch) * 729 + k * 27 + l] = s / 8.0;
+      }
+  }
+  struct Trip {
+    int r, c;
+    double v;
+  };
+  std::vector<Trip> t;
+  t.reserve(size_t(n_fine_cells) * 729);
+  std::vector<int> cells_of(n_cdof, 0);   // coarse cells containing each coarse dof
+  std::vector<char> seen;
+  std::vector<int> first_child;           // per coarse cell: counted once
+  int n_coarse_cells = 0;
+  for (int fc = 0; fc < n_fine_cells; ++fc) n_coarse_cells = std::max(n_coarse_cells, parent[fc] + 1);
+  seen.assign(n_coarse_cells, 0);
+  for (int fc = 0; fc < n_fine_cells; ++fc) {
+    const int pc = parent[fc], ch = child[fc];
+    if (pc < 0) continue;
+    const int *FineDOF = &fglob[size_t(fc) * NU_LOC];
+    const int *CoarseDOF = &cglob[size_t(pc) * NU_LOC];
+    if (!seen[pc]) {
+      seen[pc] = 1;
+      for (int k = 0; k < NU_LOC; ++k) cells_of[CoarseDOF[k]]++;
+    }
+    for (int k = 0; k < NU_LOC; ++k)
+      for (int l = 0; l < NU_LOC; ++l) {
+        const double v = qr[size_t(ch) * 729 + k * 27 + l];
+        if (v != 0.0) t.push_back({CoarseDOF[k], FineDOF[l], v});
+      }
+  }
+  std::sort(t.begin(), t.end(), [](const Trip &a, const Trip &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+  std::vector<int> rp(n_cdof + 1, 0), col;
+  std::vector<double> val;
+  for (size_t i = 0; i < t.size();) {
+    size_t j = i;
+    double s = 0;
+    while (j < t.size() && t[j].r == t[i].r && t[j].c == t[i].c) s += t[j++].v;
+    col.push_back(t[i].c);
+    val.push_back(s / cells_of[t[i].r]);
+    rp[t[i].r + 1]++;
+    i = j;
+  }
+  for (int i = 0; i < n_cdof; ++i) rp[i + 1] += rp[i];
+  RF.n_rows = n_cdof;
+  RF.n_cols = n_fdof;
+  RF.nnz = int(col.size());
+  RF.rowptr = dalloc<int>(c, rp.size());
+  RF.col = dalloc<int>(c, std::max<size_t>(1, col.size()));
+  RF.val = dalloc<double>(c, std::max<size_t>(1, val.size()));
+  h2d(c, RF.rowptr, rp.data(), rp.size());
+  h2d(c, RF.col, col.data(), col.size());
+  h2d(c, RF.val, val.data(), val.size());
+  PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
 }  // namespace
 
 namespace {
@@ -1566,8 +1654,8 @@ int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
   // the factor and inverse buffers and the streamed operator forms are chosen when a level
   // is added: these knobs are fixed from then on
   if (!c->L.empty() && (p->precision != c->prm.precision || p->patch_inverse != c->prm.patch_inverse ||
-                        p->shared_block != c->prm.shared_block)) {
-    c->err = "precision, patch_inverse and shared_block are fixed once levels exist";
+                        p->shared_block != c->prm.shared_block || p->sc_smoother_saddle != c->prm.sc_smoother_saddle)) {
+    c->err = "precision, patch_inverse, shared_block and sc_smoother_saddle are fixed once levels exist";
     return PMG_ERR_UNSUPPORTED;
   }
   if (pin_change && c->batch) {
@@ -1587,6 +1675,9 @@ int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
 
 int pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_global, int n_udof, int *n_colors,
                     int *ptr_colors, int *cell_reorder);
+int pmg_sequential_schedule(int n_cells, const int *u_global, const int *p_global, int n_udof, int n_pdof,
+                            const int *a_rowptr, const int *a_kcol, const int *bt_rowptr, const int *bt_kcol,
+                            const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets, int *cell_reorder);
 
 // merged: a level built by merge_hierarchies (cells without parent allowed, the color
 // lists may leave out the cells of problems solved directly on this level)
@@ -1636,10 +1727,15 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     }
     D.u_global = dalloc<int>(c, ug.size()); h2d(c, D.u_global, ug.data(), ug.size());
     D.p_global = dalloc<int>(c, pg.size()); h2d(c, D.p_global, pg.data(), pg.size());
-    // coloring (colorCell) - given or computed
+    // coloring (colorCell) - given or computed; smoother 1 (Gauss-Seidel order): the level
+    // sets of pmg_sequential_schedule, run as colors
     std::vector<int> ptr(D.n_cells + 1), reo(D.n_cells);
     int nc = 0;
-    if (d->ptr_colors && d->cell_reorder && (d->n_colors > 0 || merged)) {
+    if (c->prm.sc_smoother_saddle == 1) {
+      int st = pmg_sequential_schedule(D.n_cells, ug.data(), pg.data(), D.n_udof, D.n_pdof, d->a_rowptr, d->a_kcol,
+                                       d->bt_rowptr, d->bt_kcol, d->b_rowptr, d->b_kcol, &nc, ptr.data(), reo.data());
+      if (st) throw Fail{st, "Gauss-Seidel schedule failed"};
+    } else if (d->ptr_colors && d->cell_reorder && (d->n_colors > 0 || merged)) {
       nc = d->n_colors;
       if (nc < 0 || nc > D.n_cells) throw Fail{PMG_ERR_INVALID_ARGUMENT, "bad color count"};
       std::copy(d->ptr_colors, d->ptr_colors + nc + 1, ptr.begin());
@@ -1774,6 +1870,9 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
                      D.n_udof, C.n_udof, D.pu, D.ru);
       build_transfer(c, NP_LOC, NP_LOC, D.n_cells, d->parent, d->child, d->qq_p, pg, c->h_p_global[lvl - 1],
                      D.n_pdof, C.n_pdof, D.pp, D.rp);
+      if (!merged)
+        build_function_restriction(c, D.n_cells, d->parent, d->child, d->qq_u, ug, c->h_u_global[lvl - 1], D.n_udof,
+                                   C.n_udof, D.rf);
     }
     if (lvl > 0) {
       const DevLevel &C = c->L[lvl - 1];
@@ -1982,6 +2081,39 @@ int pmg_prolongate(pmg_ctx *c, int level, const double *u_coarse, double *d_fine
   });
 }
 
+int pmg_restrict_function(pmg_ctx *c, int level, const double *u_fine, double *u_coarse, int mem) {
+  if (!c || !u_fine || !u_coarse) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (c->batch) throw Fail{PMG_ERR_UNSUPPORTED, "RestrictFunction on a batch context"};
+    if (level < 1) throw Fail{PMG_ERR_INVALID_ARGUMENT, "level must be >= 1"};
+    DevLevel &F = level_of(c, level), &C = level_of(c, level - 1);
+    HostIO io(c, mem, F.n_dof);
+    const double *uf = io.in(0, u_fine);
+    double *uc = u_coarse;
+    if (mem == PMG_MEM_HOST) {
+      uc = c->stage[2];
+      PMG_HIP_CHECK(hipMemcpyAsync(uc, u_coarse, sizeof(double) * C.n_dof, hipMemcpyHostToDevice, c->stream));
+    }
+    Timed tm(c, &c->tm.transfer, double(F.rf.nnz) * 12.0 + 3.0 * 8.0 * (F.n_udof + C.n_udof), 0.0);
+    launch_csr_spmv<double>(F.rf, 3, uf, F.n_udof, uc, C.n_udof, c->stream);
+    if (mem == PMG_MEM_HOST)
+      PMG_HIP_CHECK(hipMemcpyAsync(u_coarse, uc, sizeof(double) * C.n_dof, hipMemcpyDeviceToHost, c->stream));
+    io.sync();
+  });
+}
+
+int pmg_restrict_to_all_grids(pmg_ctx *c, const double *u_finest, double *const *u_levels, int mem) {
+  if (!c || !u_finest || !u_levels) return PMG_ERR_INVALID_ARGUMENT;
+  const int L = int(c->L.size());
+  for (int l = 0; l + 1 < L; ++l)
+    if (!u_levels[l]) return PMG_ERR_INVALID_ARGUMENT;
+  for (int lev = L - 1; lev > 0; --lev) {
+    const double *src = lev == L - 1 ? u_finest : u_levels[lev];
+    if (int st = pmg_restrict_function(c, lev, src, u_levels[lev - 1], mem)) return st;
+  }
+  return PMG_OK;
+}
+
 int pmg_correct_nodes(pmg_ctx *c, int level, double *u, int mem) {
   if (!c || !u) return PMG_ERR_INVALID_ARGUMENT;
   return guard(c, [&] {
@@ -2152,6 +2284,8 @@ int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg
   c->batch = true;
   const int rc = guard(c, [&] {
     if (c->prm.sc_minit != 0) throw Fail{PMG_ERR_UNSUPPORTED, "a batch needs sc_minit = 0"};
+    if (c->prm.sc_smoother_saddle != 2)
+      throw Fail{PMG_ERR_UNSUPPORTED, "a batch runs the colored cell Vanka (smoother 2)"};
     MergedHierarchy M;
     try {
       merge_hierarchies(n_inst, inst, M);

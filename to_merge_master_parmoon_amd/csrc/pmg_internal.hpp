@@ -147,6 +147,7 @@ struct DevLevel {
   bool factors_ready = false;
   // transfers to the next coarser level (this level = fine)
   Csr pu, ru, pp, rp;                // P_U (fine rows), R_U = P_U^T, P_P, R_P
+  Csr rf;                            // RestrictFunction of the velocity: coarse rows, fine columns
   // MG vectors: sol, rhs, aux0 (defect), aux1 (aux) as the reference level owns them
   double *sol = nullptr, *rhs = nullptr, *aux0 = nullptr, *aux1 = nullptr;
   float *sol32 = nullptr, *rhs32 = nullptr, *aux0_32 = nullptr, *aux1_32 = nullptr;
