@@ -274,6 +274,55 @@ def run_reassembly(args, log):
                        "host_generator_threads": os.cpu_count()}}
 
 
+def run_channel(args, log):
+    """The time-dependent channel workload: [0,2.5] x [0,0.41]^2, 96 x 16 x 16 hexahedra
+    (4 levels), Re = 100 parabolic inflow with a seeded 1 % perturbation, Crank-Nicolson
+    dt = 0.01 for `--channel-steps` steps with 3 fixed-point Oseen iterations each, state in
+    device memory (to_merge_master_parmoon_amd.timeloop); the measured precision beside
+    the FP64 loop, whose solution at every step gives the error."""
+    from to_merge_master_parmoon_amd import problem
+    from to_merge_master_parmoon_amd.timeloop import ThetaTimeLoop, perturbed_inflow
+    prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]
+    prob = problem.channel()
+    g = perturbed_inflow(prob)
+    nu, dt = 0.0041, 0.01
+
+    def loop(p_):
+        sp_ = solver_params(p_)
+        sp_.internal_project_pressure = 0   # natural outflow: the pressure is determined
+        tl = ThetaTimeLoop(prob, sp_, nu, dt, dirichlet=g)
+        tl.step()   # warm-up step (graph capture, first-touch), then restart from rest
+        tl.close()
+        tl = ThetaTimeLoop(prob, sp_, nu, dt, dirichlet=g)
+        out = []
+        for k in range(args.channel_steps):
+            t, its = tl.step()
+            out.append((t, its, tl.solution()))
+            log(f"channel {['fp64', 'fp32', 'bf16'][p_]} step {k}: {t * 1e3:.1f} ms, FGMRES iterations {its}")
+        tl.close()
+        return out
+
+    meas = loop(prec)
+    ref = loop(0) if prec != 0 else meas
+    n_u3 = 3 * prob.descs[-1].n_udof
+    steps = []
+    for k, ((t, its, x), (t64, its64, x64)) in enumerate(zip(meas, ref)):
+        du = np.linalg.norm(x[:n_u3] - x64[:n_u3]) / np.linalg.norm(x64[:n_u3])
+        dp = np.linalg.norm(x[n_u3:] - x64[n_u3:]) / max(np.linalg.norm(x64[n_u3:]), 1e-300)
+        steps.append({"step": k, "s": round(t, 5), "fp64_s": round(t64, 5), "iterations": its,
+                      "fp64_iterations": its64, "rel_err_vs_fp64": {"velocity": float(du), "pressure": float(dp)}})
+    mean = float(np.mean([st["s"] for st in steps]))
+    mean64 = float(np.mean([st["fp64_s"] for st in steps]))
+    cfg = {"workload": "tnse3d_channel_96x16x16_q2p1disc_cn_dt0.01_picard3", "dofs": prob.sizes(-1)["n_dof"],
+           "levels": prob.n_levels, "re": 100, "nu": nu, "steps": args.channel_steps, "picard_per_step": 3,
+           "precision": args.precision, "fp64_step_s": round(mean64, 5), "speedup_vs_fp64": round(mean64 / mean, 3),
+           "max_rel_err_vs_fp64": max(max(st["rel_err_vs_fp64"].values()) for st in steps), "per_step": steps}
+    return {"metric": "TNSE3D channel 96x16x16 Crank-Nicolson step s (3 fixed-point Oseen solves)",
+            "value": round(mean, 5), "unit": "s", "n_gpus": 1, "steps": args.channel_steps, "warmup": 1,
+            "ms_per_step": round(mean * 1e3, 3), "higher_is_better": False, "scaling": "strong",
+            "vs_baseline": None, "dtype": args.precision, "data": "synthetic", "config": cfg}
+
+
 def measured_traffic(kernel: str, precision: str, n: int):
     """HBM bytes per launch of `kernel` on the finest level from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy kernel)."""
@@ -426,9 +475,11 @@ def main():
     ap.add_argument("--no-nine-blocks", action="store_true", help="skip the nine-block timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload's fields")
     ap.add_argument("--quiet", action="store_true")
-    ap.add_argument("--workload", choices=["oseen32", "batched", "reassembly"], default="oseen32",
+    ap.add_argument("--workload", choices=["oseen32", "batched", "reassembly", "channel"], default="oseen32",
                     help="batched: the independent-problems workload; reassembly: one fixed-point step's "
-                         "device re-assembly and set-up (instead of the headline solve)")
+                         "device re-assembly and set-up; channel: the Crank-Nicolson channel time loop "
+                         "(instead of the headline solve)")
+    ap.add_argument("--channel-steps", type=int, default=20)
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--batch-maxit", type=int, default=100)
     ap.add_argument("--no-graph", action="store_true", help="launch every V-cycle kernel by kernel")
@@ -451,6 +502,9 @@ def main():
         return
     if args.workload == "reassembly":
         print(json.dumps(run_reassembly(args, log)), flush=True)
+        return
+    if args.workload == "channel":
+        print(json.dumps(run_channel(args, log)), flush=True)
         return
     from to_merge_master_parmoon_amd import problem
     prec = {"fp64": 0, "fp32": 1, "bf16": 2}[args.precision]

@@ -262,6 +262,17 @@ int  pmg_eval_velocity(pmg_ctx *ctx, int level, const double *u, double *b_quad,
 int  pmg_assemble_oseen(pmg_ctx *ctx, int level, const double *b_quad, double nu, double sigma,
                         int mem);
 
+/* Replaces: the right-hand side of TSystemTNSE3D::AssembleSystMat
+   (src/System/SystemTNSE3D.C:1118-1190) for a time step of the theta scheme, scaled by
+   1 / (theta tau) (sigma = 1 / (theta tau)) and without source terms:
+   rhs = sigma M u_old - A(b) u_old on the active velocity rows, with A(b) the element
+   matrices of pmg_assemble_oseen without the mass term (b at the Gauss points as for
+   pmg_eval_velocity, usually the velocity u_old itself), matrix-free; rhs = g on the
+   Dirichlet rows; 0 on the pressure rows.  With sigma M + A(b) as the system matrix this
+   is the Crank-Nicolson step for theta = 1/2.  Needs pmg_level_geometry. */
+int  pmg_theta_rhs(pmg_ctx *ctx, int level, const double *u_old, const double *b_quad, double nu, double sigma,
+                   const double *g, double *rhs, int mem);
+
 /* ----------------------------------------------------------------------------------- */
 /* batched solves of independent problems                                                */
 /* ----------------------------------------------------------------------------------- */

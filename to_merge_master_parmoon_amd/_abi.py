@@ -134,6 +134,8 @@ PMG_FUNCS = {
     "pmg_level_geometry": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "pmg_eval_velocity": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "pmg_assemble_oseen": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_int]),
+    "pmg_theta_rhs": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                C.c_void_p, C.c_int]),
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_sequential_schedule": (C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, C.c_int, c_int_p, c_int_p, c_int_p,
                                           c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),

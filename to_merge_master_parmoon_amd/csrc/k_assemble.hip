@@ -47,20 +47,18 @@ __global__ void k_zero_active(double *__restrict__ a, long nnz, long n_act_entri
   }
 }
 
-__global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ cells, int ncells,
-                                                        const double *__restrict__ geom,
-                                                        const double *__restrict__ bq,
-                                                        const int *__restrict__ u_global,
-                                                        const int *__restrict__ pmap,
-                                                        const int *__restrict__ a_rowptr, int n_active, long nnz,
-                                                        double *__restrict__ a, double nu, double sigma) {
-  __shared__ double X[8][3], Bc[27][3], Ji[27][9], W[27];
-  __shared__ double Vq[27][27], Gx[27][27], Gy[27][27], Gz[27][27];
-  if (int(blockIdx.x) >= ncells) return;
-  const int cell = cells[blockIdx.x];
+// Geometry and basis data of one cell in LDS: weights w_q |det J| (W), values Vq[q][i] and
+// physical gradients G*[q][i] of the 27 Q2 basis functions at the 27 Gauss points, and the
+// convection at the points (Bc).  Called by all 256 threads; ends with a barrier.
+struct CellData {
+  double X[8][3], Bc[27][3], Ji[27][9], W[27];
+  double Vq[27][27], Gx[27][27], Gy[27][27], Gz[27][27];
+};
+
+__device__ void cell_setup(CellData &D, int cell, const double *__restrict__ geom, const double *__restrict__ bq) {
   const int t = threadIdx.x;
-  if (t < 24) X[t / 3][t % 3] = geom[long(cell) * 24 + t];
-  if (t < 81) Bc[t / 3][t % 3] = bq[long(cell) * 81 + t];
+  if (t < 24) D.X[t / 3][t % 3] = geom[long(cell) * 24 + t];
+  if (t < 81) D.Bc[t / 3][t % 3] = bq[long(cell) * 81 + t];
   __syncthreads();
   if (t < 27) {
     const double xi = kGP[t % 3], eta = kGP[(t / 3) % 3], zeta = kGP[t / 9];
@@ -71,7 +69,7 @@ __global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ 
           const double sx = aa ? 1 : -1, sy = b ? 1 : -1, sz = c ? 1 : -1;
           const double nx = 0.5 * (1 + sx * xi), ny = 0.5 * (1 + sy * eta), nz = 0.5 * (1 + sz * zeta);
           const double dNx = 0.5 * sx * ny * nz, dNy = 0.5 * sy * nx * nz, dNz = 0.5 * sz * nx * ny;
-          const double *P = X[aa + 2 * b + 4 * c];
+          const double *P = D.X[aa + 2 * b + 4 * c];
           for (int r = 0; r < 3; ++r) {
             J[r][0] += dNx * P[r];
             J[r][1] += dNy * P[r];
@@ -82,7 +80,7 @@ __global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ 
                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
     const double id = 1.0 / det;
-    double *I = Ji[t];
+    double *I = D.Ji[t];
     I[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) * id;
     I[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
     I[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
@@ -92,7 +90,7 @@ __global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ 
     I[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) * id;
     I[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
     I[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-    W[t] = kGW[t % 3] * kGW[(t / 3) % 3] * kGW[t / 9] * fabs(det);
+    D.W[t] = kGW[t % 3] * kGW[(t / 3) % 3] * kGW[t / 9] * fabs(det);
   }
   __syncthreads();
   // physical gradients: J^{-T} times the reference gradient
@@ -100,26 +98,45 @@ __global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ 
     const int q = idx / 27, i = idx - 27 * q;
     double v, dx, dy, dz;
     q2_at(q, i, v, dx, dy, dz);
-    const double *I = Ji[q];
-    Vq[q][i] = v;
-    Gx[q][i] = I[0] * dx + I[3] * dy + I[6] * dz;
-    Gy[q][i] = I[1] * dx + I[4] * dy + I[7] * dz;
-    Gz[q][i] = I[2] * dx + I[5] * dy + I[8] * dz;
+    const double *I = D.Ji[q];
+    D.Vq[q][i] = v;
+    D.Gx[q][i] = I[0] * dx + I[3] * dy + I[6] * dz;
+    D.Gy[q][i] = I[1] * dx + I[4] * dy + I[7] * dz;
+    D.Gz[q][i] = I[2] * dx + I[5] * dy + I[8] * dz;
   }
   __syncthreads();
+}
+
+// element entry (test i, ansatz j) of NSType4Galerkin3D with the mass term sigma
+__device__ inline double element_entry(const CellData &D, int i, int j, double nu, double sigma) {
+  double s = 0.0;
+  for (int q = 0; q < 27; ++q) {
+    double val = nu * (D.Gx[q][i] * D.Gx[q][j] + D.Gy[q][i] * D.Gy[q][j] + D.Gz[q][i] * D.Gz[q][j]);
+    val += (D.Bc[q][0] * D.Gx[q][j] + D.Bc[q][1] * D.Gy[q][j] + D.Bc[q][2] * D.Gz[q][j]) * D.Vq[q][i];
+    if (sigma != 0.0) val += sigma * D.Vq[q][j] * D.Vq[q][i];
+    s += D.W[q] * val;
+  }
+  return s;
+}
+
+__global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ cells, int ncells,
+                                                        const double *__restrict__ geom,
+                                                        const double *__restrict__ bq,
+                                                        const int *__restrict__ u_global,
+                                                        const int *__restrict__ pmap,
+                                                        const int *__restrict__ a_rowptr, int n_active, long nnz,
+                                                        double *__restrict__ a, double nu, double sigma) {
+  __shared__ CellData D;
+  if (int(blockIdx.x) >= ncells) return;
+  const int cell = cells[blockIdx.x];
+  cell_setup(D, cell, geom, bq);
   const int *ug = u_global + long(cell) * NU_LOC;
   const int *mp = pmap + long(cell) * PM_STRIDE;
-  for (int idx = t; idx < 27 * 27; idx += 256) {
+  for (int idx = threadIdx.x; idx < 27 * 27; idx += 256) {
     const int i = idx / 27, j = idx - 27 * i;
     const int row = ug[i];
     if (row >= n_active) continue;   // Dirichlet rows keep their identity
-    double s = 0.0;
-    for (int q = 0; q < 27; ++q) {
-      double val = nu * (Gx[q][i] * Gx[q][j] + Gy[q][i] * Gy[q][j] + Gz[q][i] * Gz[q][j]);
-      val += (Bc[q][0] * Gx[q][j] + Bc[q][1] * Gy[q][j] + Bc[q][2] * Gz[q][j]) * Vq[q][i];
-      if (sigma != 0.0) val += sigma * Vq[q][j] * Vq[q][i];
-      s += W[q] * val;
-    }
+    const double s = element_entry(D, i, j, nu, sigma);
     // entry (row, col j) in the CSR: the gather map holds its row-blocked offset
     // 9 k0 + (k - k0) of block 0
     const int off = mp[PM_A + i * NU_LOC + j];
@@ -128,6 +145,51 @@ __global__ __launch_bounds__(256) void k_assemble_oseen(const int *__restrict__ 
     a[k] += s;
     a[4 * nnz + k] += s;
     a[8 * nnz + k] += s;
+  }
+}
+
+// rhs_c(row i) -= sum_j E_ij(nu, sigma') u_c(j) for the active rows of one color's cells:
+// with sigma' = -sigma, sigma M u - A u of the theta scheme's right-hand side
+__global__ __launch_bounds__(256) void k_theta_rhs(const int *__restrict__ cells, int ncells,
+                                                   const double *__restrict__ geom, const double *__restrict__ bq,
+                                                   const int *__restrict__ u_global, int n_udof, int n_active,
+                                                   const double *__restrict__ u, double *__restrict__ rhs, double nu,
+                                                   double sigma_neg) {
+  __shared__ CellData D;
+  __shared__ double E[27][28];
+  if (int(blockIdx.x) >= ncells) return;
+  const int cell = cells[blockIdx.x];
+  cell_setup(D, cell, geom, bq);
+  for (int idx = threadIdx.x; idx < 27 * 27; idx += 256) {
+    const int i = idx / 27, j = idx - 27 * i;
+    E[i][j] = element_entry(D, i, j, nu, sigma_neg);
+  }
+  __syncthreads();
+  const int *ug = u_global + long(cell) * NU_LOC;
+  const int t = threadIdx.x;
+  if (t < 81) {
+    const int i = t / 3, c = t - 3 * i;
+    const int row = ug[i];
+    if (row < n_active) {
+      double s = 0.0;
+      for (int j = 0; j < 27; ++j) s += E[i][j] * u[long(c) * n_udof + ug[j]];
+      rhs[long(c) * n_udof + row] -= s;
+    }
+  }
+}
+
+// theta-scheme right-hand side outside the active velocity rows: Dirichlet values from g
+// (velocity rows >= n_active), zero pressure rows; the active velocity rows start at 0
+__global__ void k_theta_rhs_init(double *__restrict__ rhs, const double *__restrict__ g, int n_udof, int n_active,
+                                 int n_pdof) {
+  const long i = blockIdx.x * long(blockDim.x) + threadIdx.x;
+  const long n = 3L * n_udof + n_pdof;
+  if (i >= n) return;
+  if (i < 3L * n_udof) {
+    const int r = int(i % n_udof);
+    rhs[i] = r < n_active ? 0.0 : g[i];
+  } else {
+    rhs[i] = 0.0;
   }
 }
 
@@ -167,6 +229,20 @@ void launch_assemble_oseen(const DevLevel &L, const double *geom, const double *
     if (nc <= 0) continue;
     k_assemble_oseen<<<nc, 256, 0, s>>>(L.cell_reorder + p0, nc, geom, bq, L.u_global, L.pmap, L.a_rowptr, L.n_active,
                                         L.a_nnz, L.a, nu, sigma);
+    PMG_HIP_CHECK(hipGetLastError());
+  }
+}
+
+void launch_theta_rhs(const DevLevel &L, const double *geom, const double *bq, double nu, double sigma,
+                      const double *u, const double *g, double *rhs, hipStream_t s) {
+  const long n = 3L * L.n_udof + L.n_pdof;
+  k_theta_rhs_init<<<int((n + 255) / 256), 256, 0, s>>>(rhs, g, L.n_udof, L.n_active, L.n_pdof);
+  PMG_HIP_CHECK(hipGetLastError());
+  for (int col = 0; col + 1 < int(L.ptr_colors.size()); ++col) {
+    const int p0 = L.ptr_colors[col], nc = L.ptr_colors[col + 1] - p0;
+    if (nc <= 0) continue;
+    k_theta_rhs<<<nc, 256, 0, s>>>(L.cell_reorder + p0, nc, geom, bq, L.u_global, L.n_udof, L.n_active, u, rhs, nu,
+                                   -sigma);
     PMG_HIP_CHECK(hipGetLastError());
   }
 }

@@ -2275,6 +2275,35 @@ int pmg_assemble_oseen(pmg_ctx *c, int level, const double *b_quad, double nu, d
   });
 }
 
+int pmg_theta_rhs(pmg_ctx *c, int level, const double *u_old, const double *b_quad, double nu, double sigma,
+                  const double *g, double *rhs, int mem) {
+  if (!c || !u_old || !b_quad || !g || !rhs) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    if (c->batch) throw Fail{PMG_ERR_UNSUPPORTED, "theta right-hand side on a batch context"};
+    DevLevel &D = level_of(c, level);
+    if (!D.geom) throw Fail{PMG_ERR_NOT_READY, "pmg_level_geometry not called for this level"};
+    const long nq = long(D.n_cells) * 81;
+    HostIO io(c, mem, std::max<long>(D.n_dof, nq));
+    const double *du = u_old, *dg = g, *db = b_quad;
+    double *dr = rhs;
+    if (mem == PMG_MEM_HOST) {
+      // staging: u, g and the quadrature values in the three stage buffers (each holds
+      // max(n_dof, 81 n_cells) doubles), the result in the level's defect vector
+      PMG_HIP_CHECK(hipMemcpyAsync(c->stage[0], u_old, sizeof(double) * D.n_dof, hipMemcpyHostToDevice, c->stream));
+      PMG_HIP_CHECK(hipMemcpyAsync(c->stage[1], g, sizeof(double) * D.n_dof, hipMemcpyHostToDevice, c->stream));
+      PMG_HIP_CHECK(hipMemcpyAsync(c->stage[2], b_quad, sizeof(double) * nq, hipMemcpyHostToDevice, c->stream));
+      du = c->stage[0];
+      dg = c->stage[1];
+      db = c->stage[2];
+      dr = D.aux0;
+    }
+    launch_theta_rhs(D, D.geom, db, nu, sigma, du, dg, dr, c->stream);
+    if (mem == PMG_MEM_HOST)
+      PMG_HIP_CHECK(hipMemcpyAsync(rhs, dr, sizeof(double) * D.n_dof, hipMemcpyDeviceToHost, c->stream));
+    io.sync();
+  });
+}
+
 int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg_instance_desc *inst,
                      pmg_ctx **out) {
   if (!out || n_inst <= 0 || !inst) return PMG_ERR_INVALID_ARGUMENT;

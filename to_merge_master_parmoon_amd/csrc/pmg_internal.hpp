@@ -340,6 +340,10 @@ void launch_scal(double *y, const double *x, double a, long n, hipStream_t s);  
 void launch_assemble_oseen(const DevLevel &L, const double *geom, const double *bq, double nu, double sigma,
                            hipStream_t s);
 void launch_eval_velocity(const DevLevel &L, const double *u, double *bq, hipStream_t s);
+// theta-scheme right-hand side: rhs = sigma M u - A(b) u on the active velocity rows (A: the
+// same element matrices without the mass term), g on the Dirichlet rows, 0 on the pressure
+void launch_theta_rhs(const DevLevel &L, const double *geom, const double *bq, double nu, double sigma,
+                      const double *u, const double *g, double *rhs, hipStream_t s);
 
 // batched path (k_batch.hip): IntoL20 per (offset, length) pressure block; copies between a
 // problem's numbering and the merged numbering (dst[i] = src[idx[i]], dst[idx[i]] = src[i]);
