@@ -305,6 +305,28 @@ int  pmg_batch_fgmres_solve(pmg_ctx *ctx, double *const *sol, const double *cons
 int  pmg_batch_size(const pmg_ctx *ctx);
 
 /* ----------------------------------------------------------------------------------- */
+/* level dumps (host code)                                                               */
+/* ----------------------------------------------------------------------------------- */
+/* The on-disk form of one level: a directory with manifest.txt ("key value" lines:
+   n_udof n_active hanging_bound n_dirichlet n_pdof a_nnz b_nnz bt_nnz n_cells n_ulocal
+   n_plocal n_children n_colors) and one raw native-endian file per array, doubles as
+   ParMooN's SaveData writes them (src/General/MainUtilities.C:3360-3440) and int arrays as
+   32-bit ints: a_rowptr.i32 a_kcol.i32 A11.f64 .. A33.f64 (the TSquareStructure3D arrays
+   and Entries, include/FE/Matrix.h:91-129), b_rowptr.i32 b_kcol.i32 B1.f64 B2.f64 B3.f64,
+   bt_rowptr.i32 bt_kcol.i32 B1T.f64 B2T.f64 B3T.f64 (TStructure3D), u_begin.i32 (BeginIndex,
+   n_cells + 1 entries) u_global.i32 (GlobalNumbers), p_begin.i32 p_global.i32, and on a
+   refined level parent.i32 child.i32 qq_u.f64 qq_p.f64, with a coloring ptr_colors.i32
+   cell_reorder.i32.  pmg_level_read fills *out with arrays owned by *handle (release
+   with pmg_level_read_free after pmg_level_add); malformed dumps give
+   PMG_ERR_INVALID_ARGUMENT.  No reference counterpart beyond SaveData / ReadData. */
+int  pmg_level_read(const char *dir, pmg_level_desc *out, void **handle);
+int  pmg_level_read_free(void *handle);
+int  pmg_level_write(const char *dir, const pmg_level_desc *desc);
+/* Replaces: ReadData / SaveData (src/General/MainUtilities.C:3390-3440) for one vector. */
+int  pmg_vector_read(const char *path, double *x, long n);
+int  pmg_vector_write(const char *path, const double *x, long n);
+
+/* ----------------------------------------------------------------------------------- */
 /* coloring                                                                              */
 /* ----------------------------------------------------------------------------------- */
 /* Replaces: TParFEMapper3D::colorCell (src/Parallel/ParFEMapper3D.C:750-1084): cells are

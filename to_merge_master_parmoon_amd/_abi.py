@@ -139,6 +139,11 @@ PMG_FUNCS = {
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_sequential_schedule": (C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, C.c_int, c_int_p, c_int_p, c_int_p,
                                           c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
+    "pmg_level_read": (C.c_int, [C.c_char_p, C.POINTER(PmgLevelDesc), C.POINTER(C.c_void_p)]),
+    "pmg_level_read_free": (C.c_int, [C.c_void_p]),
+    "pmg_level_write": (C.c_int, [C.c_char_p, C.POINTER(PmgLevelDesc)]),
+    "pmg_vector_read": (C.c_int, [C.c_char_p, c_double_p, C.c_long]),
+    "pmg_vector_write": (C.c_int, [C.c_char_p, c_double_p, C.c_long]),
     "pmg_device_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "pmg_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pmg_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
