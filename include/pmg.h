@@ -60,8 +60,13 @@ typedef struct {
                                                   Vanka in the Gauss-Seidel order (cells in
                                                   collection order, run as the level sets of
                                                   pmg_sequential_schedule; any coloring in the
-                                                  level descriptors is then ignored).  Fixed once
-                                                  levels exist */
+                                                  level descriptors is then ignored); 3 / 4:
+                                                  nodal Vanka (patch = one pressure dof and the
+                                                  velocity dofs of its B row, pressure dofs in
+                                                  order as the level sets of pmg_nodal_schedule)
+                                                  with the diagonal (3) or the full (4) local
+                                                  system; FP64 and single problems only.  Fixed
+                                                  once levels exist */
   int    sc_pre_smooth_saddle;                 /* nu1 */
   int    sc_post_smooth_saddle;                /* nu2 */
   double sc_smooth_damp_factor_saddle;         /* global post-sweep relaxation alpha      */
@@ -346,6 +351,15 @@ int  pmg_sequential_schedule(int n_cells, const int *u_global, const int *p_glob
                              const int *a_rowptr, const int *a_kcol, const int *bt_rowptr, const int *bt_kcol,
                              const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets,
                              int *cell_reorder);
+
+/* Replaces: the pressure-dof order of TNSE_MGLevel4::NodalVanka (src/FE/NSE_MGLevel4.C:
+   4098-4549, smoothers 3 and 4, DOF_Reorder = 0): patch i = pressure dof i and the velocity
+   dofs of B row i; the dofs in level sets as in pmg_sequential_schedule (coupled patches in
+   order, each set uncoupled), ascending inside a set.  ptr_sets must hold n_pdof+1 ints,
+   pdof_order n_pdof ints. */
+int  pmg_nodal_schedule(int n_udof, int n_pdof, const int *a_rowptr, const int *a_kcol, const int *bt_rowptr,
+                        const int *bt_kcol, const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets,
+                        int *pdof_order);
 
 /* ----------------------------------------------------------------------------------- */
 /* device memory helpers for callers without a device allocator                          */

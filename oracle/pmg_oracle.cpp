@@ -489,6 +489,129 @@ int CellVankaGS(oracle_mg &mg, OLevel &L, double *u1, const double *rhs1, double
   return info_all;
 }
 
+// SolveDiagonalVanka3D with N_P = 1 (MGComponents3D.C:32-85): the velocity block replaced
+// by its diagonal, taken from the first component's rows for all three (Ai[i] = 1/A(i,i),
+// i < N_U); a = System column-major with leading dimension LDA, b: rhs in, solution out.
+void SolveDiagonalVanka3D(const double *a, double *b, int N_U, int LDA) {
+  auto AT = [&](int i, int j) { return a[size_t(j) * LDA + i]; };
+  const int row = 3 * N_U;
+  std::vector<double> Ai(N_U);
+  for (int i = 0; i < N_U; i++) Ai[i] = 1 / AT(i, i);
+  double dp = 0, ffp = b[row];
+  for (int i = 0, j = N_U, k = 2 * N_U; i < N_U; i++, j++, k++) {
+    const double tmp = Ai[i];
+    dp -= tmp * (AT(row, i) * AT(i, row) + AT(row, j) * AT(j, row) + AT(row, k) * AT(k, row));
+    ffp -= tmp * (AT(row, i) * b[i] + AT(row, j) * b[j] + AT(row, k) * b[k]);
+  }
+  const double pp = ffp / dp;
+  b[row] = pp;
+  for (int i = 0, j = N_U, k = 2 * N_U; i < N_U; i++, j++, k++) {
+    const double tmp = Ai[i];
+    b[i] = tmp * (b[i] - AT(i, row) * pp);
+    b[j] = tmp * (b[j] - AT(j, row) * pp);
+    b[k] = tmp * (b[k] - AT(k, row) * pp);
+  }
+}
+
+// TNSE_MGLevel4::NodalVanka, smoothers 3 (diagonal) and 4 (full LU): NSE_MGLevel4.C:4098-4549
+// (one process, DOF_Reorder = 0): pressure dofs in order, each with the velocity dofs of
+// its B row, local system assembled (columns found by linear search in UDOFs), solved,
+// updates applied at once; then the global relaxation (without the |1 - alpha| test of
+// CellVanka) and the Dirichlet copy.
+int NodalVanka(oracle_mg &mg, OLevel &L, double *u1, const double *rhs1, double *aux, int smoother) {
+  const pmg_level_desc &d = L.d;
+  const int N_UDOF = L.N_UDOF, HangingBound = L.HangingNodeBound;
+  const double damp = mg.prm.sc_smooth_damp_factor_coarse_saddle;
+  const double alpha = mg.prm.sc_smooth_damp_factor_saddle;
+  double *u2 = u1 + N_UDOF, *u3 = u2 + N_UDOF, *p = u1 + 3 * N_UDOF;
+  const double *rhs2 = rhs1 + N_UDOF, *rhs3 = rhs2 + N_UDOF, *rhsp = rhs1 + 3 * N_UDOF;
+  SetDirichletNodes(L, u1, rhs1);                      // :4182-4188
+  double *uold = aux;
+  std::memcpy(uold, u1, sizeof(double) * L.N_DOF);
+  int info_all = 0;
+  std::vector<int> UDOFs;
+  std::vector<double> System, Rhs;
+  std::vector<int> piv;
+  for (int i = 0; i < L.N_PDOF; i++) {
+    const int begin = d.b_rowptr[i], end = d.b_rowptr[i + 1];
+    double value = rhsp[i];
+    UDOFs.clear();
+    std::vector<double> bl;
+    for (int k = begin; k < end; k++) {
+      const int l = d.b_kcol[k];
+      const double v1 = d.b[0][k], v2 = d.b[1][k], v3 = d.b[2][k];
+      value -= v1 * u1[l] + v2 * u2[l] + v3 * u3[l];
+      UDOFs.push_back(l);
+      bl.push_back(v1); bl.push_back(v2); bl.push_back(v3);
+    }
+    const int N_U = int(UDOFs.size()), N_U2 = 2 * N_U, N_UGEO = 3 * N_U, N_LocalDOF = N_UGEO + 1;
+    System.assign(size_t(N_LocalDOF) * N_LocalDOF, 0.0);
+    Rhs.assign(N_LocalDOF, 0.0);
+    Rhs[N_LocalDOF - 1] = value;
+    for (int k = 0; k < N_U; k++) {                    // local B in the pressure row
+      System[size_t(k) * N_LocalDOF + N_UGEO] = bl[3 * k];
+      System[size_t(k + N_U) * N_LocalDOF + N_UGEO] = bl[3 * k + 1];
+      System[size_t(k + N_U2) * N_LocalDOF + N_UGEO] = bl[3 * k + 2];
+    }
+    for (int j = 0; j < N_U; j++) {
+      const int j1 = j, j2 = j + N_U, j3 = j2 + N_U;
+      const int UDOF = UDOFs[j];
+      Rhs[j1] = rhs1[UDOF];
+      Rhs[j2] = rhs2[UDOF];
+      Rhs[j3] = rhs3[UDOF];
+      for (int k = d.a_rowptr[UDOF]; k < d.a_rowptr[UDOF + 1]; k++) {
+        const int l = d.a_kcol[k];
+        double v11 = d.a[0][k], v12 = d.a[1][k], v13 = d.a[2][k], v21 = d.a[3][k], v22 = d.a[4][k], v23 = d.a[5][k],
+               v31 = d.a[6][k], v32 = d.a[7][k], v33 = d.a[8][k];
+        if (UDOF >= HangingBound) v12 = v13 = v21 = v23 = v31 = v32 = 0;   // Dirichlet node
+        Rhs[j1] -= v11 * u1[l] + v12 * u2[l] + v13 * u3[l];
+        Rhs[j2] -= v21 * u1[l] + v22 * u2[l] + v23 * u3[l];
+        Rhs[j3] -= v31 * u1[l] + v32 * u2[l] + v33 * u3[l];
+        for (int m = 0; m < N_U; m++)
+          if (UDOFs[m] == l) {
+            const size_t k1 = size_t(m) * N_LocalDOF, k2 = size_t(m + N_U) * N_LocalDOF,
+                         k3 = size_t(m + 2 * N_U) * N_LocalDOF;
+            System[k1 + j1] = v11; System[k2 + j1] = v12; System[k3 + j1] = v13;
+            System[k1 + j2] = v21; System[k2 + j2] = v22; System[k3 + j2] = v23;
+            System[k1 + j3] = v31; System[k2 + j3] = v32; System[k3 + j3] = v33;
+            break;
+          }
+      }
+      if (UDOF < HangingBound) {                        // active dof: BT row
+        for (int k = d.bt_rowptr[UDOF]; k < d.bt_rowptr[UDOF + 1]; k++) {
+          const int l = d.bt_kcol[k];
+          const double v1 = d.bt[0][k], v2 = d.bt[1][k], v3 = d.bt[2][k], pv = p[l];
+          Rhs[j1] -= v1 * pv;
+          Rhs[j2] -= v2 * pv;
+          Rhs[j3] -= v3 * pv;
+          if (i == l) {
+            const size_t j4 = size_t(N_UGEO) * N_LocalDOF;
+            System[j4 + j1] = v1; System[j4 + j2] = v2; System[j4 + j3] = v3;
+          }
+        }
+      }
+    }
+    if (smoother == 3) {
+      SolveDiagonalVanka3D(System.data(), Rhs.data(), N_U, N_LocalDOF);
+    } else {
+      piv.resize(N_LocalDOF);
+      const int info = lu_factor(N_LocalDOF, System.data(), piv.data());
+      if (info) info_all = std::max(info_all, i + 1);
+      else lu_solve(N_LocalDOF, System.data(), piv.data(), Rhs.data());
+    }
+    for (int j = 0; j < N_U; j++) {
+      const int l = UDOFs[j];
+      u1[l] += damp * Rhs[j];
+      u2[l] += damp * Rhs[j + N_U];
+      u3[l] += damp * Rhs[j + N_U2];
+    }
+    p[i] += damp * Rhs[N_UGEO];
+  }
+  for (int j = 0; j < L.N_DOF; j++) u1[j] = uold[j] + alpha * (u1[j] - uold[j]);
+  SetDirichletNodes(L, u1, rhs1);
+  return info_all;
+}
+
 // children of every coarse cell in child order, from the fine level's parent/child maps
 // (cell->GetParent / parent->GetChild(j))
 void ChildrenOf(const OLevel &F, int n_coarse, std::vector<int> &kids) {
@@ -774,6 +897,9 @@ int SolveExact(oracle_mg &mg, OLevel &L, double *u1, const double *rhs1) {
 }
 
 int CellVanka(oracle_mg &mg, OLevel &L, double *u, const double *f, double *aux) {
+  // smoother dispatch of Cycle (NSE_MultiGrid.C:1193-1329): 3/4 nodal Vanka
+  if (mg.prm.sc_smoother_saddle == 3 || mg.prm.sc_smoother_saddle == 4)
+    return NodalVanka(mg, L, u, f, aux, mg.prm.sc_smoother_saddle);
   return mg.variant == 0 ? CellVankaColored(mg, L, u, f, aux) : CellVankaGS(mg, L, u, f, aux);
 }
 

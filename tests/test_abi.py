@@ -109,3 +109,16 @@ def test_invalid_params_rejected_before_device():
     p.precision = 7
     assert lib.pmg_ctx_create(0, C.byref(p), C.byref(ctx)) == 1
     assert not ctx.value
+
+
+def test_smoother_choices_checked_before_device():
+    """Smoothers 1-4 exist (cell Vanka in the Gauss-Seidel / colored order, nodal Vanka with
+    the diagonal / full local system); the nodal ones run in FP64 only, anything else is
+    PMG_ERR_UNSUPPORTED (10) before a device is touched."""
+    from to_merge_master_parmoon_amd import _abi
+    lib = _abi.load_pmg()
+    for sm, prec in ((0, 0), (5, 0), (3, 1), (4, 2)):
+        ctx = C.c_void_p()
+        p = _abi.default_params(sc_smoother_saddle=sm, precision=prec)
+        assert lib.pmg_ctx_create(0, C.byref(p), C.byref(ctx)) == 10, (sm, prec)
+        assert not ctx.value

@@ -68,3 +68,18 @@ def test_noalloc_refuses_oversized_and_mixed():
     assert mm._lib.pmg_vanka_smooth_noalloc(mm._ctx, 1, u.ptr, f.ptr, 1) == 10
     u.free(); f.free()
     mm.close()
+
+
+def test_noalloc_nodal_sweep_matches(oseen16):
+    """Nodal Vanka (smoother 4) relaxes on every sweep: its copy of u is always taken from
+    the array; two calls in a row, the second on the smaller level."""
+    from to_merge_master_parmoon_amd.multigrid import build_hierarchy
+    mg = build_hierarchy(oseen16, params(sc_smoother_saddle=4, sc_smooth_damp_factor_saddle=1.0))
+    for call, lvl in enumerate([2, 1]):
+        n = mg.n_dof[lvl]
+        rng = np.random.default_rng(620 + call)
+        u0, f = rng.standard_normal(n), rng.standard_normal(n)
+        a, b = _smooth_both(mg, lvl, u0, f, 2)
+        assert not np.array_equal(a, u0)
+        assert np.array_equal(a, b)
+    mg.close()

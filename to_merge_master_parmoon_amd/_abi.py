@@ -139,6 +139,8 @@ PMG_FUNCS = {
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_sequential_schedule": (C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, C.c_int, c_int_p, c_int_p, c_int_p,
                                           c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
+    "pmg_nodal_schedule": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
+                                     c_int_p, c_int_p]),
     "pmg_level_read": (C.c_int, [C.c_char_p, C.POINTER(PmgLevelDesc), C.POINTER(C.c_void_p)]),
     "pmg_level_read_free": (C.c_int, [C.c_void_p]),
     "pmg_level_write": (C.c_int, [C.c_char_p, C.POINTER(PmgLevelDesc)]),

@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["pmg_api.cpp", "batch_merge.cpp", "coloring.cpp", "k_operator.hip", "k_vanka.hip", "k_dense.hip",
-               "k_batch.hip", "k_assemble.hip", "k_coarse.hip", "ingest.cpp"]
+               "k_batch.hip", "k_assemble.hip", "k_coarse.hip", "ingest.cpp", "k_nodal.hip"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
     # no packed FP32 instructions anywhere in the device code

@@ -60,16 +60,85 @@ extern "C" int pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, co
   return PMG_OK;
 }
 
-// TNSE_MGLevel4::CellVanka, Gauss-Seidel variant (_SEQ, src/FE/NSE_MGLevel4.C:1653-2085),
-// visits the cells in collection order and updates u after every cell.  Cell j's local
-// residual reads the rows of its patch dofs, i.e. every dof in a column of those rows, so
-// its result depends on an earlier cell i exactly when i's patch dofs meet j's columns or
-// j's patch dofs meet i's columns ("coupled").  Level sets with
+// Level sets of a sequential patch order.  Patch j (j = 0 .. n_patches-1, visited in this
+// order by the reference) owns the velocity dofs vidx[vptr[j] .. vptr[j+1]) and the pressure
+// dofs pidx[pptr[j] .. pptr[j+1]); its local residual reads the rows of those dofs, i.e. every
+// dof in a column of those rows, so its result depends on an earlier patch i exactly when
+// i's dofs meet j's columns or j's dofs meet i's columns ("coupled").  Level sets with
 //     set(j) = 1 + max { set(i) : i < j, i coupled with j }   (0 if there is none)
-// list every coupled pair in collection order and hold only uncoupled cells together, so
-// running the sets one after the other, each as one colored step (all local residuals of
-// the set, then all updates), performs the Gauss-Seidel sweep's arithmetic exactly.
-// Output as pmg_color_cells: sets in order, cells ascending inside a set.
+// list every coupled pair in visiting order and hold only uncoupled patches together, so
+// running the sets one after the other, each as one colored step, performs the
+// sequential sweep's arithmetic exactly.
+namespace {
+int patch_level_sets(int n_patches, const std::vector<int> &vptr, const std::vector<int> &vidx,
+                     const std::vector<int> &pptr, const std::vector<int> &pidx, int n_udof, int n_pdof,
+                     const int *a_rowptr, const int *a_kcol, const int *bt_rowptr, const int *bt_kcol,
+                     const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets, int *order) {
+  // dof -> patches
+  auto invert = [&](const std::vector<int> &ptr, const std::vector<int> &idx, int ndof, std::vector<int> &dp,
+                    std::vector<int> &pc) {
+    dp.assign(size_t(ndof) + 1, 0);
+    for (int g : idx) {
+      if (g < 0 || g >= ndof) return false;
+      dp[size_t(g) + 1]++;
+    }
+    for (int i = 0; i < ndof; ++i) dp[i + 1] += dp[i];
+    pc.resize(dp[ndof]);
+    std::vector<int> pos(dp.begin(), dp.end() - 1);
+    for (int q = 0; q < n_patches; ++q)
+      for (int k = ptr[q]; k < ptr[q + 1]; ++k) pc[pos[idx[k]]++] = q;
+    return true;
+  };
+  std::vector<int> up, uc, pp, pc;
+  if (!invert(vptr, vidx, n_udof, up, uc) || !invert(pptr, pidx, n_pdof, pp, pc)) return PMG_ERR_INVALID_ARGUMENT;
+  // forward relation: patches whose dofs meet the columns of patch j's rows
+  std::vector<int> mark(n_patches, -1);
+  std::vector<std::vector<int>> nb(n_patches);
+  for (int j = 0; j < n_patches; ++j) {
+    auto hit = [&](const std::vector<int> &dp, const std::vector<int> &dc, int col) {
+      for (int q = dp[col]; q < dp[col + 1]; ++q)
+        if (mark[dc[q]] != j) {
+          mark[dc[q]] = j;
+          nb[j].push_back(dc[q]);
+        }
+    };
+    for (int k = vptr[j]; k < vptr[j + 1]; ++k) {
+      const int r = vidx[k];
+      for (int e = a_rowptr[r]; e < a_rowptr[r + 1]; ++e) hit(up, uc, a_kcol[e]);
+      for (int e = bt_rowptr[r]; e < bt_rowptr[r + 1]; ++e) hit(pp, pc, bt_kcol[e]);
+    }
+    for (int k = pptr[j]; k < pptr[j + 1]; ++k) {
+      const int r = pidx[k];
+      for (int e = b_rowptr[r]; e < b_rowptr[r + 1]; ++e) hit(up, uc, b_kcol[e]);
+    }
+  }
+  // symmetric closure: i coupled with j if either meets the other's columns
+  for (int j = 0; j < n_patches; ++j)
+    for (int i : std::vector<int>(nb[j]))
+      if (i != j) nb[i].push_back(j);
+  std::vector<int> set(n_patches, 0);
+  int nsets = 0;
+  for (int j = 0; j < n_patches; ++j) {
+    int sj = 0;
+    for (int i : nb[j])
+      if (i < j && set[i] + 1 > sj) sj = set[i] + 1;
+    set[j] = sj;
+    if (sj + 1 > nsets) nsets = sj + 1;
+  }
+  std::vector<int> cnt(size_t(nsets) + 1, 0);
+  for (int q = 0; q < n_patches; ++q) cnt[set[q] + 1]++;
+  for (int k = 0; k < nsets; ++k) cnt[k + 1] += cnt[k];
+  for (int k = 0; k <= nsets; ++k) ptr_sets[k] = cnt[k];
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int q = 0; q < n_patches; ++q) order[fill[set[q]]++] = q;
+  *n_sets = nsets;
+  return PMG_OK;
+}
+}  // namespace
+
+// TNSE_MGLevel4::CellVanka, Gauss-Seidel variant (_SEQ, src/FE/NSE_MGLevel4.C:1653-2085):
+// cells in collection order, each patch its 27 velocity and 4 pressure dofs.  Output as
+// pmg_color_cells: sets in order, cells ascending inside a set.
 extern "C" int pmg_sequential_schedule(int n_cells, const int *u_global, const int *p_global, int n_udof, int n_pdof,
                                        const int *a_rowptr, const int *a_kcol, const int *bt_rowptr,
                                        const int *bt_kcol, const int *b_rowptr, const int *b_kcol, int *n_sets,
@@ -83,69 +152,29 @@ extern "C" int pmg_sequential_schedule(int n_cells, const int *u_global, const i
     ptr_sets[0] = 0;
     return PMG_OK;
   }
-  // dof -> cells for velocity and pressure dofs
-  auto invert = [&](const int *g, int nl, int ndof, std::vector<int> &ptr, std::vector<int> &cells) {
-    ptr.assign(size_t(ndof) + 1, 0);
-    for (long k = 0; k < long(n_cells) * nl; ++k) {
-      if (g[k] < 0 || g[k] >= ndof) return false;
-      ptr[size_t(g[k]) + 1]++;
-    }
-    for (int i = 0; i < ndof; ++i) ptr[i + 1] += ptr[i];
-    cells.resize(ptr[ndof]);
-    std::vector<int> pos(ptr.begin(), ptr.end() - 1);
-    for (int c = 0; c < n_cells; ++c)
-      for (int j = 0; j < nl; ++j) cells[pos[g[long(c) * nl + j]]++] = c;
-    return true;
-  };
-  std::vector<int> up, uc, pp, pc;
-  if (!invert(u_global, NU, n_udof, up, uc) || !invert(p_global, NP, n_pdof, pp, pc)) return PMG_ERR_INVALID_ARGUMENT;
-  // forward relation: cells whose patch dofs meet the columns of cell j's rows
-  std::vector<int> mark(n_cells, -1);
-  std::vector<std::vector<int>> nb(n_cells);
-  for (int j = 0; j < n_cells; ++j) {
-    auto hit_u = [&](int col) {
-      for (int q = up[col]; q < up[col + 1]; ++q)
-        if (mark[uc[q]] != j) {
-          mark[uc[q]] = j;
-          nb[j].push_back(uc[q]);
-        }
-    };
-    auto hit_p = [&](int col) {
-      for (int q = pp[col]; q < pp[col + 1]; ++q)
-        if (mark[pc[q]] != j) {
-          mark[pc[q]] = j;
-          nb[j].push_back(pc[q]);
-        }
-    };
-    for (int l = 0; l < NU; ++l) {
-      const int r = u_global[long(j) * NU + l];
-      for (int k = a_rowptr[r]; k < a_rowptr[r + 1]; ++k) hit_u(a_kcol[k]);
-      for (int k = bt_rowptr[r]; k < bt_rowptr[r + 1]; ++k) hit_p(bt_kcol[k]);
-    }
-    for (int l = 0; l < NP; ++l) {
-      const int r = p_global[long(j) * NP + l];
-      for (int k = b_rowptr[r]; k < b_rowptr[r + 1]; ++k) hit_u(b_kcol[k]);
-    }
+  std::vector<int> vptr(size_t(n_cells) + 1), pptr(size_t(n_cells) + 1);
+  for (int c = 0; c <= n_cells; ++c) {
+    vptr[c] = c * NU;
+    pptr[c] = c * NP;
   }
-  // symmetric closure: i coupled with j if either meets the other's columns
-  for (int j = 0; j < n_cells; ++j)
-    for (int i : std::vector<int>(nb[j]))
-      if (i != j) nb[i].push_back(j);
-  std::vector<int> set(n_cells, 0);
-  int nsets = 0;
-  for (int j = 0; j < n_cells; ++j) {
-    int s = 0;
-    for (int i : nb[j])
-      if (i < j && set[i] + 1 > s) s = set[i] + 1;
-    set[j] = s;
-    if (s + 1 > nsets) nsets = s + 1;
-  }
-  std::vector<int> cnt(size_t(nsets) + 1, 0);
-  for (int c = 0; c < n_cells; ++c) cnt[set[c] + 1]++;
-  for (int k = 0; k < nsets; ++k) cnt[k + 1] += cnt[k];
-  for (int k = 0; k <= nsets; ++k) ptr_sets[k] = cnt[k];
-  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-  for (int c = 0; c < n_cells; ++c) cell_reorder[fill[set[c]]++] = c;
-  *n_sets = nsets;
-  return PMG_OK;
+  std::vector<int> vidx(u_global, u_global + size_t(n_cells) * NU), pidx(p_global, p_global + size_t(n_cells) * NP);
+  return patch_level_sets(n_cells, vptr, vidx, pptr, pidx, n_udof, n_pdof, a_rowptr, a_kcol, bt_rowptr, bt_kcol,
+                          b_rowptr, b_kcol, n_sets, ptr_sets, cell_reorder);
+}
+
+// TNSE_MGLevel4::NodalVanka (src/FE/NSE_MGLevel4.C:4098-4549): pressure dofs in order, each
+// patch the velocity dofs of its B row and the pressure dof itself.  Output: sets in order,
+// pressure dofs ascending inside a set (ptr_sets holds n_pdof + 1 ints).
+extern "C" int pmg_nodal_schedule(int n_udof, int n_pdof, const int *a_rowptr, const int *a_kcol,
+                                  const int *bt_rowptr, const int *bt_kcol, const int *b_rowptr, const int *b_kcol,
+                                  int *n_sets, int *ptr_sets, int *pdof_order) {
+  if (n_udof <= 0 || n_pdof <= 0 || !a_rowptr || !a_kcol || !bt_rowptr || !bt_kcol || !b_rowptr || !b_kcol ||
+      !n_sets || !ptr_sets || !pdof_order)
+    return PMG_ERR_INVALID_ARGUMENT;
+  std::vector<int> vptr(b_rowptr, b_rowptr + size_t(n_pdof) + 1), vidx(b_kcol, b_kcol + b_rowptr[n_pdof]);
+  std::vector<int> pptr(size_t(n_pdof) + 1), pidx(n_pdof);
+  for (int i = 0; i <= n_pdof; ++i) pptr[i] = i;
+  for (int i = 0; i < n_pdof; ++i) pidx[i] = i;
+  return patch_level_sets(n_pdof, vptr, vidx, pptr, pidx, n_udof, n_pdof, a_rowptr, a_kcol, bt_rowptr, bt_kcol,
+                          b_rowptr, b_kcol, n_sets, ptr_sets, pdof_order);
 }

@@ -369,12 +369,31 @@ std::vector<int> build_patch_map(const pmg_level_desc *d, const std::vector<int>
 // Results reported by the device after the fact: the coarse LU (second stream) pivot check
 // and its time, and the wait-bound flag of the coarse triangular sweeps.
 void check_coarse(pmg_ctx *c, Coarse &C);
+void check_nodal(pmg_ctx *c);
 void check_deferred(pmg_ctx *c) {
   if (!c->batch) {
     check_coarse(c, c->coarse);
+    check_nodal(c);
     return;
   }
   for (auto &b : c->bi) check_coarse(c, b.co);
+}
+// nodal Vanka: the largest pressure dof (+1) whose local system had a zero pivot in the
+// sweeps since the last check; reported once and cleared (the stream is idle here)
+void check_nodal(pmg_ctx *c) {
+  for (size_t l = 0; l < c->L.size(); ++l) {
+    DevLevel &D = c->L[l];
+    if (!D.nodal_info) continue;
+    int e = 0;
+    PMG_HIP_CHECK(hipMemcpyAsync(&e, D.nodal_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (e) {
+      launch_fill<int>(D.nodal_info, 0, 1, c->stream);
+      PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+      throw Fail{PMG_ERR_SINGULAR_PATCH, "zero pivot in the nodal Vanka patch of pressure dof " +
+                                             std::to_string(e - 1) + " of level " + std::to_string(l)};
+    }
+  }
 }
 void check_coarse(pmg_ctx *c, Coarse &C) {
   if (C.info_pending) {
@@ -555,12 +574,43 @@ inline double solve_bytes(int fbytes, int vs) {
 }
 
 // TNSE_MGLevel4::CellVanka, colored (NSE_MGLevel4.C:603-1649), one sweep
+// TNSE_MGLevel4::NodalVanka (NSE_MGLevel4.C:4098-4549), one sweep: SetDirichletNodes, the
+// pressure-dof patches set by set, then the relaxation with alpha on every dof (always, the
+// nodal sweep has no |1 - alpha| test) and SetDirichletNodes again
+void nodal_sweep(pmg_ctx *c, DevLevel &D, double *u, const double *f, double *aux, bool last) {
+  const double alpha = c->prm.sc_smooth_damp_factor_saddle;
+  const double damp = c->prm.sc_smooth_damp_factor_coarse_saddle;
+  const bool full = c->prm.sc_smoother_saddle == 4;
+  set_dirichlet(c, D, u, f);
+  launch_copy<double>(aux, u, D.n_dof, c->stream);
+  const OpView<double> op = D.op64();
+  const int ns = int(D.nodal_ptr.size()) - 1;
+  for (int k = 0; k < ns; ++k) {
+    const int p0 = D.nodal_ptr[k], n = D.nodal_ptr[k + 1] - p0;
+    const double nl = 3.0 * NODAL_MAXU + 1;
+    Timed tm(c, &c->tm.vanka_solve, D.nodal_bytes[k], n * (full ? (2.0 / 3.0) * nl * nl * nl : 20.0 * nl));
+    launch_nodal_vanka(op, full, D.nodal_order + p0, n, f, u, damp, D.nodal_info, c->stream);
+  }
+  launch_relax<double>(u, aux, alpha, D.n_dof, c->stream);
+  if (last) set_dirichlet(c, D, u, f);
+}
+
 template <typename T>
 // last = false: another sweep with the same f follows at once and sets the Dirichlet values
 // again before anything reads them, so this sweep's closing SetDirichletNodes is left out
 void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, bool last = true) {
   const bool prof = c->profiling;
   auto t0 = std::chrono::steady_clock::now();
+  if constexpr (sizeof(T) == 8) {
+    if (D.nodal) {
+      nodal_sweep(c, D, u, f, aux, last);
+      if (prof) {
+        c->tm.sweep_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        c->tm.sweep_calls += 1;
+      }
+      return;
+    }
+  }
   const double alpha = c->prm.sc_smooth_damp_factor_saddle;
   const T damp = T(c->prm.sc_smooth_damp_factor_coarse_saddle);
   const bool relax = std::fabs(1 - alpha) > 1e-3;
@@ -979,6 +1029,10 @@ void setup_factors(pmg_ctx *c) {
   for (size_t l = 1; l < c->L.size(); ++l) {
     DevLevel &D = c->L[l];
     if (D.factors_ready) continue;
+    if (D.nodal) {   // nodal Vanka assembles and solves its patches at every visit
+      D.factors_ready = true;
+      continue;
+    }
     if (D.minv32 || D.minv16) {
       // the block form needs the shared velocity block; the captured V-cycle launches the
       // local solves of the form it was captured with
@@ -1577,7 +1631,11 @@ const char *pmg_status_string(int s) {
 }
 
 static int check_params(const pmg_params *p, std::string &why) {
-  if (p->sc_smoother_saddle != 1 && p->sc_smoother_saddle != 2) { why = "only cell Vanka smoothers 1/2"; return PMG_ERR_UNSUPPORTED; }
+  if (p->sc_smoother_saddle < 1 || p->sc_smoother_saddle > 4) { why = "only the Vanka smoothers 1-4"; return PMG_ERR_UNSUPPORTED; }
+  if (p->sc_smoother_saddle >= 3 && p->precision != PMG_PREC_FP64) {
+    why = "the nodal Vanka smoothers (3, 4) run in FP64 only";
+    return PMG_ERR_UNSUPPORTED;
+  }
   if (p->sc_coarse_smoother_saddle != 17) { why = "only the dense coarse solve (17)"; return PMG_ERR_UNSUPPORTED; }
   if (p->precision < 0 || p->precision > 2) { why = "bad precision"; return PMG_ERR_INVALID_ARGUMENT; }
   if (p->sc_pre_smooth_saddle < 0 || p->sc_post_smooth_saddle < 0) { why = "negative smoothing steps"; return PMG_ERR_INVALID_ARGUMENT; }
@@ -1678,6 +1736,9 @@ int pmg_color_cells(int n_cells, int n_ulocal, const int *u_begin, const int *u_
 int pmg_sequential_schedule(int n_cells, const int *u_global, const int *p_global, int n_udof, int n_pdof,
                             const int *a_rowptr, const int *a_kcol, const int *bt_rowptr, const int *bt_kcol,
                             const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets, int *cell_reorder);
+int pmg_nodal_schedule(int n_udof, int n_pdof, const int *a_rowptr, const int *a_kcol, const int *bt_rowptr,
+                       const int *bt_kcol, const int *b_rowptr, const int *b_kcol, int *n_sets, int *ptr_sets,
+                       int *pdof_order);
 
 // merged: a level built by merge_hierarchies (cells without parent allowed, the color
 // lists may leave out the cells of problems solved directly on this level)
@@ -1838,7 +1899,42 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     if (lvl > 0) {
       const size_t lun = size_t(D.n_cells) * LU_STRIDE + LU_PAD;
       const size_t minvn = size_t(D.n_cells) * MINV_STRIDE + LU_PAD;
-      if (c->prm.precision == PMG_PREC_FP64) {
+      if (c->prm.sc_smoother_saddle >= 3) {
+        // nodal Vanka: no stored patch data; the pressure dofs in the level sets of their order
+        if (merged) throw Fail{PMG_ERR_UNSUPPORTED, "nodal Vanka is provided for single problems only"};
+        for (int i = 0; i < D.n_pdof; ++i)
+          if (d->b_rowptr[i + 1] - d->b_rowptr[i] > NODAL_MAXU)
+            throw Fail{PMG_ERR_UNSUPPORTED, "nodal Vanka: a B row with more than " + std::to_string(NODAL_MAXU) +
+                                                " velocity dofs"};
+        std::vector<int> nptr(size_t(D.n_pdof) + 1), nord(D.n_pdof);
+        int ns = 0;
+        int st = pmg_nodal_schedule(D.n_udof, D.n_pdof, d->a_rowptr, d->a_kcol, d->bt_rowptr, d->bt_kcol, d->b_rowptr,
+                                    d->b_kcol, &ns, nptr.data(), nord.data());
+        if (st) throw Fail{st, "nodal schedule failed"};
+        D.nodal = true;
+        D.nodal_ptr.assign(nptr.begin(), nptr.begin() + ns + 1);
+        D.nodal_order = dalloc<int>(c, D.n_pdof);
+        h2d(c, D.nodal_order, nord.data(), D.n_pdof);
+        D.nodal_info = dalloc<int>(c, 1);
+        launch_fill<int>(D.nodal_info, 0, 1, c->stream);
+        // algorithmic bytes per set: B row (columns, 3 values, 3 gathered velocities), the
+        // velocity rows (A: column + 3 values + 3 gathered velocities per entry, BT: column,
+        // value, pressure), right-hand sides, and the updates (read + write)
+        D.nodal_bytes.assign(ns, 0.0);
+        for (int k = 0; k < ns; ++k)
+          for (int q = nptr[k]; q < nptr[k + 1]; ++q) {
+            const int i = nord[q];
+            double b = 8.0 + 8.0 + 16.0;
+            for (int e = d->b_rowptr[i]; e < d->b_rowptr[i + 1]; ++e) {
+              const int r = d->b_kcol[e];
+              b += 4.0 + 24.0 + 24.0;
+              b += 3.0 * (d->a_rowptr[r + 1] - d->a_rowptr[r]) * (4.0 + 24.0 + 24.0);
+              b += 3.0 * (d->bt_rowptr[r + 1] - d->bt_rowptr[r]) * (4.0 + 8.0 + 8.0);
+              b += 3.0 * (8.0 + 16.0);
+            }
+            D.nodal_bytes[k] += b;
+          }
+      } else if (c->prm.precision == PMG_PREC_FP64) {
         D.lu = dalloc<double>(c, lun);
         launch_fill<double>(D.lu, 0.0, long(lun), c->stream);
       } else if (c->prm.patch_inverse && c->prm.precision == PMG_PREC_FP32) {
@@ -2028,7 +2124,8 @@ int pmg_vanka_smooth_noalloc(pmg_ctx *c, int level, double *u, const double *f, 
       throw Fail{PMG_ERR_UNSUPPORTED, "pmg_vanka_smooth_noalloc runs the FP64 form only (the mixed forms convert "
                                       "through level-owned FP32 vectors)"};
     DevLevel &D = level_of(c, level);
-    const bool relax = std::fabs(1 - c->prm.sc_smooth_damp_factor_saddle) > 1e-3;
+    // the relaxation copy of u: cell Vanka with |1 - alpha| > 1e-3, nodal Vanka always
+    const bool relax = D.nodal || std::fabs(1 - c->prm.sc_smooth_damp_factor_saddle) > 1e-3;
     const size_t n_rloc = size_t(D.max_cells_per_color) * RLOC_STRIDE;
     const size_t n_aux = relax ? size_t(D.n_dof) : 0;
     const size_t off_aux = (n_rloc + 31) / 32 * 32;   // 256-byte aligned

@@ -49,6 +49,8 @@ constexpr int PM_MT = PM_BT + NU_LOC * NP_LOC;   // 864
 constexpr int PM_B = PM_MT + NU_LOC;             // 891
 constexpr int PM_NB = PM_B + NP_LOC * NU_LOC;    // 999
 constexpr int PM_STRIDE = 1008;
+// nodal Vanka: velocity dofs of one B row (27 for Q2/P1disc), upper bound
+constexpr int NODAL_MAXU = 28;
 static_assert(PM_NB + NP_LOC <= PM_STRIDE, "patch map stride");
 
 // Type-4 operator.  Two value layouts:
@@ -148,6 +150,13 @@ struct DevLevel {
   // transfers to the next coarser level (this level = fine)
   Csr pu, ru, pp, rp;                // P_U (fine rows), R_U = P_U^T, P_P, R_P
   Csr rf;                            // RestrictFunction of the velocity: coarse rows, fine columns
+  // nodal Vanka (smoothers 3, 4): level sets of the pressure dofs (host pointers, device
+  // dof list), the singular-patch flag and the algorithmic bytes of every set's launch
+  bool nodal = false;
+  std::vector<int> nodal_ptr;
+  std::vector<double> nodal_bytes;
+  int *nodal_order = nullptr;
+  int *nodal_info = nullptr;
   // MG vectors: sol, rhs, aux0 (defect), aux1 (aux) as the reference level owns them
   double *sol = nullptr, *rhs = nullptr, *aux0 = nullptr, *aux1 = nullptr;
   float *sol32 = nullptr, *rhs32 = nullptr, *aux0_32 = nullptr, *aux1_32 = nullptr;
@@ -342,6 +351,11 @@ void launch_assemble_oseen(const DevLevel &L, const double *geom, const double *
 void launch_eval_velocity(const DevLevel &L, const double *u, double *bq, hipStream_t s);
 // theta-scheme right-hand side: rhs = sigma M u - A(b) u on the active velocity rows (A: the
 // same element matrices without the mass term), g on the Dirichlet rows, 0 on the pressure
+// TNSE_MGLevel4::NodalVanka on the pressure dofs pdofs[0 .. n) of one level set: residual,
+// local solve (full: LU of the 3 N_U + 1 system; else the diagonal velocity block) and
+// damped update per patch; a singular patch sets *info = max(*info, dof + 1)
+void launch_nodal_vanka(const OpView<double> &op, bool full, const int *pdofs, int n, const double *f, double *u,
+                        double damp, int *info, hipStream_t s);
 void launch_theta_rhs(const DevLevel &L, const double *geom, const double *bq, double nu, double sigma,
                       const double *u, const double *g, double *rhs, hipStream_t s);
 
