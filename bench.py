@@ -42,7 +42,8 @@ SHARED_BLOCK = 1
 PATCH_INVERSE = 1
 
 
-def solver_params(precision: int, shared_block: int | None = None, patch_inverse: int | None = None):
+def solver_params(precision: int, shared_block: int | None = None, patch_inverse: int | None = None,
+                  refine_steps: int = 0):
     from to_merge_master_parmoon_amd import _abi
     # FGMRES(30), SC_LIN_RED_FACTOR_SADDLE = 1e-10, SC_LIN_RES_NORM_MIN_SADDLE = 0,
     # SC_LIN_MAXIT_SADDLE = 1000, SC_AMG_PREC_IT = 1, V(2,2), damp = alpha = beta = 1
@@ -51,17 +52,18 @@ def solver_params(precision: int, shared_block: int | None = None, patch_inverse
                                sc_pre_smooth_saddle=2, sc_post_smooth_saddle=2, precision=precision,
                                use_graph=USE_GRAPH,
                                shared_block=SHARED_BLOCK if shared_block is None else shared_block,
-                               patch_inverse=PATCH_INVERSE if patch_inverse is None else patch_inverse)
+                               patch_inverse=PATCH_INVERSE if patch_inverse is None else patch_inverse,
+                               bf16_refine_steps=refine_steps)
 
 
 def run_gpu(prob, precision: int, steps: int, warmup: int, log, shared_block=None, patch_inverse=None,
-            solve_only_steps: int = 0, profile: bool = True):
+            solve_only_steps: int = 0, profile: bool = True, refine_steps: int = 0):
     """`steps` timed solves including the value-dependent set-up (pmg_refactor), then
     `solve_only_steps` solves with the factors kept (their difference is the set-up's cost
     on the critical path), then one profiling pass."""
     from to_merge_master_parmoon_amd.multigrid import DeviceVector, build_hierarchy
     t0 = time.time()
-    mg = build_hierarchy(prob, solver_params(precision, shared_block, patch_inverse))
+    mg = build_hierarchy(prob, solver_params(precision, shared_block, patch_inverse, refine_steps))
     log(f"hierarchy on device ({time.time() - t0:.1f}s), levels {mg.n_dof}, "
         f"shared block {[mg.shared_block(l) for l in range(mg.n_levels)]}")
     n = mg.n_dof[-1]
@@ -474,6 +476,7 @@ def main():
     ap.add_argument("--no-fp64-ref", action="store_true", help="skip the FP64 GPU solve timed beside it")
     ap.add_argument("--no-nine-blocks", action="store_true", help="skip the nine-block timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload's fields")
+    ap.add_argument("--no-refine", action="store_true", help="skip the timings with local refinement steps")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--workload", choices=["oseen32", "batched", "reassembly", "channel"], default="oseen32",
                     help="batched: the independent-problems workload; reassembly: one fixed-point step's "
@@ -544,6 +547,18 @@ def main():
         cfg.update({"fp64_solve_s": round(fp64_ms / 1e3, 5), "fp64_iterations": int(ref["iters"][-1]),
                     "speedup_vs_fp64": round(fp64_ms / ms, 3),
                     "rel_err_vs_fp64": {"velocity": du, "pressure": dp}})
+    if prec and PATCH_INVERSE and not args.no_refine:
+        # FP32 refinement steps after every local solve with the stored (BF16) data
+        cfg["refine_steps"] = {}
+        for k in (1, 2, 3):
+            r = run_gpu(prob, prec, 2, 1, log, profile=False, refine_steps=k)
+            r["mg"].close()
+            rk = {"solve_s": round(float(np.mean(r["times"])), 5), "iterations": int(r["iters"][-1])}
+            if ref is not None:
+                du, dp = rel_err(r["x"], ref["x"], n_u3)
+                rk["rel_err_vs_fp64"] = {"velocity": du, "pressure": dp}
+            cfg["refine_steps"][str(k)] = rk
+            log(f"{k} refinement steps: {1e3 * rk['solve_s']:.1f} ms, {rk['iterations']} iterations")
     if not args.no_nine_blocks and SHARED_BLOCK:
         # all nine A blocks streamed (the general type-4 form of the operator, no credit for
         # A11 = A22 = A33 or the vanishing coupling blocks)

@@ -82,7 +82,12 @@ typedef struct {
   int    sc_minit;
   int    sc_amg_prec_it;
   int    precision;                            /* pmg_precision                           */
-  int    bf16_refine_steps;                    /* must be 0: no local refinement; FGMRES refines */
+  int    bf16_refine_steps;                    /* 0..3 refinement steps of every local solve of the
+                                                  mixed forms with stored inverses (patch_inverse
+                                                  = 1): x += M^{-1} (r - K x) with M^{-1} the
+                                                  stored (BF16 or FP32) data and K the FP32 patch
+                                                  matrix; 0 (default): one product, the outer FP64
+                                                  FGMRES refines */
   int    use_graph;                            /* 1: every V-cycle replays a HIP graph captured on
                                                   first use (default); 0: launched kernel by kernel */
   int    shared_block;                         /* 1 (default): when A11 = A22 = A33 and the other

@@ -122,3 +122,17 @@ def test_smoother_choices_checked_before_device():
         p = _abi.default_params(sc_smoother_saddle=sm, precision=prec)
         assert lib.pmg_ctx_create(0, C.byref(p), C.byref(ctx)) == 10, (sm, prec)
         assert not ctx.value
+
+
+def test_refine_steps_checked_before_device():
+    """bf16_refine_steps: 0..3 (else PMG_ERR_INVALID_ARGUMENT), and only with the stored
+    inverses of the mixed forms (FP64 or patch_inverse = 0: PMG_ERR_UNSUPPORTED)."""
+    from to_merge_master_parmoon_amd import _abi
+    lib = _abi.load_pmg()
+    for kw, want in ((dict(bf16_refine_steps=4, precision=2), 1), (dict(bf16_refine_steps=-1, precision=2), 1),
+                     (dict(bf16_refine_steps=1, precision=0), 10),
+                     (dict(bf16_refine_steps=1, precision=2, patch_inverse=0), 10)):
+        ctx = C.c_void_p()
+        p = _abi.default_params(**kw)
+        assert lib.pmg_ctx_create(0, C.byref(p), C.byref(ctx)) == want, kw
+        assert not ctx.value

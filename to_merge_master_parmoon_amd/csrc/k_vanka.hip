@@ -579,8 +579,9 @@ template <typename T, bool SH>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, int pos0,
                                                    const uint8_t *__restrict__ invperm,
                                                    const int *__restrict__ items, const int *__restrict__ desc,
-                                                   const T *__restrict__ u, const float *__restrict__ ua,
-                                                   const T *__restrict__ f, T *__restrict__ rloc) {
+                                                   const int *__restrict__ xg, const T *__restrict__ u,
+                                                   const float *__restrict__ ua, const T *__restrict__ f,
+                                                   T *__restrict__ rloc) {
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active;
   const T *__restrict__ u1 = u;
@@ -588,7 +589,16 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
   const T *__restrict__ u3 = u + 2 * N;
   const T *__restrict__ up = u + 3 * N;
   int i0, i1;
-  chunk_of_block(n_items, i0, i1);
+  if (xg) {
+    // workgroups are dealt to the 8 XCDs round-robin: block b runs on XCD b % 8 and takes
+    // its share of group b % 8 (the grid is a multiple of 8)
+    const int x = blockIdx.x & 7, k = blockIdx.x >> 3, K = gridDim.x >> 3;
+    const long a = xg[x], n = xg[x + 1] - a;
+    i0 = int(a + n * k / K);
+    i1 = int(a + n * (k + 1) / K);
+  } else {
+    chunk_of_block(n_items, i0, i1);
+  }
   int i = i0 + grp;
   int it_n = 0;
   pmg_vi4 d_n = {0, 0, 0, 0};
@@ -877,14 +887,25 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
 // chain: every product is independent; non-temporal, so the stream of inverses does not
 // evict the vectors the next residual gathers from L2), the local residual is broadcast
 // from the lanes holding it, then u[dof] += damp * x.
-template <typename F>
+// Local solve with the explicit inverse M = (P A)^{-1}, one wavefront per patch: lane i
+// holds rows i and 64 + i of M, the permuted local residual P r is broadcast by lane reads,
+// x = M (P r) = A^{-1} r in the natural order, u += damp x.
+// REF: nref refinement steps: rr = r - K x with the FP32 patch matrix K (gathered from the
+// FP32 operator through the patch map into LDS once), permuted like the residual (row i to
+// position invperm[i]) and multiplied by M again, x += M (P rr).
+template <typename F, bool REF>
 __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, const int *__restrict__ cells,
                                                     int ncells, int pos0, const int *__restrict__ slots,
                                                     const float *__restrict__ rloc, const int *__restrict__ u_global,
                                                     const int *__restrict__ p_global, int N, float *__restrict__ u,
-                                                    float *__restrict__ ua, float damp) {
+                                                    float *__restrict__ ua, float damp, OpView<float> op,
+                                                    const int *__restrict__ pmap, const uint8_t *__restrict__ invperm,
+                                                    int nref) {
   constexpr int NV = MINV_LD * int(sizeof(F)) / 16;   // 16-byte vectors per row
   constexpr int PER = 16 / int(sizeof(F));             // entries per vector
+  constexpr int KLD = NLOC + 1;
+  __shared__ float Kf[REF ? NLOC * KLD : 1];
+  __shared__ float xs[REF ? NLOC : 1], rp[REF ? RLOC_STRIDE : 1];
   const int lane = threadIdx.x;
   if (int(blockIdx.x) >= ncells) return;
   const int slot = slots ? slots[blockIdx.x] : int(blockIdx.x);
@@ -923,25 +944,70 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
   const float *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
   const float r0 = rs[lane];
   const float r1 = two ? rs[64 + lane] : 0.0f;
+  if constexpr (REF) {
+    if (nref > 0) {
+      const int *__restrict__ pm = pmap + long(cell) * PM_STRIDE;
+      for (int e = lane; e < NLOC * NLOC; e += 64) {
+        const int i = e / NLOC, m = e - NLOC * i;
+        Kf[i * KLD + m] = patch_entry<float>(op, pm, i, m);
+      }
+    }
+  }
   // the values to update, loaded before the products (no round trip after them)
   const float v0 = u[o0];
   const float v1 = two ? u[o1] : 0.0f;
-  float acc0 = 0.0f, acc1 = 0.0f;
+  // (acc0, acc1) = rows lane and 64 + lane of M times the vector held as (q0, q1)
+  auto apply = [&](float q0, float q1, float &acc0, float &acc1) {
+    acc0 = 0.0f;
+    acc1 = 0.0f;
 #pragma unroll
-  for (int j = 0; j < MINV_LD; ++j) {
-    const float rj = j < 64 ? bcast(r0, j) : bcast(r1, j - 64);   // lanes >= 21 hold r1 = 0
-    const int q = j / PER, k = j % PER;
-    const unsigned w0 = m0[q][sizeof(F) == 2 ? k / 2 : k], w1 = m1[q][sizeof(F) == 2 ? k / 2 : k];
-    float a0, a1;
-    if constexpr (sizeof(F) == 2) {
-      a0 = __uint_as_float(k % 2 ? (w0 & 0xffff0000u) : (w0 << 16));
-      a1 = __uint_as_float(k % 2 ? (w1 & 0xffff0000u) : (w1 << 16));
-    } else {
-      a0 = __uint_as_float(w0);
-      a1 = __uint_as_float(w1);
+    for (int j = 0; j < MINV_LD; ++j) {
+      const float rj = j < 64 ? bcast(q0, j) : bcast(q1, j - 64);   // lanes >= 21 hold q1 = 0
+      const int q = j / PER, k = j % PER;
+      const unsigned w0 = m0[q][sizeof(F) == 2 ? k / 2 : k], w1 = m1[q][sizeof(F) == 2 ? k / 2 : k];
+      float a0, a1;
+      if constexpr (sizeof(F) == 2) {
+        a0 = __uint_as_float(k % 2 ? (w0 & 0xffff0000u) : (w0 << 16));
+        a1 = __uint_as_float(k % 2 ? (w1 & 0xffff0000u) : (w1 << 16));
+      } else {
+        a0 = __uint_as_float(w0);
+        a1 = __uint_as_float(w1);
+      }
+      acc0 = fmaf(a0, rj, acc0);
+      acc1 = fmaf(a1, rj, acc1);
     }
-    acc0 = fmaf(a0, rj, acc0);
-    acc1 = fmaf(a1, rj, acc1);
+  };
+  float acc0, acc1;
+  apply(r0, r1, acc0, acc1);
+  if constexpr (REF) {
+    if (nref > 0) {
+      const uint8_t *__restrict__ ip = invperm + long(pos) * PERM_STRIDE;
+      const int p0 = ip[lane], p1 = two ? ip[64 + lane] : 0;
+      // the natural-order residual of this lane's rows: r_nat[i] = (P r)[invperm[i]]
+      rp[lane] = r0;
+      if (two) rp[64 + lane] = r1;
+      __syncthreads();
+      const float rn0 = rp[p0], rn1 = two ? rp[p1] : 0.0f;
+      for (int it = 0; it < nref; ++it) {
+        __syncthreads();
+        xs[lane] = acc0;
+        if (two) xs[64 + lane] = acc1;
+        __syncthreads();
+        float e0 = rn0, e1 = rn1;
+        for (int m = 0; m < NLOC; ++m) {
+          const float xm = xs[m];
+          e0 = fmaf(-Kf[lane * KLD + m], xm, e0);
+          if (two) e1 = fmaf(-Kf[(64 + lane) * KLD + m], xm, e1);
+        }
+        rp[p0] = e0;
+        if (two) rp[p1] = e1;
+        __syncthreads();
+        float d0, d1;
+        apply(rp[lane], two ? rp[64 + lane] : 0.0f, d0, d1);
+        acc0 += d0;
+        acc1 += d1;
+      }
+    }
   }
   const float n0 = v0 + damp * acc0;
   u[o0] = n0;
@@ -957,17 +1023,23 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
 // staged in LDS (non-temporal 16-byte loads), then y_c = A^{-1} r_c (lanes 0..26, three
 // components), g = r_p - sum_c B_c y_c, p = S^{-1} g (lanes 0..3), x_c = y_c - W_c p, and
 // u += damp * (x, p) (with the interleaved velocity copy when ua is given).
-template <typename F>
+// REF: nref refinement steps after the first solve: the local residual r - K x with the
+// FP32 patch matrix K (A, BT_c and B_c gathered from the FP32 operator through the patch
+// map, as the set-up gathers them), solved with the stored data and added to x.
+template <typename F, bool REF>
 __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ sch, const int *__restrict__ cells,
                                                           int ncells, int pos0, const int *__restrict__ slots,
                                                           const float *__restrict__ rloc,
                                                           const int *__restrict__ u_global,
                                                           const int *__restrict__ p_global, int N,
-                                                          float *__restrict__ u, float *__restrict__ ua, float damp) {
+                                                          float *__restrict__ u, float *__restrict__ ua, float damp,
+                                                          OpView<float> op, const int *__restrict__ pmap, int nref) {
   constexpr int PER = 16 / int(sizeof(F));
   constexpr int NVEC = SCH_STRIDE / PER;
   __shared__ float M[SCH_STRIDE];
   __shared__ float r[RLOC_STRIDE], y[3][NU_LOC], g[NP_LOC], pp[NP_LOC];
+  __shared__ float Kf[REF ? NU_LOC * (NU_LOC + 1) + 2 * 3 * NU_LOC * NP_LOC : 1];   // A, BT_c, B_c (FP32)
+  __shared__ float xs[REF ? NLOC : 1], rr[REF ? NLOC : 1];
   const int lane = threadIdx.x;
   if (int(blockIdx.x) >= ncells) return;
   const int slot = slots ? slots[blockIdx.x] : int(blockIdx.x);
@@ -990,7 +1062,22 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
   r[lane] = rs[lane];
   if (lane < NLOC - 64) r[64 + lane] = rs[64 + lane];
   const int cell = cells[pos];
-  int g0 = 0, op = 0;
+  constexpr int KA = 0, KBT = NU_LOC * (NU_LOC + 1), KB = KBT + 3 * NU_LOC * NP_LOC;
+  if constexpr (REF) {
+    if (nref > 0) {
+      const int *__restrict__ mp = pmap + long(cell) * PM_STRIDE;
+      for (int e = lane; e < NU_LOC * NU_LOC; e += 64) {
+        const int j = e / NU_LOC, m = e - NU_LOC * j;
+        Kf[KA + j * (NU_LOC + 1) + m] = patch_entry<float>(op, mp, j, m);
+      }
+      for (int e = lane; e < 3 * NU_LOC * NP_LOC; e += 64) {
+        const int c = e / (NU_LOC * NP_LOC), q = e - c * NU_LOC * NP_LOC, j = q / NP_LOC, k = q - NP_LOC * j;
+        Kf[KBT + e] = patch_entry<float>(op, mp, NU_LOC * c + j, 3 * NU_LOC + k);       // BT_c[j][k]
+        Kf[KB + (c * NP_LOC + k) * NU_LOC + j] = patch_entry<float>(op, mp, 3 * NU_LOC + k, NU_LOC * c + j);
+      }
+    }
+  }
+  int g0 = 0, op_ = 0;
   float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, vp = 0.0f;
   if (lane < NU_LOC) {
     g0 = u_global[long(cell) * NU_LOC + lane];
@@ -998,51 +1085,104 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
     v1 = u[N + g0];
     v2 = u[2 * N + g0];
   } else if (lane < NU_LOC + NP_LOC) {
-    op = 3 * N + p_global[long(cell) * NP_LOC + (lane - NU_LOC)];
-    vp = u[op];
+    op_ = 3 * N + p_global[long(cell) * NP_LOC + (lane - NU_LOC)];
+    vp = u[op_];
   }
   __syncthreads();
+  // x = M^{-1} rhs with the stored data; lane j < 27 returns x_c[j] (c = 0..2) in x[0..2],
+  // lanes 27..30 return p[lane - 27] in x[0]
+  auto solve = [&](const float *rhs, float (&x)[3]) {
+    if (lane < NU_LOC) {
+      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+      for (int m = 0; m < NU_LOC; ++m) {
+        const float w = M[SCH_AINV + lane * NU_LOC + m];
+        a0 = fmaf(w, rhs[m], a0);
+        a1 = fmaf(w, rhs[NU_LOC + m], a1);
+        a2 = fmaf(w, rhs[2 * NU_LOC + m], a2);
+      }
+      y[0][lane] = a0;
+      y[1][lane] = a1;
+      y[2][lane] = a2;
+    }
+    __syncthreads();
+    {
+      // g = r_p - sum_c B_c y_c: the 16 lanes of group k share pressure row k's 81 terms
+      const int k = lane >> 4, part = lane & 15;
+      float acc = 0.0f;
+      for (int q = part; q < 3 * NU_LOC; q += 16) {
+        const int c = q / NU_LOC, j = q - NU_LOC * c;
+        acc = fmaf(M[SCH_B + (c * NP_LOC + k) * NU_LOC + j], y[c][j], acc);
+      }
+      acc += __shfl_xor(acc, 8, 16);
+      acc += __shfl_xor(acc, 4, 16);
+      acc += __shfl_xor(acc, 2, 16);
+      acc += __shfl_xor(acc, 1, 16);
+      if (part == 0) g[k] = rhs[3 * NU_LOC + k] - acc;
+    }
+    __syncthreads();
+    if (lane < NP_LOC) {
+      float acc = 0.0f;
+      for (int l = 0; l < NP_LOC; ++l) acc = fmaf(M[SCH_SINV + lane * NP_LOC + l], g[l], acc);
+      pp[lane] = acc;
+    }
+    __syncthreads();
+    x[0] = x[1] = x[2] = 0.0f;
+    if (lane < NU_LOC) {
+      for (int c = 0; c < 3; ++c) {
+        float acc = y[c][lane];
+        for (int k = 0; k < NP_LOC; ++k) acc = fmaf(-M[SCH_W + (c * NU_LOC + lane) * NP_LOC + k], pp[k], acc);
+        x[c] = acc;
+      }
+    } else if (lane < NU_LOC + NP_LOC) {
+      x[0] = pp[lane - NU_LOC];
+    }
+  };
+  float x[3];
+  solve(r, x);
+  if constexpr (REF) {
+    for (int it = 0; it < nref; ++it) {
+      // x to LDS (natural local order), then rr = r - K x
+      if (lane < NU_LOC) {
+        xs[lane] = x[0];
+        xs[NU_LOC + lane] = x[1];
+        xs[2 * NU_LOC + lane] = x[2];
+      } else if (lane < NU_LOC + NP_LOC) {
+        xs[3 * NU_LOC + lane - NU_LOC] = x[0];
+      }
+      __syncthreads();
+      if (lane < NU_LOC) {
+        float a0 = r[lane], a1 = r[NU_LOC + lane], a2 = r[2 * NU_LOC + lane];
+        for (int m = 0; m < NU_LOC; ++m) {
+          const float w = Kf[KA + lane * (NU_LOC + 1) + m];
+          a0 = fmaf(-w, xs[m], a0);
+          a1 = fmaf(-w, xs[NU_LOC + m], a1);
+          a2 = fmaf(-w, xs[2 * NU_LOC + m], a2);
+        }
+        for (int k = 0; k < NP_LOC; ++k) {
+          const float pk = xs[3 * NU_LOC + k];
+          a0 = fmaf(-Kf[KBT + (0 * NU_LOC + lane) * NP_LOC + k], pk, a0);
+          a1 = fmaf(-Kf[KBT + (1 * NU_LOC + lane) * NP_LOC + k], pk, a1);
+          a2 = fmaf(-Kf[KBT + (2 * NU_LOC + lane) * NP_LOC + k], pk, a2);
+        }
+        rr[lane] = a0;
+        rr[NU_LOC + lane] = a1;
+        rr[2 * NU_LOC + lane] = a2;
+      } else if (lane < NU_LOC + NP_LOC) {
+        const int k = lane - NU_LOC;
+        float a = r[3 * NU_LOC + k];
+        for (int c = 0; c < 3; ++c)
+          for (int j = 0; j < NU_LOC; ++j) a = fmaf(-Kf[KB + (c * NP_LOC + k) * NU_LOC + j], xs[c * NU_LOC + j], a);
+        rr[3 * NU_LOC + k] = a;
+      }
+      __syncthreads();
+      float dx[3];
+      solve(rr, dx);
+      x[0] += dx[0];
+      x[1] += dx[1];
+      x[2] += dx[2];
+    }
+  }
   if (lane < NU_LOC) {
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-    for (int m = 0; m < NU_LOC; ++m) {
-      const float w = M[SCH_AINV + lane * NU_LOC + m];
-      a0 = fmaf(w, r[m], a0);
-      a1 = fmaf(w, r[NU_LOC + m], a1);
-      a2 = fmaf(w, r[2 * NU_LOC + m], a2);
-    }
-    y[0][lane] = a0;
-    y[1][lane] = a1;
-    y[2][lane] = a2;
-  }
-  __syncthreads();
-  {
-    // g = r_p - sum_c B_c y_c: the 16 lanes of group k share pressure row k's 81 terms
-    const int k = lane >> 4, part = lane & 15;
-    float acc = 0.0f;
-    for (int q = part; q < 3 * NU_LOC; q += 16) {
-      const int c = q / NU_LOC, j = q - NU_LOC * c;
-      acc = fmaf(M[SCH_B + (c * NP_LOC + k) * NU_LOC + j], y[c][j], acc);
-    }
-    acc += __shfl_xor(acc, 8, 16);
-    acc += __shfl_xor(acc, 4, 16);
-    acc += __shfl_xor(acc, 2, 16);
-    acc += __shfl_xor(acc, 1, 16);
-    if (part == 0) g[k] = r[3 * NU_LOC + k] - acc;
-  }
-  __syncthreads();
-  if (lane < NP_LOC) {
-    float acc = 0.0f;
-    for (int l = 0; l < NP_LOC; ++l) acc = fmaf(M[SCH_SINV + lane * NP_LOC + l], g[l], acc);
-    pp[lane] = acc;
-  }
-  __syncthreads();
-  if (lane < NU_LOC) {
-    float x[3];
-    for (int c = 0; c < 3; ++c) {
-      float acc = y[c][lane];
-      for (int k = 0; k < NP_LOC; ++k) acc = fmaf(-M[SCH_W + (c * NU_LOC + lane) * NP_LOC + k], pp[k], acc);
-      x[c] = acc;
-    }
     const float n0 = v0 + damp * x[0], n1 = v1 + damp * x[1], n2 = v2 + damp * x[2];
     u[g0] = n0;
     u[N + g0] = n1;
@@ -1053,7 +1193,7 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
       ua[4 * g0 + 2] = n2;
     }
   } else if (lane < NU_LOC + NP_LOC) {
-    u[op] = vp + damp * pp[lane - NU_LOC];
+    u[op_] = vp + damp * x[0];
   }
 }
 
@@ -1105,51 +1245,65 @@ void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s) {
 template <typename F>
 void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
                               const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, hipStream_t s) {
+                              float *ua, float damp, const OpView<float> &op, const int *pmap, int nref,
+                              hipStream_t s) {
   if (ncells <= 0) return;
-  k_vanka_apply_schur<F><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u,
-                                               ua, damp);
+  if (nref > 0)
+    k_vanka_apply_schur<F, true><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global,
+                                                       n_udof, u, ua, damp, op, pmap, nref);
+  else
+    k_vanka_apply_schur<F, false><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global,
+                                                        n_udof, u, ua, damp, op, pmap, 0);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_apply_schur<float>(const float *, const int *, int, int, const int *, const float *,
-                                              const int *, const int *, int, float *, float *, float, hipStream_t);
+                                              const int *, const int *, int, float *, float *, float,
+                                              const OpView<float> &, const int *, int, hipStream_t);
 template void launch_vanka_apply_schur<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
-                                                 const int *, const int *, int, float *, float *, float, hipStream_t);
+                                                 const int *, const int *, int, float *, float *, float,
+                                                 const OpView<float> &, const int *, int, hipStream_t);
 
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
                         const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
-                        hipStream_t s) {
+                        const OpView<float> &op, const int *pmap, const uint8_t *invperm, int nref, hipStream_t s) {
   if (ncells <= 0) return;
-  k_vanka_apply<F><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof, u, ua,
-                                         damp);
+  if (nref > 0)
+    k_vanka_apply<F, true><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof,
+                                                 u, ua, damp, op, pmap, invperm, nref);
+  else
+    k_vanka_apply<F, false><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global,
+                                                  n_udof, u, ua, damp, op, pmap, invperm, 0);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_apply<float>(const float *, const int *, int, int, const int *, const float *, const int *,
-                                        const int *, int, float *, float *, float, hipStream_t);
+                                        const int *, int, float *, float *, float, const OpView<float> &, const int *,
+                                        const uint8_t *, int, hipStream_t);
 template void launch_vanka_apply<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
-                                           const int *, const int *, int, float *, float *, float, hipStream_t);
+                                           const int *, const int *, int, float *, float *, float,
+                                           const OpView<float> &, const int *, const uint8_t *, int, hipStream_t);
 
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
-                           const int *items, const int *desc, const T *u, const float *ua, const T *f,
-                           T *rloc, hipStream_t s) {
+                           const int *items, const int *desc, const int *xg, const T *u, const float *ua,
+                           const T *f, T *rloc, hipStream_t s) {
   if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
-  const int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
+  int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
+  if (xg) grid = (grid + 7) / 8 * 8;
   if (op.a1)
-    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, ua, f, rloc);
+    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc);
   else
-    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, u, ua, f, rloc);
+    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,
-                                            const int *, const double *, const float *, const double *, double *,
-                                            hipStream_t);
+                                            const int *, const int *, const double *, const float *, const double *,
+                                            double *, hipStream_t);
 template void launch_vanka_residual<float>(const OpView<float> &, long, int, const uint8_t *, const int *,
-                                           const int *, const float *, const float *, const float *, float *,
-                                           hipStream_t);
+                                           const int *, const int *, const float *, const float *, const float *,
+                                           float *, hipStream_t);
 
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,

@@ -651,25 +651,32 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
     {
       const double rb = D.res_bytes[D.a_shared][vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0, &D == &c->L.back() ? &c->tm.vanka_res_finest : nullptr);
-      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, u, ua, f, rloc, c->stream);
+      const int *xg = !D.compacted && D.h_xcd_grouped[col] ? D.xcd_goff + 9 * col : nullptr;
+      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream);
     }
-    Timed tm(c, &c->tm.vanka_solve, nc * solve_bytes(fb, vs), nc * (2.0 * NLOC * NLOC));
+    // refinement steps (stored inverses only): the FP32 patch matrix gathered once per patch
+    // (value + map entry per entry), one more local product and residual per step
+    const int nref = sizeof(T) == 4 ? c->prm.bf16_refine_steps : 0;
+    const double kent = D.schur ? double(NU_LOC * NU_LOC + 2 * 3 * NU_LOC * NP_LOC) : double(NLOC * NLOC);
+    Timed tm(c, &c->tm.vanka_solve, nc * (solve_bytes(fb, vs) + (nref ? 8.0 * kent : 0.0)),
+             nc * (2.0 * NLOC * NLOC) * (1 + nref) + nc * 2.0 * kent * nref);
     if constexpr (sizeof(T) == 8) {
       launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                          D.n_udof, u, damp, c->stream);
     } else {
+      const OpView<float> o32 = D.op32();
       if (D.schur && D.minv16)
         launch_vanka_apply_schur<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                           D.n_udof, u, ua, damp, c->stream);
+                                           D.n_udof, u, ua, damp, o32, D.pmap, nref, c->stream);
       else if (D.schur && D.minv32)
         launch_vanka_apply_schur<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                        D.n_udof, u, ua, damp, c->stream);
+                                        D.n_udof, u, ua, damp, o32, D.pmap, nref, c->stream);
       else if (D.minv16)
         launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
-                                     u, ua, damp, c->stream);
+                                     u, ua, damp, o32, D.pmap, D.invperm, nref, c->stream);
       else if (D.minv32)
         launch_vanka_apply<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof, u,
-                                  ua, damp, c->stream);
+                                  ua, damp, o32, D.pmap, D.invperm, nref, c->stream);
       else if (c->prm.precision == PMG_PREC_FP32_BF16)
         launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                             D.n_udof, u, damp, c->stream);
@@ -1639,8 +1646,9 @@ static int check_params(const pmg_params *p, std::string &why) {
   if (p->sc_coarse_smoother_saddle != 17) { why = "only the dense coarse solve (17)"; return PMG_ERR_UNSUPPORTED; }
   if (p->precision < 0 || p->precision > 2) { why = "bad precision"; return PMG_ERR_INVALID_ARGUMENT; }
   if (p->sc_pre_smooth_saddle < 0 || p->sc_post_smooth_saddle < 0) { why = "negative smoothing steps"; return PMG_ERR_INVALID_ARGUMENT; }
-  if (p->bf16_refine_steps != 0) {
-    why = "local refinement steps are not provided (the outer FP64 FGMRES refines the solution)";
+  if (p->bf16_refine_steps < 0 || p->bf16_refine_steps > 3) { why = "bf16_refine_steps must be 0..3"; return PMG_ERR_INVALID_ARGUMENT; }
+  if (p->bf16_refine_steps > 0 && (p->precision == PMG_PREC_FP64 || !p->patch_inverse)) {
+    why = "local refinement steps apply to the stored patch inverses of the mixed forms (patch_inverse = 1)";
     return PMG_ERR_UNSUPPORTED;
   }
   if (p->sc_gmres_restart <= 0 || p->sc_lin_maxit_saddle < 0) { why = "bad FGMRES restart/maxit"; return PMG_ERR_INVALID_ARGUMENT; }
@@ -1825,23 +1833,41 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     {
       // residual work items of each color sorted by the operator row they stream
       std::vector<int> items(size_t(D.n_cells) * (NU_LOC + NP_LOC)), rows(items.size());
+      // Large colors of a single problem (>= XCD_GROUP_MIN cells) are split into 8 groups of
+      // consecutive cells (the color lists cells in ascending index, and cell numbers are
+      // spatially local), one per XCD: each group's items sorted by row, stored group after
+      // group, and the residual kernel runs group x on the workgroups that land on XCD x, so
+      // each XCD's L2 holds the u entries of one compact part of the mesh.
       std::vector<std::pair<int, int>> key;
+      std::vector<int> xg(size_t(nc) * 9, 0);
+      D.h_xcd_grouped.assign(nc, 0);
       for (int k = 0; k < nc; ++k) {
-        key.clear();
-        for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
-          const int cell = reo[q], slot = q - ptr[k];
-          for (int j = 0; j < NU_LOC; ++j) key.emplace_back(ug[size_t(cell) * NU_LOC + j], slot * 32 + j);
-          for (int j = 0; j < NP_LOC; ++j)
-            key.emplace_back(D.n_udof + pg[size_t(cell) * NP_LOC + j], slot * 32 + NU_LOC + j);
+        const int nck = ptr[k + 1] - ptr[k];
+        const bool grouped = !merged && nck >= XCD_GROUP_MIN;
+        const int ng = grouped ? 8 : 1;
+        D.h_xcd_grouped[k] = grouped;
+        size_t at = size_t(ptr[k]) * (NU_LOC + NP_LOC);
+        for (int gi = 0; gi < ng; ++gi) {
+          key.clear();
+          const int q0 = ptr[k] + int(long(nck) * gi / ng), q1 = ptr[k] + int(long(nck) * (gi + 1) / ng);
+          for (int q = q0; q < q1; ++q) {
+            const int cell = reo[q], slot = q - ptr[k];
+            for (int j = 0; j < NU_LOC; ++j) key.emplace_back(ug[size_t(cell) * NU_LOC + j], slot * 32 + j);
+            for (int j = 0; j < NP_LOC; ++j)
+              key.emplace_back(D.n_udof + pg[size_t(cell) * NP_LOC + j], slot * 32 + NU_LOC + j);
+          }
+          std::sort(key.begin(), key.end());
+          if (grouped) xg[size_t(k) * 9 + gi] = int(at - size_t(ptr[k]) * (NU_LOC + NP_LOC));
+          for (size_t i = 0; i < key.size(); ++i, ++at) {
+            items[at] = key[i].second;
+            // global row: velocity dof, or pressure dof (keys of pressure rows are offset by n_udof)
+            rows[at] = key[i].first < D.n_udof ? key[i].first : key[i].first - D.n_udof;
+          }
         }
-        std::sort(key.begin(), key.end());
-        for (size_t i = 0; i < key.size(); ++i) {
-          items[size_t(ptr[k]) * (NU_LOC + NP_LOC) + i] = key[i].second;
-          // global row: velocity dof, or pressure dof (keys of pressure rows are offset by n_udof)
-          rows[size_t(ptr[k]) * (NU_LOC + NP_LOC) + i] = key[i].first < D.n_udof ? key[i].first
-                                                                                 : key[i].first - D.n_udof;
-        }
+        if (grouped) xg[size_t(k) * 9 + 8] = nck * (NU_LOC + NP_LOC);
       }
+      D.xcd_goff = dalloc<int>(c, xg.size());
+      h2d(c, D.xcd_goff, xg.data(), xg.size());
       D.res_items = dalloc<int>(c, items.size());
       h2d(c, D.res_items, items.data(), items.size());
       D.res_rows = dalloc<int>(c, rows.size());

@@ -51,6 +51,8 @@ constexpr int PM_NB = PM_B + NP_LOC * NU_LOC;    // 999
 constexpr int PM_STRIDE = 1008;
 // nodal Vanka: velocity dofs of one B row (27 for Q2/P1disc), upper bound
 constexpr int NODAL_MAXU = 28;
+// colors of at least this many cells run their residual items in 8 per-XCD groups
+constexpr int XCD_GROUP_MIN = 512;
 static_assert(PM_NB + NP_LOC <= PM_STRIDE, "patch map stride");
 
 // Type-4 operator.  Two value layouts:
@@ -123,6 +125,10 @@ struct DevLevel {
   int *res_items = nullptr;
   int *res_rows = nullptr;           // global row of every work item
   int *res_desc = nullptr;           // its descriptor (row, A/B start, BT start, lengths)
+  // colors split into 8 groups of consecutive cells, one per XCD (host flag per color;
+  // device: 9 item offsets per color, relative to the color's first item)
+  std::vector<char> h_xcd_grouped;
+  int *xcd_goff = nullptr;
   // cells
   int *u_global = nullptr;           // n_cells * 27 (UGlobalNumbers + UBeginIndex[cell])
   int *p_global = nullptr;           // n_cells * 4
@@ -261,23 +267,27 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 // mixed precision, explicit inverses: FP32 Gauss-Jordan into minv32 or bf16 minv16
 void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s);
+// nref > 0: that many refinement steps with the FP32 patch matrix (op, pmap) after the
+// first product (residual permuted by invperm like the local residual)
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
                         const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
-                        hipStream_t s);
+                        const OpView<float> &op, const int *pmap, const uint8_t *invperm, int nref, hipStream_t s);
 // block form (shared velocity block): set-up into minv32 / minv16 with SCH_STRIDE per patch,
 // and its local solves
 void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s);
 template <typename F>
 void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
                               const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, hipStream_t s);
+                              float *ua, float damp, const OpView<float> &op, const int *pmap, int nref,
+                              hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
+// xg (may be null): the 9 item offsets of the color's 8 XCD groups
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
-                           const int *items, const int *desc, const T *u, const float *ua, const T *f,
-                           T *rloc, hipStream_t s);
+                           const int *items, const int *desc, const int *xg, const T *u, const float *ua,
+                           const T *f, T *rloc, hipStream_t s);
 // FP32 sweep start: Dirichlet slices of u from f, ua[4 i + c] = u_c(i) (c < 3)
 void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s);
 // descriptors of residual work items (4 ints each) from their items and global rows
