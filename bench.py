@@ -56,16 +56,25 @@ def solver_params(precision: int, shared_block: int | None = None, patch_inverse
                                bf16_refine_steps=refine_steps)
 
 
+AUTOTUNE = True
+
+
 def run_gpu(prob, precision: int, steps: int, warmup: int, log, shared_block=None, patch_inverse=None,
             solve_only_steps: int = 0, profile: bool = True, refine_steps: int = 0):
     """`steps` timed solves including the value-dependent set-up (pmg_refactor), then
     `solve_only_steps` solves with the factors kept (their difference is the set-up's cost
-    on the critical path), then one profiling pass."""
+    on the critical path), then one profiling pass.  Before the warm-up the launch shapes
+    are tuned on the device (pmg_autotune, once per hierarchy, outside the timed steps)."""
     from to_merge_master_parmoon_amd.multigrid import DeviceVector, build_hierarchy
     t0 = time.time()
     mg = build_hierarchy(prob, solver_params(precision, shared_block, patch_inverse, refine_steps))
     log(f"hierarchy on device ({time.time() - t0:.1f}s), levels {mg.n_dof}, "
         f"shared block {[mg.shared_block(l) for l in range(mg.n_levels)]}")
+    shapes = None
+    if AUTOTUNE:
+        t0 = time.time()
+        shapes = mg.autotune(20)
+        log(f"launch shapes tuned ({time.time() - t0:.2f}s): {shapes}")
     n = mg.n_dof[-1]
     rhs = DeviceVector(mg, n, prob.rhs(-1))
     zero = DeviceVector(mg, n, np.zeros(n))
@@ -110,7 +119,7 @@ def run_gpu(prob, precision: int, steps: int, warmup: int, log, shared_block=Non
         step()
         mg.set_profiling(False)
         tm = mg.timers()
-    return dict(times=times, iters=iters, x=x, hist=hist, timers=tm, mg=mg, solve_only=solve_only)
+    return dict(times=times, iters=iters, x=x, hist=hist, timers=tm, mg=mg, solve_only=solve_only, shapes=shapes)
 
 
 def batched_problems(count: int, log):
@@ -136,6 +145,9 @@ def batched_solve(probs, precision: int, maxit: int, steps: int, warmup: int, lo
     t = time.time()
     B = NSEBatch(probs, params=sp_)
     log(f"batch context ({time.time() - t:.1f}s)")
+    if AUTOTUNE:
+        B.refactor()
+        log(f"batch launch shapes tuned: {B.autotune(20)}")
     rh = [DeviceVector(B, ndof[i], probs[i].rhs(-1)) for i in range(len(probs))]
     zs = [DeviceVector(B, ndof[i], np.zeros(ndof[i])) for i in range(len(probs))]
     nb = len(probs)
@@ -477,6 +489,7 @@ def main():
     ap.add_argument("--no-nine-blocks", action="store_true", help="skip the nine-block timings")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload's fields")
     ap.add_argument("--no-refine", action="store_true", help="skip the timings with local refinement steps")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--workload", choices=["oseen32", "batched", "reassembly", "channel"], default="oseen32",
                     help="batched: the independent-problems workload; reassembly: one fixed-point step's "
@@ -491,8 +504,9 @@ def main():
     ap.add_argument("--nine-blocks", action="store_true",
                     help="stream all nine A blocks even where A11 = A22 = A33 and the coupling blocks vanish")
     args = ap.parse_args()
-    global USE_GRAPH, SHARED_BLOCK, PATCH_INVERSE
+    global USE_GRAPH, SHARED_BLOCK, PATCH_INVERSE, AUTOTUNE
     USE_GRAPH = 0 if args.no_graph else 1
+    AUTOTUNE = not args.no_autotune
     SHARED_BLOCK = 0 if args.nine_blocks else 1
     PATCH_INVERSE = 0 if args.lu_sweeps else 1
     log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))
@@ -540,7 +554,8 @@ def main():
            "solve_with_factors_kept_s": round(solve_only_ms / 1e3, 5),
            "setup_on_critical_path_ms": round(ms - solve_only_ms, 3),
            "patch_setup_ms": round(g["timers"]["patch_lu"]["ms"], 3),
-           "coarse_setup_ms_second_stream": round(g["timers"]["coarse_lu"]["ms"], 3)}
+           "coarse_setup_ms_second_stream": round(g["timers"]["coarse_lu"]["ms"], 3),
+           "autotuned_launch_shapes": g["shapes"]}
     if ref is not None:
         fp64_ms = 1e3 * float(np.mean(ref["times"]))
         du, dp = rel_err(g["x"], ref["x"], n_u3)

@@ -394,6 +394,20 @@ typedef struct {
   double sweep_ms; long long sweep_calls;   /* whole colored sweeps                       */
   pmg_kstat vanka_res_finest;  /* the per-color local residuals of the finest level alone */
 } pmg_timers;
+/* Launch-shape autotuner (no reference counterpart; the reference's kernels have fixed
+   shapes): after pmg_setup, times the candidate launch shapes of every smoothed level's
+   kernels on the device, reps launches each, and keeps the fastest per level: work items
+   per workgroup of the Vanka residual (16, 32, 64), whether colors of >= 512 cells run in
+   per-XCD groups, operator rows per workgroup of the type-4 defect / matvec (32 .. 256).
+   The shapes change how work is distributed, not what is computed (only the summation
+   order of the pressure means differs).  Uses the levels' own work vectors.
+   pmg_level_tuning returns {residual items per workgroup FP64, FP32; XCD groups FP64,
+   FP32; operator rows per workgroup FP64, FP32} (0: built-in default). */
+int  pmg_autotune(pmg_ctx *ctx, int reps);
+int  pmg_level_tuning(pmg_ctx *ctx, int level, int *out6);
+/* sets a level's launch shapes (the six values of pmg_level_tuning, e.g. saved from an
+   earlier pmg_autotune); 0 entries mean the built-in default */
+int  pmg_set_level_tuning(pmg_ctx *ctx, int level, const int *in6);
 int  pmg_timers_get(const pmg_ctx *ctx, pmg_timers *t);
 int  pmg_timers_reset(pmg_ctx *ctx);
 int  pmg_set_profiling(pmg_ctx *ctx, int on);

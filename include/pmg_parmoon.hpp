@@ -11,12 +11,25 @@
 //    instead of the global ParamDB;
 //  * where the reference calls exit(4711) or prints an error, these classes throw
 //    pmg_parmoon::Error carrying the pmg status code.
+//
+// The free functions MatVect_NSE4 / Defect_NSE4 (the MatVecProc / DefectProc signatures of
+// include/General/Constants.h:159-160), DefectRestriction / Prolongate (the TFESpace3D
+// signatures of MGComponents3D.C:184-1240) and TFgmresIte::Iterate(A, B, sol, rhs)
+// (ItMethod.h:105) also exist with the reference's arguments: the caller's matrix and space
+// pointers are keys of the levels registered with them (AddLevel(desc, A, B, U, P)); they
+// are only compared, never dereferenced, so ParMooN's classes stay declared, not defined.
 #pragma once
+#include <algorithm>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "pmg.h"
+
+class TSquareMatrix;
+class TMatrix;
+class TFESpace3D;
 
 namespace pmg_parmoon {
 
@@ -37,6 +50,23 @@ inline void check(pmg_ctx *ctx, int st, const char *where) {
 
 class TNSE_MultiGrid;
 
+// level lookup of the reference-signature entry points: matrix / space pointer -> level
+struct LevelKey {
+  TNSE_MultiGrid *mg;
+  int level;
+  int kind;   // 0: the level's first square matrix, 1: velocity space, 2: pressure space
+};
+inline std::map<const void *, LevelKey> &level_registry() {
+  static std::map<const void *, LevelKey> r;
+  return r;
+}
+inline LevelKey lookup(const void *key, int kind, const char *where) {
+  auto it = level_registry().find(key);
+  if (it == level_registry().end() || it->second.kind != kind)
+    throw Error(PMG_ERR_INVALID_ARGUMENT, std::string(where) + ": matrix or space not registered with AddLevel");
+  return it->second;
+}
+
 // One level: the TNSE_MGLevel4 operations on that level's vectors.  Vectors are laid out as
 // in the reference ([u1 | u2 | u3 | p], n_dof = 3 N_UDOF + N_PDOF) and live in host memory
 // unless the multigrid was created with device vectors.
@@ -47,9 +77,10 @@ class TNSE_MGLevel4 {
   void Defect(double *u1, double *f1, double *d1, double &res);
   // TNSE_MGLevel4::CorrectNodes (NSE_MGLevel4.C:542-601)
   void CorrectNodes(double *u1);
-  // TNSE_MGLevel4::CellVanka (NSE_MGLevel4.C:603-1649): one colored sweep; aux, the
-  // parameter array, smoother and N_Levels are accepted for signature compatibility
-  // (the sweep keeps its old iterate internally; smoother 1/2 are both the cell Vanka)
+  // TNSE_MGLevel4::CellVanka (NSE_MGLevel4.C:603-1649) / NodalVanka (:4098-4549): one
+  // sweep of the smoother the multigrid was created with (sc_smoother_saddle 1-4); aux,
+  // the parameter array, smoother and N_Levels are accepted for signature compatibility
+  // (the sweep keeps its old iterate internally)
   void CellVanka(double *u1, double *rhs1, double *aux, int N_Parameters, double *Parameters, int smoother,
                  int N_Levels);
   // TNSE_MGLevel4::SolveExact (NSE_MGLevel4.C:5728-5846), coarsest level only
@@ -75,6 +106,8 @@ class TNSE_MultiGrid {
     check(nullptr, pmg_ctx_create(device, &params, &ctx_), "pmg_ctx_create");
   }
   ~TNSE_MultiGrid() {
+    auto &r = level_registry();
+    for (auto it = r.begin(); it != r.end();) it = it->second.mg == this ? r.erase(it) : std::next(it);
     if (ctx_) pmg_ctx_destroy(ctx_);
   }
   TNSE_MultiGrid(const TNSE_MultiGrid &) = delete;
@@ -85,8 +118,23 @@ class TNSE_MultiGrid {
     int lvl = -1;
     check(ctx_, pmg_level_add(ctx_, &desc, &lvl), "AddLevel");
     levels_.emplace_back(*this, lvl);
+    n_udof_.push_back(desc.n_udof);
+    n_pdof_.push_back(desc.n_pdof);
     return lvl;
   }
+  // the same, registering the level's sqmatrices / matrices arrays (first entries) and FE
+  // spaces for the reference-signature entry points
+  int AddLevel(const pmg_level_desc &desc, TSquareMatrix **A, TMatrix **B, TFESpace3D *U, TFESpace3D *P) {
+    const int lvl = AddLevel(desc);
+    auto &r = level_registry();
+    if (A && A[0]) r[A[0]] = LevelKey{this, lvl, 0};
+    if (U) r[U] = LevelKey{this, lvl, 1};
+    if (P) r[P] = LevelKey{this, lvl, 2};
+    (void)B;
+    return lvl;
+  }
+  int N_UDOF(int level) const { return n_udof_.at(level); }
+  int N_PDOF(int level) const { return n_pdof_.at(level); }
   TNSE_MGLevel4 *GetLevel(int i) { return &levels_.at(i); }
   int GetN_Levels() const { return int(levels_.size()); }
   void Setup() { check(ctx_, pmg_setup(ctx_), "Setup"); }
@@ -110,6 +158,7 @@ class TNSE_MultiGrid {
   pmg_ctx *ctx_ = nullptr;
   int mem_;
   std::vector<TNSE_MGLevel4> levels_;
+  std::vector<int> n_udof_, n_pdof_;
 };
 
 // TFgmresIte (FgmresIte.C:75-677), flexible branch, preconditioned by the multigrid.
@@ -127,6 +176,14 @@ class TFgmresIte {
     converged_ = st == PMG_OK;
     if (st != PMG_OK && st != PMG_ERR_NOT_CONVERGED) check(mg_->ctx(), st, "TFgmresIte::Iterate");
     return it;
+  }
+  // the reference's signature (ItMethod.h:105): A must be the finest level's sqmatrices
+  int Iterate(TSquareMatrix **A, TMatrix **B, double *sol, double *rhs) {
+    const LevelKey k = lookup(A ? A[0] : nullptr, 0, "TFgmresIte::Iterate");
+    if (k.mg != mg_ || k.level != mg_->GetN_Levels() - 1)
+      throw Error(PMG_ERR_INVALID_ARGUMENT, "TFgmresIte::Iterate: not the finest level of this multigrid");
+    (void)B;
+    return Iterate(sol, rhs);
   }
   double FinalResidual() const { return final_res_; }
   bool Converged() const { return converged_; }
@@ -157,6 +214,68 @@ inline void Prolongate(TNSE_MultiGrid &mg, int fine, double *u_coarse, double *d
   check(mg.ctx(), pmg_prolongate(mg.ctx(), fine, u_coarse, d_fine, mg.mem()), "Prolongate");
 }
 
+// The reference's signatures.  MatVect_NSE4 / Defect_NSE4: A = the level's sqmatrices
+// (A11..A33), B = its matrices (B1T..B3T, B1..B3), vectors [u1 | u2 | u3 | p] of that level.
+inline void MatVect_NSE4(TSquareMatrix **A, TMatrix **B, double *x, double *y) {
+  const LevelKey k = lookup(A ? A[0] : nullptr, 0, "MatVect_NSE4");
+  (void)B;
+  MatVect_NSE4(*k.mg, k.level, x, y);
+}
+inline void Defect_NSE4(TSquareMatrix **A, TMatrix **B, double *x, double *b, double *r) {
+  const LevelKey k = lookup(A ? A[0] : nullptr, 0, "Defect_NSE4");
+  (void)B;
+  Defect_NSE4(*k.mg, k.level, x, b, r);
+}
+namespace detail {
+// one component block (velocity: N_Functions = 3 blocks of N_UDOF; pressure: 1 of N_PDOF)
+// of a transfer, through the whole-vector entry point; host vectors only
+inline void transfer_part(TFESpace3D *coarse, TFESpace3D *fine, int n_functions, double *coarse_f, double *fine_f,
+                          bool restrict_, const char *where) {
+  const auto it = level_registry().find(fine);
+  if (it == level_registry().end() || it->second.kind == 0)
+    throw Error(PMG_ERR_INVALID_ARGUMENT, std::string(where) + ": fine space not registered with AddLevel");
+  const LevelKey f = it->second;
+  const auto ic = level_registry().find(coarse);
+  if (ic == level_registry().end() || ic->second.mg != f.mg || ic->second.kind != f.kind ||
+      ic->second.level != f.level - 1)
+    throw Error(PMG_ERR_INVALID_ARGUMENT, std::string(where) + ": coarse space is not the next coarser level");
+  TNSE_MultiGrid &mg = *f.mg;
+  if (mg.mem() != PMG_MEM_HOST)
+    throw Error(PMG_ERR_UNSUPPORTED, std::string(where) + ": the space form takes host vectors");
+  const bool vel = f.kind == 1;
+  if (n_functions != (vel ? 3 : 1))
+    throw Error(PMG_ERR_INVALID_ARGUMENT, std::string(where) + ": N_Functions must be 3 (velocity) or 1 (pressure)");
+  const int lf = f.level, lc = f.level - 1;
+  const size_t nf = 3 * size_t(mg.N_UDOF(lf)) + mg.N_PDOF(lf), nc = 3 * size_t(mg.N_UDOF(lc)) + mg.N_PDOF(lc);
+  const size_t of = vel ? 0 : 3 * size_t(mg.N_UDOF(lf)), oc = vel ? 0 : 3 * size_t(mg.N_UDOF(lc));
+  const size_t lenf = vel ? 3 * size_t(mg.N_UDOF(lf)) : size_t(mg.N_PDOF(lf));
+  const size_t lenc = vel ? 3 * size_t(mg.N_UDOF(lc)) : size_t(mg.N_PDOF(lc));
+  std::vector<double> F(nf, 0.0), Cv(nc, 0.0);
+  if (restrict_) {
+    std::copy(fine_f, fine_f + lenf, F.begin() + of);
+    check(mg.ctx(), pmg_restrict_defect(mg.ctx(), lf, F.data(), Cv.data(), PMG_MEM_HOST), where);
+    std::copy(Cv.begin() + oc, Cv.begin() + oc + lenc, coarse_f);
+  } else {
+    std::copy(coarse_f, coarse_f + lenc, Cv.begin() + oc);
+    check(mg.ctx(), pmg_prolongate(mg.ctx(), lf, Cv.data(), F.data(), PMG_MEM_HOST), where);
+    std::copy(F.begin() + of, F.begin() + of + lenf, fine_f);
+  }
+}
+}  // namespace detail
+// DefectRestriction (MGComponents3D.C:711-1240): CoarseFunction := R FineFunction for the
+// velocity space (N_Functions = 3) or the pressure space (N_Functions = 1); aux unused
+inline void DefectRestriction(TFESpace3D *CoarseSpace, TFESpace3D *FineSpace, int N_Functions, double *CoarseFunction,
+                              double *FineFunction, double *aux) {
+  (void)aux;
+  detail::transfer_part(CoarseSpace, FineSpace, N_Functions, CoarseFunction, FineFunction, true, "DefectRestriction");
+}
+// Prolongate (MGComponents3D.C:184-708): FineFunction := P CoarseFunction; aux unused
+inline void Prolongate(TFESpace3D *CoarseSpace, TFESpace3D *FineSpace, int N_Functions, double *CoarseFunction,
+                       double *FineFunction, double *aux) {
+  (void)aux;
+  detail::transfer_part(CoarseSpace, FineSpace, N_Functions, CoarseFunction, FineFunction, false, "Prolongate");
+}
+
 // ------------------------------------------------------------------ TNSE_MGLevel4 bodies
 inline void TNSE_MGLevel4::Defect(double *u1, double *f1, double *d1, double &res) {
   check(mg_->ctx(), pmg_defect(mg_->ctx(), level_, u1, f1, d1, &res, mg_->mem()), "Defect");
@@ -165,7 +284,7 @@ inline void TNSE_MGLevel4::CorrectNodes(double *u1) {
   check(mg_->ctx(), pmg_correct_nodes(mg_->ctx(), level_, u1, mg_->mem()), "CorrectNodes");
 }
 inline void TNSE_MGLevel4::CellVanka(double *u1, double *rhs1, double *, int, double *, int smoother, int) {
-  if (smoother != 1 && smoother != 2) throw Error(PMG_ERR_UNSUPPORTED, "CellVanka: smoother must be 1 or 2");
+  if (smoother < 1 || smoother > 4) throw Error(PMG_ERR_UNSUPPORTED, "CellVanka: smoother must be 1 .. 4");
   check(mg_->ctx(), pmg_vanka_smooth(mg_->ctx(), level_, u1, rhs1, 1, mg_->mem()), "CellVanka");
 }
 inline void TNSE_MGLevel4::SolveExact(double *u1, double *rhs1) {

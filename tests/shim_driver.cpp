@@ -4,7 +4,12 @@
 //   shim_driver solve N OUT     Oseen N^3 hierarchy (levels 4^3..N^3), FGMRES(30) to 1e-10
 //                               through TNSE_MultiGrid / TFgmresIte, one level's Defect and
 //                               CellVanka through TNSE_MGLevel4; writes the solution to OUT
-//                               and prints "iterations <j> residual <r>"
+//                               and prints "iterations <j> residual <r>"; the same
+//                               operations through the reference's signatures (matrix and
+//                               space pointers registered with AddLevel) must agree bit for
+//                               bit: prints "dropin <max difference>"
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -46,12 +51,24 @@ int main(int argc, char **argv) {
   cfg.seed_rhs = 1002;
   pmgp_problem *prob = nullptr;
   if (pmgp_create(&cfg, &prob) != 0) return 3;
+  // stand-ins for ParMooN's matrix and space objects: only their addresses are used
+  std::vector<char> objs(size_t(levels) * 4);
+  std::vector<TSquareMatrix *> sq(size_t(levels) * 9);
+  std::vector<TMatrix *> mt(size_t(levels) * 6);
+  auto sqm = [&](int l) { return sq.data() + 9 * l; };
+  auto mat = [&](int l) { return mt.data() + 6 * l; };
+  auto uspace = [&](int l) { return reinterpret_cast<TFESpace3D *>(&objs[4 * l + 1]); };
+  auto pspace = [&](int l) { return reinterpret_cast<TFESpace3D *>(&objs[4 * l + 2]); };
+  for (int l = 0; l < levels; ++l) {
+    for (int b = 0; b < 9; ++b) sqm(l)[b] = reinterpret_cast<TSquareMatrix *>(&objs[4 * l]);
+    for (int b = 0; b < 6; ++b) mat(l)[b] = reinterpret_cast<TMatrix *>(&objs[4 * l + 3]);
+  }
   try {
     pmg_parmoon::TNSE_MultiGrid mg(prm);
     for (int l = 0; l < levels; ++l) {
       pmg_level_desc d;
       pmgp_level_desc(prob, l, &d);
-      mg.AddLevel(d);
+      mg.AddLevel(d, sqm(l), mat(l), uspace(l), pspace(l));
     }
     mg.Setup();
     const int nd = pmgp_n_dof(prob, levels - 1);
@@ -64,8 +81,38 @@ int main(int argc, char **argv) {
     fine->CellVanka(u.data(), rhs.data(), aux.data(), 0, nullptr, 2, levels);
     fine->Defect(u.data(), rhs.data(), d.data(), res1);
     std::printf("defect %.6e after_sweep %.6e\n", res0, res1);
+    {
+      // the reference's signatures against the level-index forms
+      const int lf = levels - 1, lc = levels - 2;
+      const int ndc = pmgp_n_dof(prob, lc);
+      double diff = 0;
+      auto cmp = [&](const std::vector<double> &a, const std::vector<double> &b) {
+        for (size_t i = 0; i < a.size(); ++i) diff = std::max(diff, std::fabs(a[i] - b[i]));
+      };
+      std::vector<double> x(nd), y1(nd), y2(nd);
+      for (int i = 0; i < nd; ++i) x[i] = std::sin(0.37 * i);
+      pmg_parmoon::MatVect_NSE4(mg, lf, x.data(), y1.data());
+      pmg_parmoon::MatVect_NSE4(sqm(lf), mat(lf), x.data(), y2.data());
+      cmp(y1, y2);
+      pmg_parmoon::Defect_NSE4(mg, lf, x.data(), rhs.data(), y1.data());
+      pmg_parmoon::Defect_NSE4(sqm(lf), mat(lf), x.data(), rhs.data(), y2.data());
+      cmp(y1, y2);
+      std::vector<double> c1(ndc), c2(ndc), a(nd);
+      pmg_parmoon::DefectRestriction(mg, lf, x.data(), c1.data());
+      const int nuf = 3 * mg.N_UDOF(lf), nuc = 3 * mg.N_UDOF(lc);
+      pmg_parmoon::DefectRestriction(uspace(lc), uspace(lf), 3, c2.data(), x.data(), a.data());
+      pmg_parmoon::DefectRestriction(pspace(lc), pspace(lf), 1, c2.data() + nuc, x.data() + nuf, a.data());
+      cmp(c1, c2);
+      std::vector<double> xc(ndc);
+      for (int i = 0; i < ndc; ++i) xc[i] = std::cos(0.21 * i);
+      pmg_parmoon::Prolongate(mg, lf, xc.data(), y1.data());
+      pmg_parmoon::Prolongate(uspace(lc), uspace(lf), 3, xc.data(), y2.data(), a.data());
+      pmg_parmoon::Prolongate(pspace(lc), pspace(lf), 1, xc.data() + nuc, y2.data() + nuf, a.data());
+      cmp(y1, y2);
+      std::printf("dropin %.3e\n", diff);
+    }
     pmg_parmoon::TFgmresIte gm(mg);
-    const int it = gm.Iterate(sol.data(), rhs.data());
+    const int it = gm.Iterate(sqm(levels - 1), mat(levels - 1), sol.data(), rhs.data());
     std::printf("iterations %d residual %.6e converged %d\n", it, gm.FinalResidual(), int(gm.Converged()));
     FILE *f = std::fopen(argv[3], "wb");
     if (!f) return 4;

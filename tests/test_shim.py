@@ -44,6 +44,9 @@ def test_shim_solve_matches_oracle(driver, tmp_path):
     d0, d1 = (float(v) for v in lines["defect"].split()[::2])
     # one sweep from zero changes the level defect (its norm need not drop in one sweep)
     assert np.isfinite(d0) and np.isfinite(d1) and d1 != d0
+    # MatVect_NSE4, Defect_NSE4, DefectRestriction and Prolongate with the reference's
+    # arguments (matrix / space pointers) equal the level-index forms bit for bit
+    assert float(lines["dropin"]) == 0.0
     it = int(lines["iterations"].split()[0])
     x = np.fromfile(out, dtype=np.float64)
     prob = problem.oseen3d(8)

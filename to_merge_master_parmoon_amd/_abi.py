@@ -139,6 +139,9 @@ PMG_FUNCS = {
     "pmg_color_cells": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, C.c_int, c_int_p, c_int_p, c_int_p]),
     "pmg_sequential_schedule": (C.c_int, [C.c_int, c_int_p, c_int_p, C.c_int, C.c_int, c_int_p, c_int_p, c_int_p,
                                           c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
+    "pmg_autotune": (C.c_int, [C.c_void_p, C.c_int]),
+    "pmg_level_tuning": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
+    "pmg_set_level_tuning": (C.c_int, [C.c_void_p, C.c_int, c_int_p]),
     "pmg_nodal_schedule": (C.c_int, [C.c_int, C.c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p,
                                      c_int_p, c_int_p]),
     "pmg_level_read": (C.c_int, [C.c_char_p, C.POINTER(PmgLevelDesc), C.POINTER(C.c_void_p)]),

@@ -313,7 +313,7 @@ inline int grid_for(long n, int bs = 256, int cap = 8192) {
 template <typename T>
 void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project, double *res,
                   RedScratch &red, hipStream_t s) {
-  const int grid = chunk_grid(long(op.n_udof) + op.n_pdof, 64, RED_BLOCKS);
+  const int grid = chunk_grid(long(op.n_udof) + op.n_pdof, op.rpb > 0 ? op.rpb : 64, RED_BLOCKS);
   const int want_norm = res != nullptr;
   const int proj = (mode != 0 && project) ? 1 : 0;
   auto go = [&](auto mode_c, int pr, int wn, double *rs, double *sl) {

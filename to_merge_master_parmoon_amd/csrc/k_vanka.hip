@@ -347,7 +347,8 @@ __host__ __device__ constexpr int minv_off(int r, int c) {
 template <typename F>
 __global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const int *__restrict__ cells,
                                                    const int *__restrict__ pmap, int ncells, F *__restrict__ minv,
-                                                   uint8_t *__restrict__ invperm, int *__restrict__ info) {
+                                                   uint8_t *__restrict__ invperm, int *__restrict__ info,
+                                                   float *__restrict__ kref) {
   __shared__ int sigma[NLOC];
   __shared__ float lbuf[96], ubuf[96];
   __shared__ float stage[16][97];
@@ -364,6 +365,17 @@ __global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const in
   for (int a = 0; a < 6; ++a)
 #pragma unroll
     for (int b = 0; b < 6; ++b) A[a][b] = patch_entry<float>(op, mp, ty + 16 * a, tx + 16 * b);
+  if (kref) {
+    // the FP32 patch matrix for the refinement steps, row-major 85 x 85
+    float *kr = kref + long(pos) * KREF_FULL;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = 0; b < 6; ++b) {
+        const int r = ty + 16 * a, cc = tx + 16 * b;
+        if (r < NLOC && cc < NLOC) kr[r * NLOC + cc] = A[a][b];
+      }
+  }
   __syncthreads();
   gj_columns<0>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
   gj_columns<1>(A, t, tx, ty, sigma, lbuf, ubuf, piv_s, nz_s, info_s);
@@ -403,7 +415,8 @@ __global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const in
 template <typename F>
 __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int *__restrict__ cells,
                                                      const int *__restrict__ pmap, int ncells, F *__restrict__ sch,
-                                                     uint8_t *__restrict__ invperm, int *__restrict__ info) {
+                                                     uint8_t *__restrict__ invperm, int *__restrict__ info,
+                                                     float *__restrict__ kref) {
   __shared__ float A[NU_LOC][NU_LOC + 1];
   __shared__ float BT[3][NU_LOC][NP_LOC];
   __shared__ float Bm[3][NP_LOC][NU_LOC];
@@ -427,6 +440,16 @@ __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int
     Bm[c][k][j] = patch_entry<float>(op, mp, 3 * NU_LOC + k, NU_LOC * c + j);
   }
   __syncthreads();
+  if (kref) {
+    // the FP32 block form for the refinement steps: A row-major, BT_c[j][k], B_c[k][j]
+    float *kr = kref + long(pos) * KREF_BLK;
+    for (int e = t; e < NU_LOC * NU_LOC; e += 128) kr[e] = A[e / NU_LOC][e % NU_LOC];
+    for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
+      const int c = e / (NU_LOC * NP_LOC), q = e - c * NU_LOC * NP_LOC;
+      kr[KREF_BT + e] = BT[c][q / NP_LOC][q % NP_LOC];
+      kr[KREF_B + e] = Bm[c][q / NU_LOC][q % NU_LOC];
+    }
+  }
   for (int k = 0; k < NU_LOC; ++k) {
     if (t < 64) {
       float bv = -1.0f;
@@ -890,21 +913,21 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
 // Local solve with the explicit inverse M = (P A)^{-1}, one wavefront per patch: lane i
 // holds rows i and 64 + i of M, the permuted local residual P r is broadcast by lane reads,
 // x = M (P r) = A^{-1} r in the natural order, u += damp x.
-// REF: nref refinement steps: rr = r - K x with the FP32 patch matrix K (gathered from the
-// FP32 operator through the patch map into LDS once), permuted like the residual (row i to
-// position invperm[i]) and multiplied by M again, x += M (P rr).
+// REF: nref refinement steps: rr = r - K x with the FP32 patch matrix K (stored at set-up,
+// staged in LDS once), permuted like the residual (row i to position invperm[i]) and
+// multiplied by M again, x += M (P rr).
 template <typename F, bool REF>
 __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, const int *__restrict__ cells,
                                                     int ncells, int pos0, const int *__restrict__ slots,
                                                     const float *__restrict__ rloc, const int *__restrict__ u_global,
                                                     const int *__restrict__ p_global, int N, float *__restrict__ u,
-                                                    float *__restrict__ ua, float damp, OpView<float> op,
-                                                    const int *__restrict__ pmap, const uint8_t *__restrict__ invperm,
-                                                    int nref) {
+                                                    float *__restrict__ ua, float damp,
+                                                    const float *__restrict__ kref,
+                                                    const uint8_t *__restrict__ invperm, int nref) {
   constexpr int NV = MINV_LD * int(sizeof(F)) / 16;   // 16-byte vectors per row
   constexpr int PER = 16 / int(sizeof(F));             // entries per vector
-  constexpr int KLD = NLOC + 1;
-  __shared__ float Kf[REF ? NLOC * KLD : 1];
+  constexpr int KLD = NLOC;   // odd row stride: lane i reading row i is conflict-free
+  __shared__ __align__(16) float Kf[REF ? KREF_FULL : 4];
   __shared__ float xs[REF ? NLOC : 1], rp[REF ? RLOC_STRIDE : 1];
   const int lane = threadIdx.x;
   if (int(blockIdx.x) >= ncells) return;
@@ -946,10 +969,24 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
   const float r1 = two ? rs[64 + lane] : 0.0f;
   if constexpr (REF) {
     if (nref > 0) {
-      const int *__restrict__ pm = pmap + long(cell) * PM_STRIDE;
-      for (int e = lane; e < NLOC * NLOC; e += 64) {
-        const int i = e / NLOC, m = e - NLOC * i;
-        Kf[i * KLD + m] = patch_entry<float>(op, pm, i, m);
+      // the stored matrix has the LDS layout: a straight 16-byte copy, 8 loads in flight per
+      // lane before their stores
+      const v4u *__restrict__ kr = reinterpret_cast<const v4u *>(kref + long(pos) * KREF_FULL);
+      v4u *kl = reinterpret_cast<v4u *>(Kf);
+      constexpr int NQ = KREF_FULL / 4, PL = (NQ + 63) / 64;
+#pragma unroll
+      for (int b0 = 0; b0 < PL; b0 += 8) {
+        v4u tq[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int q = lane + 64 * (b0 + k);
+          if (b0 + k < PL && q < NQ) tq[k] = __builtin_nontemporal_load(kr + q);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int q = lane + 64 * (b0 + k);
+          if (b0 + k < PL && q < NQ) kl[q] = tq[k];
+        }
       }
     }
   }
@@ -1024,8 +1061,8 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
 // components), g = r_p - sum_c B_c y_c, p = S^{-1} g (lanes 0..3), x_c = y_c - W_c p, and
 // u += damp * (x, p) (with the interleaved velocity copy when ua is given).
 // REF: nref refinement steps after the first solve: the local residual r - K x with the
-// FP32 patch matrix K (A, BT_c and B_c gathered from the FP32 operator through the patch
-// map, as the set-up gathers them), solved with the stored data and added to x.
+// FP32 patch matrix K (A, BT_c and B_c as the set-up gathered them, stored beside the
+// inverse data), solved with the stored data and added to x.
 template <typename F, bool REF>
 __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ sch, const int *__restrict__ cells,
                                                           int ncells, int pos0, const int *__restrict__ slots,
@@ -1033,12 +1070,12 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
                                                           const int *__restrict__ u_global,
                                                           const int *__restrict__ p_global, int N,
                                                           float *__restrict__ u, float *__restrict__ ua, float damp,
-                                                          OpView<float> op, const int *__restrict__ pmap, int nref) {
+                                                          const float *__restrict__ kref, int nref) {
   constexpr int PER = 16 / int(sizeof(F));
   constexpr int NVEC = SCH_STRIDE / PER;
   __shared__ float M[SCH_STRIDE];
   __shared__ float r[RLOC_STRIDE], y[3][NU_LOC], g[NP_LOC], pp[NP_LOC];
-  __shared__ float Kf[REF ? NU_LOC * (NU_LOC + 1) + 2 * 3 * NU_LOC * NP_LOC : 1];   // A, BT_c, B_c (FP32)
+  __shared__ __align__(16) float Kf[REF ? KREF_BLK : 4];   // A, BT_c, B_c (FP32), the stored layout
   __shared__ float xs[REF ? NLOC : 1], rr[REF ? NLOC : 1];
   const int lane = threadIdx.x;
   if (int(blockIdx.x) >= ncells) return;
@@ -1062,19 +1099,21 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
   r[lane] = rs[lane];
   if (lane < NLOC - 64) r[64 + lane] = rs[64 + lane];
   const int cell = cells[pos];
-  constexpr int KA = 0, KBT = NU_LOC * (NU_LOC + 1), KB = KBT + 3 * NU_LOC * NP_LOC;
+  constexpr int KA = 0, KBT = KREF_BT, KB = KREF_B;
   if constexpr (REF) {
     if (nref > 0) {
-      const int *__restrict__ mp = pmap + long(cell) * PM_STRIDE;
-      for (int e = lane; e < NU_LOC * NU_LOC; e += 64) {
-        const int j = e / NU_LOC, m = e - NU_LOC * j;
-        Kf[KA + j * (NU_LOC + 1) + m] = patch_entry<float>(op, mp, j, m);
-      }
-      for (int e = lane; e < 3 * NU_LOC * NP_LOC; e += 64) {
-        const int c = e / (NU_LOC * NP_LOC), q = e - c * NU_LOC * NP_LOC, j = q / NP_LOC, k = q - NP_LOC * j;
-        Kf[KBT + e] = patch_entry<float>(op, mp, NU_LOC * c + j, 3 * NU_LOC + k);       // BT_c[j][k]
-        Kf[KB + (c * NP_LOC + k) * NU_LOC + j] = patch_entry<float>(op, mp, 3 * NU_LOC + k, NU_LOC * c + j);
-      }
+      // A row-major with the odd stride 27 (conflict-free row reads), BT_c[j][k], B_c[k][j]:
+      // the stored layout, copied in 16-byte pieces
+      const v4u *__restrict__ kr = reinterpret_cast<const v4u *>(kref + long(pos) * KREF_BLK);
+      v4u *kl = reinterpret_cast<v4u *>(Kf);
+      constexpr int NQ = KREF_BLK / 4, PL = (NQ + 63) / 64;
+      v4u tq[PL];
+#pragma unroll
+      for (int k = 0; k < PL; ++k)
+        if (lane + 64 * k < NQ) tq[k] = __builtin_nontemporal_load(kr + lane + 64 * k);
+#pragma unroll
+      for (int k = 0; k < PL; ++k)
+        if (lane + 64 * k < NQ) kl[lane + 64 * k] = tq[k];
     }
   }
   int g0 = 0, op_ = 0;
@@ -1153,7 +1192,7 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
       if (lane < NU_LOC) {
         float a0 = r[lane], a1 = r[NU_LOC + lane], a2 = r[2 * NU_LOC + lane];
         for (int m = 0; m < NU_LOC; ++m) {
-          const float w = Kf[KA + lane * (NU_LOC + 1) + m];
+          const float w = Kf[KA + lane * NU_LOC + m];
           a0 = fmaf(-w, xs[m], a0);
           a1 = fmaf(-w, xs[NU_LOC + m], a1);
           a2 = fmaf(-w, xs[2 * NU_LOC + m], a2);
@@ -1220,77 +1259,77 @@ void launch_res_desc(const DevLevel &L, const int *items, const int *rows, long 
   PMG_HIP_CHECK(hipGetLastError());
 }
 
-void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s) {
+void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s, float *kref) {
   if (L.n_cells <= 0) return;
   if (L.minv32)
     k_patch_inv<float><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv32, L.invperm,
-                                                 info);
+                                                 info, kref);
   else
     k_patch_inv<uint16_t><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv16,
-                                                    L.invperm, info);
+                                                    L.invperm, info, kref);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
-void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s) {
+void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s, float *kref) {
   if (L.n_cells <= 0) return;
   if (L.minv32)
     k_patch_schur<float><<<L.n_cells, 128, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv32, L.invperm,
-                                                   info);
+                                                   info, kref);
   else
     k_patch_schur<uint16_t><<<L.n_cells, 128, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv16,
-                                                      L.invperm, info);
+                                                      L.invperm, info, kref);
   PMG_HIP_CHECK(hipGetLastError());
 }
 
 template <typename F>
 void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
                               const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, const OpView<float> &op, const int *pmap, int nref,
-                              hipStream_t s) {
+                              float *ua, float damp, const float *kref, int nref, hipStream_t s) {
   if (ncells <= 0) return;
   if (nref > 0)
     k_vanka_apply_schur<F, true><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global,
-                                                       n_udof, u, ua, damp, op, pmap, nref);
+                                                       n_udof, u, ua, damp, kref, nref);
   else
     k_vanka_apply_schur<F, false><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global,
-                                                        n_udof, u, ua, damp, op, pmap, 0);
+                                                        n_udof, u, ua, damp, nullptr, 0);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_apply_schur<float>(const float *, const int *, int, int, const int *, const float *,
-                                              const int *, const int *, int, float *, float *, float,
-                                              const OpView<float> &, const int *, int, hipStream_t);
+                                              const int *, const int *, int, float *, float *, float, const float *,
+                                              int, hipStream_t);
 template void launch_vanka_apply_schur<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
                                                  const int *, const int *, int, float *, float *, float,
-                                                 const OpView<float> &, const int *, int, hipStream_t);
+                                                 const float *, int, hipStream_t);
 
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
                         const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
-                        const OpView<float> &op, const int *pmap, const uint8_t *invperm, int nref, hipStream_t s) {
+                        const float *kref, const uint8_t *invperm, int nref, hipStream_t s) {
   if (ncells <= 0) return;
   if (nref > 0)
     k_vanka_apply<F, true><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof,
-                                                 u, ua, damp, op, pmap, invperm, nref);
+                                                 u, ua, damp, kref, invperm, nref);
   else
     k_vanka_apply<F, false><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global,
-                                                  n_udof, u, ua, damp, op, pmap, invperm, 0);
+                                                  n_udof, u, ua, damp, nullptr, invperm, 0);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_apply<float>(const float *, const int *, int, int, const int *, const float *, const int *,
-                                        const int *, int, float *, float *, float, const OpView<float> &, const int *,
-                                        const uint8_t *, int, hipStream_t);
+                                        const int *, int, float *, float *, float, const float *, const uint8_t *, int,
+                                        hipStream_t);
 template void launch_vanka_apply<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
-                                           const int *, const int *, int, float *, float *, float,
-                                           const OpView<float> &, const int *, const uint8_t *, int, hipStream_t);
+                                           const int *, const int *, int, float *, float *, float, const float *,
+                                           const uint8_t *, int, hipStream_t);
 
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
                            const int *items, const int *desc, const int *xg, const T *u, const float *ua,
-                           const T *f, T *rloc, hipStream_t s) {
+                           const T *f, T *rloc, hipStream_t s, int per_block) {
   if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
-  int grid = chunk_grid(n_items, n_items < 65536 ? 16 : 32, 8192);
+  if (per_block <= 0) per_block = n_items < 65536 ? 16 : 32;
+  int grid = chunk_grid(n_items, per_block, 8192);
   if (xg) grid = (grid + 7) / 8 * 8;
   if (op.a1)
     k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc);
@@ -1300,10 +1339,10 @@ void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const ui
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,
                                             const int *, const int *, const double *, const float *, const double *,
-                                            double *, hipStream_t);
+                                            double *, hipStream_t, int);
 template void launch_vanka_residual<float>(const OpView<float> &, long, int, const uint8_t *, const int *,
                                            const int *, const int *, const float *, const float *, const float *,
-                                           float *, hipStream_t);
+                                           float *, hipStream_t, int);
 
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,

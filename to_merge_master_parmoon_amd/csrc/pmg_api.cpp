@@ -651,32 +651,35 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
     {
       const double rb = D.res_bytes[D.a_shared][vs == 8 ? 0 : 1][col] * part;
       Timed tm(c, &c->tm.vanka_res, rb, 0.0, &D == &c->L.back() ? &c->tm.vanka_res_finest : nullptr);
-      const int *xg = !D.compacted && D.h_xcd_grouped[col] ? D.xcd_goff + 9 * col : nullptr;
-      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream);
+      const int pi = sizeof(T) == 8 ? 0 : 1;
+      const int *xg = !D.compacted && D.h_xcd_grouped[col] && D.tune_res_xcd[pi] ? D.xcd_goff + 9 * col : nullptr;
+      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream,
+                               D.tune_res_pb[pi]);
     }
-    // refinement steps (stored inverses only): the FP32 patch matrix gathered once per patch
-    // (value + map entry per entry), one more local product and residual per step
+    // refinement steps (stored inverses only): the stored FP32 patch matrix read once per
+    // patch, one more local residual and product per step
     const int nref = sizeof(T) == 4 ? c->prm.bf16_refine_steps : 0;
-    const double kent = D.schur ? double(NU_LOC * NU_LOC + 2 * 3 * NU_LOC * NP_LOC) : double(NLOC * NLOC);
-    Timed tm(c, &c->tm.vanka_solve, nc * (solve_bytes(fb, vs) + (nref ? 8.0 * kent : 0.0)),
+    if (nref > 0 && !D.kref)
+      throw Fail{PMG_ERR_NOT_READY, "bf16_refine_steps was set after the set-up: call pmg_refactor"};
+    const double kent = D.schur ? double(KREF_BLK) : double(KREF_FULL);
+    Timed tm(c, &c->tm.vanka_solve, nc * (solve_bytes(fb, vs) + (nref ? 4.0 * kent : 0.0)),
              nc * (2.0 * NLOC * NLOC) * (1 + nref) + nc * 2.0 * kent * nref);
     if constexpr (sizeof(T) == 8) {
       launch_vanka_solve<double, double>(D.lu, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                          D.n_udof, u, damp, c->stream);
     } else {
-      const OpView<float> o32 = D.op32();
       if (D.schur && D.minv16)
         launch_vanka_apply_schur<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                           D.n_udof, u, ua, damp, o32, D.pmap, nref, c->stream);
+                                           D.n_udof, u, ua, damp, D.kref, nref, c->stream);
       else if (D.schur && D.minv32)
         launch_vanka_apply_schur<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                        D.n_udof, u, ua, damp, o32, D.pmap, nref, c->stream);
+                                        D.n_udof, u, ua, damp, D.kref, nref, c->stream);
       else if (D.minv16)
         launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
-                                     u, ua, damp, o32, D.pmap, D.invperm, nref, c->stream);
+                                     u, ua, damp, D.kref, D.invperm, nref, c->stream);
       else if (D.minv32)
         launch_vanka_apply<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof, u,
-                                  ua, damp, o32, D.pmap, D.invperm, nref, c->stream);
+                                  ua, damp, D.kref, D.invperm, nref, c->stream);
       else if (c->prm.precision == PMG_PREC_FP32_BF16)
         launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                             D.n_udof, u, damp, c->stream);
@@ -1048,10 +1051,22 @@ void setup_factors(pmg_ctx *c) {
       const bool schur = D.a_shared && D.max_cells_per_color > 2048;
       if (D.schur != schur) drop_graph(c);
       D.schur = schur;
+      // refinement steps: the FP32 patch matrices are stored beside the inverse data
+      float *kref = nullptr;
+      if (c->prm.bf16_refine_steps > 0) {
+        const size_t need = size_t(D.n_cells) * (schur ? KREF_BLK : KREF_FULL);
+        if (D.kref_len < need) {
+          drop_graph(c);
+          dfree(c, D.kref);
+          D.kref = dalloc<float>(c, need);
+          D.kref_len = need;
+        }
+        kref = D.kref;
+      }
       if (D.schur)
-        launch_patch_schur(D, D.patch_info, c->stream);
+        launch_patch_schur(D, D.patch_info, c->stream, kref);
       else
-        launch_patch_inv(D, D.patch_info, c->stream);
+        launch_patch_inv(D, D.patch_info, c->stream, kref);
     } else {
       launch_patch_lu(D, D.patch_info, c->stream);
     }
@@ -1728,7 +1743,14 @@ int pmg_set_params(pmg_ctx *c, const pmg_params *p) {
     c->err = "the problems of a batch carry their own pressure projection";
     return PMG_ERR_UNSUPPORTED;
   }
+  const bool refine_change = (p->bf16_refine_steps > 0) != (c->prm.bf16_refine_steps > 0);
   c->prm = *p;
+  if (refine_change) {
+    // refinement steps need the FP32 patch matrices stored at the set-up: solves report
+    // PMG_ERR_NOT_READY until pmg_setup / pmg_refactor stores them (or drops their use)
+    for (auto &D : c->L) D.factors_ready = false;
+    c->setup_done = false;
+  }
   if (pin_change) {
     // the coarse system pins (or frees) its pressure row: solves report PMG_ERR_NOT_READY
     // until pmg_setup / pmg_refactor builds the coarse factors again
@@ -2138,6 +2160,137 @@ int pmg_vanka_smooth(pmg_ctx *c, int level, double *u, const double *f, int nswe
     }
     io.out(0, u);
     io.sync();
+  });
+}
+
+// ------------------------------------------------------------------ launch-shape autotuner
+// Times the candidate launch shapes of the level kernels on the device (HIP events around
+// `reps` launches each, after one untimed launch) and keeps the fastest per level and
+// precision: work items per workgroup of the Vanka residual (16, 32, 64), whether large
+// colors run in their per-XCD groups, and operator rows per workgroup of the type-4
+// kernels (32 .. 256).  Every candidate computes the same values (the work is only
+// distributed differently), so the choice changes timings, not results, except for the
+// summation order of the grid-wide pressure reductions.
+extern "C++" {
+static float time_launches(pmg_ctx *c, int reps, const std::function<void()> &go) {
+  go();
+  PMG_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
+  for (int r = 0; r < reps; ++r) go();
+  PMG_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
+  PMG_HIP_CHECK(hipEventSynchronize(c->ev1));
+  float ms = 0;
+  PMG_HIP_CHECK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  return ms / reps;
+}
+
+template <typename T>
+static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  T *u = Vec<T>::sol(D), *f = Vec<T>::rhs(D), *rloc = Vec<T>::rloc(D);
+  launch_fill<T>(u, T(0.5), D.n_dof, c->stream);
+  launch_fill<T>(f, T(1.0), D.n_dof, c->stream);
+  const OpView<T> op = opview<T>(D);
+  if (D.n_colors > 0 && !D.nodal) {
+    float *ua = nullptr;
+    if constexpr (sizeof(T) == 4) {
+      if (D.ua32 && D.a_shared && (D.minv16 || D.minv32)) {
+        ua = D.ua32;
+        launch_sweep_start(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, ua, c->stream);
+      }
+    }
+    const int p0 = D.ptr_colors[0];
+    const long n_items = long(D.ptr_colors[1] - p0) * (NU_LOC + NP_LOC);
+    const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
+    const int *desc = D.res_desc + 4 * size_t(p0) * (NU_LOC + NP_LOC);
+    const bool grouped = D.h_xcd_grouped[0];
+    float best = 1e30f;
+    for (int xcd = grouped ? 0 : 1; xcd <= 1; ++xcd)
+      for (int pb : {16, 32, 64}) {
+        const int *xg = grouped && xcd ? D.xcd_goff : nullptr;
+        const float ms = time_launches(c, reps, [&] {
+          launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream, pb);
+        });
+        if (ms < best) {
+          best = ms;
+          D.tune_res_pb[pi] = pb;
+          D.tune_res_xcd[pi] = xcd;
+        }
+      }
+  }
+  // the MG defect of this level (and, finest level of a mixed context, the FP64 matvec)
+  auto tune_op = [&](auto zero, int mode, int slot) {
+    using U = decltype(zero);
+    U *x = Vec<U>::sol(D), *b = Vec<U>::rhs(D), *yy = Vec<U>::aux0(D);
+    if (!std::is_same<U, T>::value) {
+      launch_fill<U>(x, U(0.5), D.n_dof, c->stream);
+      launch_fill<U>(b, U(1.0), D.n_dof, c->stream);
+    }
+    float best = 1e30f;
+    int choice = 0;
+    for (int rpb : {32, 64, 128, 256}) {
+      OpView<U> o = opview<U>(D);
+      o.rpb = rpb;
+      const float ms = time_launches(c, reps, [&] {
+        launch_type4<U>(o, mode, x, b, yy, mode != 0 && fused_l20(c), nullptr, c->red, c->stream);
+      });
+      if (ms < best) {
+        best = ms;
+        choice = rpb;
+      }
+    }
+    D.tune_op_rpb[slot] = choice;
+  };
+  tune_op(T(0), 2, pi);
+  if (finest && pi == 1) tune_op(double(0), 0, 0);
+}
+}  // extern "C++"
+
+int pmg_autotune(pmg_ctx *c, int reps) {
+  if (!c || reps <= 0 || reps > 1000) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    PMG_NEED_READY();
+    drop_graph(c);
+    const bool mx = mixed(c);
+    // the coarsest level is solved directly (no smoother, no defect in the cycle)
+    for (size_t l = 1; l < c->L.size(); ++l) {
+      DevLevel &D = c->L[l];
+      const bool finest = l + 1 == c->L.size();
+      if (mx)
+        tune_level<float>(c, D, reps, finest);
+      else
+        tune_level<double>(c, D, reps, finest);
+    }
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int pmg_set_level_tuning(pmg_ctx *c, int level, const int *in6) {
+  if (!c || !in6) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    DevLevel &D = level_of(c, level);
+    for (int k = 0; k < 2; ++k) {
+      if (in6[k] != 0 && (in6[k] < 1 || in6[k] > 1024))
+        throw Fail{PMG_ERR_INVALID_ARGUMENT, "residual items per workgroup out of range"};
+      if (in6[2 + k] != 0 && in6[2 + k] != 1) throw Fail{PMG_ERR_INVALID_ARGUMENT, "XCD group flag must be 0 or 1"};
+      if (in6[4 + k] != 0 && (in6[4 + k] < 1 || in6[4 + k] > 4096))
+        throw Fail{PMG_ERR_INVALID_ARGUMENT, "operator rows per workgroup out of range"};
+    }
+    for (int k = 0; k < 2; ++k) {
+      D.tune_res_pb[k] = in6[k];
+      D.tune_res_xcd[k] = in6[2 + k];
+      D.tune_op_rpb[k] = in6[4 + k];
+    }
+    drop_graph(c);
+  });
+}
+
+int pmg_level_tuning(pmg_ctx *c, int level, int *out6) {
+  if (!c || !out6) return PMG_ERR_INVALID_ARGUMENT;
+  return guard(c, [&] {
+    const DevLevel &D = level_of(c, level);
+    const int v[6] = {D.tune_res_pb[0], D.tune_res_pb[1], D.tune_res_xcd[0], D.tune_res_xcd[1], D.tune_op_rpb[0],
+                      D.tune_op_rpb[1]};
+    std::copy(v, v + 6, out6);
   });
 }
 

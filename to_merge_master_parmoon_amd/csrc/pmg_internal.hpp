@@ -49,6 +49,12 @@ constexpr int PM_MT = PM_BT + NU_LOC * NP_LOC;   // 864
 constexpr int PM_B = PM_MT + NU_LOC;             // 891
 constexpr int PM_NB = PM_B + NP_LOC * NU_LOC;    // 999
 constexpr int PM_STRIDE = 1008;
+// FP32 patch matrices kept for the refinement steps of the mixed local solves: the block
+// form (A 27 x 27 row-major, then BT_c[j][k] and B_c[k][j] of the three components) or the
+// full 85 x 85 matrix (row-major), per patch in color order
+constexpr int KREF_BLK = 1380;      // 729 + 324 + 324 = 1377, padded to 4 entries
+constexpr int KREF_FULL = 7232;     // 85 * 85 = 7225, padded
+constexpr int KREF_BT = 729, KREF_B = 729 + 324;
 // nodal Vanka: velocity dofs of one B row (27 for Q2/P1disc), upper bound
 constexpr int NODAL_MAXU = 28;
 // colors of at least this many cells run their residual items in 8 per-XCD groups
@@ -79,6 +85,7 @@ struct OpView {
   // the shared velocity block in CSR order when A11 = A22 = A33 and A_rc = 0 (r != c) on
   // the active rows (the Galerkin Oseen form); nullptr: the nine blocks differ
   const T *a1;
+  int rpb;                  // operator rows per workgroup of the type-4 kernels (0: default)
 };
 
 // Streaming kernels split their n items (rows, residual work items) into one contiguous
@@ -152,6 +159,8 @@ struct DevLevel {
   uint8_t *invperm = nullptr;        // n_cells * PERM_STRIDE
   int *pmap = nullptr;               // n_cells * PM_STRIDE patch gather map
   int *patch_info = nullptr;         // n_cells
+  float *kref = nullptr;             // FP32 patch matrices for refinement steps (KREF_*)
+  size_t kref_len = 0;
   bool factors_ready = false;
   // transfers to the next coarser level (this level = fine)
   Csr pu, ru, pp, rp;                // P_U (fine rows), R_U = P_U^T, P_P, R_P
@@ -191,15 +200,22 @@ struct DevLevel {
   int *act_items = nullptr, *act_rows = nullptr, *act_slots = nullptr, *act_desc = nullptr;
   double *geom = nullptr;            // n_cells * 8 corners * 3 (device re-assembly)
 
+  // launch shapes chosen by pmg_autotune (0: the built-in defaults): residual work items per
+  // workgroup, whether large colors run in their per-XCD groups, operator rows per
+  // workgroup of the type-4 kernels, per precision (0: FP64, 1: FP32)
+  int tune_res_pb[2] = {0, 0};
+  int tune_res_xcd[2] = {1, 1};
+  int tune_op_rpb[2] = {0, 0};
+
   OpView<double> op64() const {
     return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
                           b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz,
-                          av64, bv64, btv64, a_shared ? a : nullptr};
+                          av64, bv64, btv64, a_shared ? a : nullptr, tune_op_rpb[0]};
   }
   OpView<float> op32() const {
     return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, nullptr, a_nnz,
                          b_rowptr, b_kcol, nullptr, b_nnz, bt_rowptr, bt_kcol, nullptr, bt_nnz,
-                         av32, bv32, btv32, a_shared ? a1_32 : nullptr};
+                         av32, bv32, btv32, a_shared ? a1_32 : nullptr, tune_op_rpb[1]};
   }
 };
 
@@ -266,28 +282,28 @@ void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, b
 // mixed path, computed in FP32 from the FP32 operator copy into lu32 or bf16 lu16
 void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 // mixed precision, explicit inverses: FP32 Gauss-Jordan into minv32 or bf16 minv16
-void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s);
-// nref > 0: that many refinement steps with the FP32 patch matrix (op, pmap) after the
-// first product (residual permuted by invperm like the local residual)
+void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s, float *kref = nullptr);
+// nref > 0: that many refinement steps with the FP32 patch matrices kref (KREF_FULL per
+// patch) after the first product (residual permuted by invperm like the local residual)
 template <typename F>
 void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
                         const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
-                        const OpView<float> &op, const int *pmap, const uint8_t *invperm, int nref, hipStream_t s);
+                        const float *kref, const uint8_t *invperm, int nref, hipStream_t s);
 // block form (shared velocity block): set-up into minv32 / minv16 with SCH_STRIDE per patch,
 // and its local solves
-void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s);
+void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s, float *kref = nullptr);
 template <typename F>
 void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
                               const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, const OpView<float> &op, const int *pmap, int nref,
-                              hipStream_t s);
+                              float *ua, float damp, const float *kref, int nref, hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
-// xg (may be null): the 9 item offsets of the color's 8 XCD groups
+// xg (may be null): the 9 item offsets of the color's 8 XCD groups; per_block: work items
+// per workgroup (0: 16 below 65,536 items, 32 above)
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
                            const int *items, const int *desc, const int *xg, const T *u, const float *ua,
-                           const T *f, T *rloc, hipStream_t s);
+                           const T *f, T *rloc, hipStream_t s, int per_block = 0);
 // FP32 sweep start: Dirichlet slices of u from f, ua[4 i + c] = u_c(i) (c < 3)
 void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s);
 // descriptors of residual work items (4 ints each) from their items and global rows

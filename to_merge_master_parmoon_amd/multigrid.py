@@ -26,6 +26,34 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _autotune(obj, n_levels: int, reps: int) -> list:
+    obj._check(obj._lib.pmg_autotune(obj._ctx, reps), "pmg_autotune")
+    out = []
+    for l in range(n_levels):
+        v = (C.c_int * 6)()
+        obj._check(obj._lib.pmg_level_tuning(obj._ctx, l, v), "pmg_level_tuning")
+        out.append(dict(res_items_per_wg=[v[0], v[1]], res_xcd_groups=[v[2], v[3]], op_rows_per_wg=[v[4], v[5]]))
+    return out
+
+
+def save_tuning(shapes: list, path: str):
+    """Write the launch shapes of autotune() as JSON."""
+    import json
+    with open(path, "w") as fh:
+        json.dump({"levels": shapes}, fh, indent=1)
+
+
+def load_tuning(obj, path: str) -> list:
+    """Apply launch shapes saved by save_tuning to the levels of obj (pmg_set_level_tuning)."""
+    import json
+    with open(path) as fh:
+        shapes = json.load(fh)["levels"]
+    for l, s in enumerate(shapes):
+        v = (C.c_int * 6)(*s["res_items_per_wg"], *s["res_xcd_groups"], *s["op_rows_per_wg"])
+        obj._check(obj._lib.pmg_set_level_tuning(obj._ctx, l, v), "pmg_set_level_tuning")
+    return shapes
+
+
 class DeviceVector:
     """A float64 vector in device memory owned by a context."""
 
@@ -218,6 +246,11 @@ class NSEMultiGrid:
         self._check(self._lib.pmg_synchronize(self._ctx))
 
     # ------------------------------------------------------------------ timing
+    def autotune(self, reps: int = 20) -> list:
+        """Time the candidate launch shapes of every smoothed level on the device and keep
+        the fastest (pmg_autotune); returns the chosen shapes per level."""
+        return _autotune(self, self.n_levels, reps)
+
     def set_profiling(self, on: bool):
         self._lib.pmg_set_profiling(self._ctx, 1 if on else 0)
 
@@ -296,6 +329,10 @@ class NSEBatch:
             msg = self._lib.pmg_last_error_string(self._ctx)
             raise PmgError(st, f"{what} {msg.decode() if msg else ''}".strip())
         return st
+
+    def autotune(self, reps: int = 20) -> list:
+        """pmg_autotune on the merged levels (after the set-up)."""
+        return _autotune(self, max(p.n_levels for p in self.problems), reps)
 
     def close(self):
         if getattr(self, "_ctx", None):
