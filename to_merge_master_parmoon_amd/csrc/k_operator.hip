@@ -213,6 +213,15 @@ __global__ void k_sweep_start(float *__restrict__ u, const float *__restrict__ f
   ua[i] = make_float4(v1, v2, v3, 0.0f);
 }
 
+__global__ void k_ua_to_u(const float4 *__restrict__ ua, float *__restrict__ u, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const float4 v = ua[i];
+  u[i] = v.x;
+  u[N + i] = v.y;
+  u[2 * N + i] = v.z;
+}
+
 // IntoL20 of the P1disc constants in two passes without atomics: every block sums its
 // chunk of constants (k_l20_part), then every block adds all partials in the same order
 // and subtracts the mean from its chunk (k_l20_sub)
@@ -460,6 +469,12 @@ void launch_dirichlet_slices(T *u, const T *f, int N, int start, int count, hipS
   k_dir<T><<<(3 * count + 255) / 256, 256, 0, s>>>(u, f, N, start, count);
   PMG_HIP_CHECK(hipGetLastError());
 }
+void launch_ua_to_u(const float *ua, float *u, int N, hipStream_t s) {
+  if (N <= 0) return;
+  k_ua_to_u<<<(N + 255) / 256, 256, 0, s>>>(reinterpret_cast<const float4 *>(ua), u, N);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+
 void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s) {
   if (N <= 0) return;
   k_sweep_start<<<(N + 255) / 256, 256, 0, s>>>(u, f, N, start, count, reinterpret_cast<float4 *>(ua));

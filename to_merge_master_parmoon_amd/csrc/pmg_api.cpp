@@ -670,10 +670,10 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
     } else {
       if (D.schur && D.minv16)
         launch_vanka_apply_schur<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                           D.n_udof, u, ua, damp, D.kref, nref, c->stream);
+                                           D.n_udof, u, ua, damp, D.kref, nref, c->stream, ua != nullptr);
       else if (D.schur && D.minv32)
         launch_vanka_apply_schur<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                        D.n_udof, u, ua, damp, D.kref, nref, c->stream);
+                                        D.n_udof, u, ua, damp, D.kref, nref, c->stream, ua != nullptr);
       else if (D.minv16)
         launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
                                      u, ua, damp, D.kref, D.invperm, nref, c->stream);
@@ -687,6 +687,10 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
         launch_vanka_solve<float, float>(D.lu32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                          D.n_udof, u, damp, c->stream);
     }
+  }
+  if constexpr (sizeof(T) == 4) {
+    // the block-form local solves kept the velocity in the interleaved copy only
+    if (ua && D.schur) launch_ua_to_u(ua, u, D.n_udof, c->stream);
   }
   if (relax) launch_relax<T>(u, aux, T(alpha), D.n_dof, c->stream);
   if (last) set_dirichlet_t<T>(c, D, u, f);

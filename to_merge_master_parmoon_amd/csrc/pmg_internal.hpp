@@ -295,7 +295,10 @@ void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s, float *kref
 template <typename F>
 void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
                               const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, const float *kref, int nref, hipStream_t s);
+                              float *ua, float damp, const float *kref, int nref, hipStream_t s,
+                              bool ua_only = false);
+// the velocity of the interleaved copy ua back into u (u_c[i] = ua[4 i + c])
+void launch_ua_to_u(const float *ua, float *u, int N, hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
