@@ -89,6 +89,8 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
   __shared__ double Us[LU_NB][LU_T];         // the U block row of the panel (columns j0 + 16 ..)
   __shared__ double wv[2][LU_T / 64];
   __shared__ int wi[2][LU_T / 64];
+  __shared__ double WB[2][LU_T / 64][LU_NB];   // every wavefront's candidate pivot row
+  __shared__ double JB[2][LU_NB];              // row j of the current step
   __shared__ int pivs[LU_NB];
   __shared__ int sig[LU_T];
   __shared__ int rows_s[2 * LU_NB], src_s[2 * LU_NB];
@@ -100,13 +102,17 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
     const int w = min(LU_NB, n - j0);
     const int r = j0 + t;
     const bool own = r < n;
+    // the panel row of this thread in registers; per pivot step one barrier: the winning
+    // row of every wavefront and row j publish themselves before it, so after it every
+    // thread has the pivot row and the row it swaps with (no in-place exchange in LDS)
+    double rv[LU_NB];
 #pragma unroll
-    for (int cc = 0; cc < LU_NB; ++cc) Lp[t][cc] = (own && cc < w) ? a[long(r) * ld + j0 + cc] : 0.0;
-    __syncthreads();
-#pragma unroll 1
-    for (int jj = 0; jj < w; ++jj) {
+    for (int cc = 0; cc < LU_NB; ++cc) rv[cc] = (own && cc < w) ? a[long(r) * ld + j0 + cc] : 0.0;
+#pragma unroll
+    for (int jj = 0; jj < LU_NB; ++jj) {
+      if (jj >= w) break;
       const int j = j0 + jj, par = jj & 1;
-      double bv = (own && r >= j) ? fabs(Lp[t][jj]) : -1.0;
+      double bv = (own && r >= j) ? fabs(rv[jj]) : -1.0;
       int bi = (own && r >= j) ? r : n;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
@@ -121,6 +127,12 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
         wv[par][wave] = bv;
         wi[par][wave] = bi;
       }
+      if (own && r == bi)   // this wavefront's candidate pivot row
+#pragma unroll
+        for (int cc = 0; cc < LU_NB; ++cc) WB[par][wave][cc] = rv[cc];
+      if (t == jj)          // row j, which the pivot row swaps with
+#pragma unroll
+        for (int cc = 0; cc < LU_NB; ++cc) JB[par][cc] = rv[cc];
       __syncthreads();
       bv = wv[par][0];
       bi = wi[par][0];
@@ -139,18 +151,27 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
         if (bv == 0.0 && info_s == 0) info_s = j + 1;
       }
       if (bv == 0.0) continue;   // uniform: dgetf2 records the zero pivot and moves on
-      if (p != j && t < LU_NB) {
-        const double x = Lp[jj][t];
-        Lp[jj][t] = Lp[p - j0][t];
-        Lp[p - j0][t] = x;
+      const int wp = (p - j0) >> 6;
+      double prow[LU_NB];
+#pragma unroll
+      for (int cc = 0; cc < LU_NB; ++cc) prow[cc] = WB[par][wp][cc];
+      if (t == jj) {
+#pragma unroll
+        for (int cc = 0; cc < LU_NB; ++cc) rv[cc] = prow[cc];
+      } else if (r == p) {
+#pragma unroll
+        for (int cc = 0; cc < LU_NB; ++cc) rv[cc] = JB[par][cc];
       }
-      __syncthreads();
       if (own && r > j) {
-        const double l = Lp[t][jj] / Lp[jj][jj];
-        Lp[t][jj] = l;
-        for (int cc = jj + 1; cc < LU_NB; ++cc) Lp[t][cc] = fma(-l, Lp[jj][cc], Lp[t][cc]);
+        const double l = rv[jj] / prow[jj];
+        rv[jj] = l;
+#pragma unroll
+        for (int cc = 0; cc < LU_NB; ++cc)
+          if (cc > jj) rv[cc] = fma(-l, prow[cc], rv[cc]);
       }
     }
+#pragma unroll
+    for (int cc = 0; cc < LU_NB; ++cc) Lp[t][cc] = rv[cc];
     __syncthreads();
     // the factored panel to memory
     if (own)
@@ -229,28 +250,47 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
       for (int k = 0; k < LU_NB; ++k) Us[k][t] = 0.0;
     }
     __syncthreads();
-    // trailing update A22 -= L21 U12 on 16 x 16 tiles, one wavefront per tile
+    // trailing update A22 -= L21 U12 on 16 x 16 tiles, TG tiles per wavefront at a time so
+    // that their loads are in flight together (a tile alone is one memory round trip)
     const int nb = (nt + 15) / 16;
     const int col = lane & 15, rq = lane >> 4;
-    for (int tile = wave; tile < nb * nb; tile += LU_T / 64) {
-      const int ti = tile / nb, tc = tile - ti * nb;
-      const int i0 = j1 + ti * 16, c0 = j1 + tc * 16;
-      d4 acc;
+    constexpr int TG = 4;
+    for (int tile0 = wave * TG; tile0 < nb * nb; tile0 += (LU_T / 64) * TG) {
+      d4 acc[TG];
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const int i = i0 + 4 * qq + rq, c = c0 + col;
-        acc[qq] = (i < n && c < n) ? a[long(i) * ld + c] : 0.0;
+      for (int g = 0; g < TG; ++g) {
+        const int tile = tile0 + g;
+        const int ti = tile / nb, tc = tile - ti * nb;
+        const int i0 = j1 + ti * 16, c0 = j1 + tc * 16;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const int i = i0 + 4 * qq + rq, c = c0 + col;
+          acc[g][qq] = (tile < nb * nb && i < n && c < n) ? a[long(i) * ld + c] : 0.0;
+        }
       }
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const double av = -Lp[i0 - j0 + col][4 * s + rq];
-        const double bv = Us[4 * s + rq][c0 - j1 + col];
-        acc = mfma16(av, bv, acc);
+      for (int g = 0; g < TG; ++g) {
+        const int tile = tile0 + g;
+        if (tile >= nb * nb) continue;
+        const int ti = tile / nb, tc = tile - ti * nb;
+        const int i0 = j1 + ti * 16, c0 = j1 + tc * 16;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const double av = -Lp[i0 - j0 + col][4 * s + rq];
+          const double bv = Us[4 * s + rq][c0 - j1 + col];
+          acc[g] = mfma16(av, bv, acc[g]);
+        }
       }
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) {
-        const int i = i0 + 4 * qq + rq, c = c0 + col;
-        if (i < n && c < n) a[long(i) * ld + c] = acc[qq];
+      for (int g = 0; g < TG; ++g) {
+        const int tile = tile0 + g;
+        const int ti = tile / nb, tc = tile - ti * nb;
+        const int i0 = j1 + ti * 16, c0 = j1 + tc * 16;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+          const int i = i0 + 4 * qq + rq, c = c0 + col;
+          if (tile < nb * nb && i < n && c < n) a[long(i) * ld + c] = acc[g][qq];
+        }
       }
     }
     __syncthreads();
@@ -269,6 +309,7 @@ __global__ __launch_bounds__(TRSM_T) void k_trsm_wg(const double *__restrict__ l
                                                      const int *__restrict__ sigma, const double *__restrict__ R,
                                                      int ldr, int nrhs, double *__restrict__ X, int ldx) {
   __shared__ double xs[512][TRSM_NR + 1];
+  __shared__ double Lb[512][TRSM_NR + 1];   // the factors' current 16-column block, all rows
   __shared__ double dg[16][17];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int c0 = blockIdx.x * TRSM_NR;
@@ -280,13 +321,23 @@ __global__ __launch_bounds__(TRSM_T) void k_trsm_wg(const double *__restrict__ l
   }
   const int nb = (n + 15) / 16;
   const int col = lane & 15, rq = lane >> 4;
+  // the block's rows [r0, r1) and columns k0 .. k0 + 15 of the factors into LDS at once (one
+  // memory round trip per block instead of one per 16 x 16 tile)
+  auto stage = [&](int r0, int r1, int k0, int kw) {
+    for (int idx = t; idx < (r1 - r0) * 16; idx += TRSM_T) {
+      const int i = r0 + (idx >> 4), k = idx & 15;
+      Lb[i][k] = k < kw ? lu[long(i) * ld + k0 + k] : 0.0;
+    }
+  };
   // forward: unit lower L
   for (int kb = 0; kb < nb; ++kb) {
     const int k0 = kb * 16, kw = min(16, n - k0);
     __syncthreads();
+    stage(k0, n, k0, kw);
+    __syncthreads();
     {
       const int i = t >> 4, k = t & 15;
-      dg[i][k] = (i < kw && k < kw) ? lu[long(k0 + i) * ld + k0 + k] : 0.0;
+      dg[i][k] = (i < kw && k < kw) ? Lb[k0 + i][k] : 0.0;
     }
     __syncthreads();
     if (t < TRSM_NR) {
@@ -315,7 +366,7 @@ __global__ __launch_bounds__(TRSM_T) void k_trsm_wg(const double *__restrict__ l
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int i = i0 + col, k = k0 + 4 * s + rq;
-        const double av = (i < n && k < k0 + kw) ? -lu[long(i) * ld + k] : 0.0;
+        const double av = (i < n && 4 * s + rq < kw) ? -Lb[i][4 * s + rq] : 0.0;
         const double bv = (4 * s + rq < kw) ? xs[k][col] : 0.0;
         acc = mfma16(av, bv, acc);
       }
@@ -330,9 +381,11 @@ __global__ __launch_bounds__(TRSM_T) void k_trsm_wg(const double *__restrict__ l
   for (int kb = nb - 1; kb >= 0; --kb) {
     const int k0 = kb * 16, kw = min(16, n - k0);
     __syncthreads();
+    stage(0, k0 + kw, k0, kw);
+    __syncthreads();
     {
       const int i = t >> 4, k = t & 15;
-      dg[i][k] = (i < kw && k < kw) ? lu[long(k0 + i) * ld + k0 + k] : 1.0;
+      dg[i][k] = (i < kw && k < kw) ? Lb[k0 + i][k] : 1.0;
     }
     __syncthreads();
     if (t < TRSM_NR) {
@@ -359,7 +412,7 @@ __global__ __launch_bounds__(TRSM_T) void k_trsm_wg(const double *__restrict__ l
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int i = i0 + col, k = k0 + 4 * s + rq;
-        const double av = (4 * s + rq < kw) ? -lu[long(i) * ld + k] : 0.0;
+        const double av = (4 * s + rq < kw) ? -Lb[i][4 * s + rq] : 0.0;
         const double bv = (4 * s + rq < kw) ? xs[k][col] : 0.0;
         acc = mfma16(av, bv, acc);
       }
