@@ -18,6 +18,8 @@
 
 namespace pmg {
 
+typedef float v4f __attribute__((ext_vector_type(4)));   // MFMA f32 accumulator
+
 namespace {
 
 
@@ -504,19 +506,44 @@ __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int
     }
     __syncthreads();
   }
-  for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
-    const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC, j = r / NP_LOC, k = r - NP_LOC * j;
-    float acc = 0.0f;
-    for (int m = 0; m < NU_LOC; ++m) acc = fmaf(A[j][m], BT[c][m][k], acc);
-    W[c][j][k] = acc;
+  // W = A^{-1} [BT_1 BT_2 BT_3] (27 x 12) on the matrix cores: wavefront w computes rows
+  // 16 w .. 16 w + 15 as one 16 x 16 tile of v_mfma_f32_16x16x4f32 (K = 27 in 7 steps of 4,
+  // padded with zeros); lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] of each
+  // 16 x 4 / 4 x 16 step and holds D[4 (l / 16) + i][l % 16] in result i
+  {
+    const int lane = t & 63, wv = t >> 6;
+    const int lr = lane & 15, lq = lane >> 4;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kk = 0; kk < 7; ++kk) {
+      const int row = 16 * wv + lr, m = 4 * kk + lq;
+      const float av = (row < NU_LOC && m < NU_LOC) ? A[row][m] : 0.0f;
+      const float bv = (m < NU_LOC && lr < 3 * NP_LOC) ? BT[lr / NP_LOC][m][lr % NP_LOC] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 16 * wv + 4 * lq + i;
+      if (row < NU_LOC && lr < 3 * NP_LOC) W[lr / NP_LOC][row][lr % NP_LOC] = acc[i];
+    }
   }
   __syncthreads();
-  if (t < NP_LOC * NP_LOC) {
-    const int k = t / NP_LOC, l = t - NP_LOC * k;
-    float acc = 0.0f;
-    for (int c = 0; c < 3; ++c)
-      for (int j = 0; j < NU_LOC; ++j) acc = fmaf(Bm[c][k][j], W[c][j][l], acc);
-    S[k][l] = -acc;
+  // S = -[B_1 B_2 B_3] W (4 x 4, K = 81 in 21 steps of 4) on the matrix cores, wavefront 0
+  if (t < 64) {
+    const int lr = t & 15, lq = t >> 4;
+    v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 3
+    for (int kk = 0; kk < 21; ++kk) {
+      const int q = 4 * kk + lq, c = q / NU_LOC, j = q - NU_LOC * c;
+      const float av = (lr < NP_LOC && q < 3 * NU_LOC) ? Bm[c][lr][j] : 0.0f;
+      const float bv = (lr < NP_LOC && q < 3 * NU_LOC) ? W[c][j][lr] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = 4 * lq + i;
+      if (row < NP_LOC && lr < NP_LOC) S[row][lr] = -acc[i];
+    }
   }
   __syncthreads();
   if (t == 0) {
