@@ -57,6 +57,7 @@ def solver_params(precision: int, shared_block: int | None = None, patch_inverse
 
 
 AUTOTUNE = True
+TUNING_FILE = None
 
 
 def run_gpu(prob, precision: int, steps: int, warmup: int, log, shared_block=None, patch_inverse=None,
@@ -71,10 +72,22 @@ def run_gpu(prob, precision: int, steps: int, warmup: int, log, shared_block=Non
     log(f"hierarchy on device ({time.time() - t0:.1f}s), levels {mg.n_dof}, "
         f"shared block {[mg.shared_block(l) for l in range(mg.n_levels)]}")
     shapes = None
-    if AUTOTUNE:
+    key = f"fp{['64', '32', '32bf16'][precision]}_sb{mg.shared_block(mg.n_levels - 1)}"
+    if TUNING_FILE and Path(TUNING_FILE).exists() and key in json.loads(Path(TUNING_FILE).read_text()):
+        from to_merge_master_parmoon_amd.multigrid import load_tuning
+        tmp = Path(TUNING_FILE).with_suffix(".one.json")
+        tmp.write_text(json.dumps(json.loads(Path(TUNING_FILE).read_text())[key]))
+        shapes = load_tuning(mg, str(tmp))
+        tmp.unlink()
+        log(f"launch shapes loaded from {TUNING_FILE}: {shapes}")
+    elif AUTOTUNE:
         t0 = time.time()
         shapes = mg.autotune(20)
         log(f"launch shapes tuned ({time.time() - t0:.2f}s): {shapes}")
+        if TUNING_FILE:
+            saved = json.loads(Path(TUNING_FILE).read_text()) if Path(TUNING_FILE).exists() else {}
+            saved[key] = {"levels": shapes}
+            Path(TUNING_FILE).write_text(json.dumps(saved, indent=1))
     n = mg.n_dof[-1]
     rhs = DeviceVector(mg, n, prob.rhs(-1))
     zero = DeviceVector(mg, n, np.zeros(n))
@@ -490,6 +503,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload's fields")
     ap.add_argument("--no-refine", action="store_true", help="skip the timings with local refinement steps")
     ap.add_argument("--no-autotune", action="store_true", help="keep the built-in launch shapes")
+    ap.add_argument("--tuning-file", default=None,
+                    help="JSON of launch shapes: loaded when it holds this form's shapes, else tuned and saved there")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--workload", choices=["oseen32", "batched", "reassembly", "channel"], default="oseen32",
                     help="batched: the independent-problems workload; reassembly: one fixed-point step's "
@@ -504,9 +519,10 @@ def main():
     ap.add_argument("--nine-blocks", action="store_true",
                     help="stream all nine A blocks even where A11 = A22 = A33 and the coupling blocks vanish")
     args = ap.parse_args()
-    global USE_GRAPH, SHARED_BLOCK, PATCH_INVERSE, AUTOTUNE
+    global USE_GRAPH, SHARED_BLOCK, PATCH_INVERSE, AUTOTUNE, TUNING_FILE
     USE_GRAPH = 0 if args.no_graph else 1
     AUTOTUNE = not args.no_autotune
+    TUNING_FILE = args.tuning_file
     SHARED_BLOCK = 0 if args.nine_blocks else 1
     PATCH_INVERSE = 0 if args.lu_sweeps else 1
     log = (lambda *a: None) if args.quiet else (lambda *a: print("[bench]", *a, file=sys.stderr, flush=True))

@@ -2182,8 +2182,8 @@ int pmg_vanka_smooth(pmg_ctx *c, int level, double *u, const double *f, int nswe
 // summation order of the grid-wide pressure reductions.
 extern "C++" {
 // The candidate's reps launches are captured once as a HIP graph and the graph is replayed
-// three times between events (the fastest replay counts), so that the device time is
-// measured, not the host's launch rate (which, eager, exceeds a small level's kernel time).
+// twice between events (the faster replay counts), so that the device time is measured,
+// not the host's launch rate (which, eager, exceeds a small level's kernel time).
 static float time_launches(pmg_ctx *c, int reps, const std::function<void()> &go) {
   go();
   hipGraph_t g = nullptr;
@@ -2200,7 +2200,7 @@ static float time_launches(pmg_ctx *c, int reps, const std::function<void()> &go
   PMG_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
   PMG_HIP_CHECK(hipGraphLaunch(ge, c->stream));
   float best = 1e30f;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 2; ++k) {
     PMG_HIP_CHECK(hipEventRecord(c->ev0, c->stream));
     PMG_HIP_CHECK(hipGraphLaunch(ge, c->stream));
     PMG_HIP_CHECK(hipEventRecord(c->ev1, c->stream));
@@ -2234,19 +2234,23 @@ static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
     const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
     const int *desc = D.res_desc + 4 * size_t(p0) * (NU_LOC + NP_LOC);
     const bool grouped = D.h_xcd_grouped[0];
-    float best = 1e30f;
+    // three rounds over the candidates, each candidate's fastest round counts (the rounds
+    // interleave the candidates, so a slow phase of the device does not favour one)
+    std::vector<std::pair<int, int>> cand;
     for (int xcd = grouped ? 0 : 1; xcd <= 1; ++xcd)
-      for (int pb : {16, 32, 64}) {
-        const int *xg = grouped && xcd ? D.xcd_goff : nullptr;
-        const float ms = time_launches(c, reps, [&] {
+      for (int pb : {16, 32, 64}) cand.emplace_back(xcd, pb);
+    std::vector<float> best(cand.size(), 1e30f);
+    for (int round = 0; round < 3; ++round)
+      for (size_t k = 0; k < cand.size(); ++k) {
+        const int *xg = grouped && cand[k].first ? D.xcd_goff : nullptr;
+        const int pb = cand[k].second;
+        best[k] = std::min(best[k], time_launches(c, reps, [&] {
           launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream, pb);
-        });
-        if (ms < best) {
-          best = ms;
-          D.tune_res_pb[pi] = pb;
-          D.tune_res_xcd[pi] = xcd;
-        }
+        }));
       }
+    const size_t kb = size_t(std::min_element(best.begin(), best.end()) - best.begin());
+    D.tune_res_xcd[pi] = cand[kb].first;
+    D.tune_res_pb[pi] = cand[kb].second;
   }
   // the MG defect of this level (and, finest level of a mixed context, the FP64 matvec)
   auto tune_op = [&](auto zero, int mode, int slot) {
@@ -2256,20 +2260,17 @@ static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
       launch_fill<U>(x, U(0.5), D.n_dof, c->stream);
       launch_fill<U>(b, U(1.0), D.n_dof, c->stream);
     }
-    float best = 1e30f;
-    int choice = 0;
-    for (int rpb : {32, 64, 128, 256}) {
-      OpView<U> o = opview<U>(D);
-      o.rpb = rpb;
-      const float ms = time_launches(c, reps, [&] {
-        launch_type4<U>(o, mode, x, b, yy, mode != 0 && fused_l20(c), nullptr, c->red, c->stream);
-      });
-      if (ms < best) {
-        best = ms;
-        choice = rpb;
+    const int cand[4] = {32, 64, 128, 256};
+    float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    for (int round = 0; round < 3; ++round)
+      for (int k = 0; k < 4; ++k) {
+        OpView<U> o = opview<U>(D);
+        o.rpb = cand[k];
+        best[k] = std::min(best[k], time_launches(c, reps, [&] {
+          launch_type4<U>(o, mode, x, b, yy, mode != 0 && fused_l20(c), nullptr, c->red, c->stream);
+        }));
       }
-    }
-    D.tune_op_rpb[slot] = choice;
+    D.tune_op_rpb[slot] = cand[std::min_element(best, best + 4) - best];
   };
   tune_op(T(0), 2, pi);
   if (finest && pi == 1) tune_op(double(0), 0, 0);
