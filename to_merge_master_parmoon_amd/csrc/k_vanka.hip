@@ -631,7 +631,7 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
                                                    const int *__restrict__ items, const int *__restrict__ desc,
                                                    const int *__restrict__ xg, const T *__restrict__ u,
                                                    const float *__restrict__ ua, const T *__restrict__ f,
-                                                   T *__restrict__ rloc) {
+                                                   T *__restrict__ rloc, T *__restrict__ uloc) {
   const int lane = threadIdx.x & 15, grp = threadIdx.x >> 4;
   const int N = op.n_udof, NA = op.n_active;
   const T *__restrict__ u1 = u;
@@ -672,6 +672,21 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
     if (r < NU_LOC) {
       const int UDOF = grow;
       const T f1 = f[UDOF], f2 = f[N + UDOF], f3 = f[2 * N + UDOF];
+      // u at the row's own dof, loaded beside f, for the color's local solve (no same-color
+      // patch writes it before that solve runs)
+      T c1 = 0, c2 = 0, c3 = 0;
+      if (uloc && lane == 0) {
+        if (sizeof(T) == 4 && ua) {
+          const float4 v = reinterpret_cast<const float4 *>(ua)[UDOF];
+          c1 = T(v.x);
+          c2 = T(v.y);
+          c3 = T(v.z);
+        } else {
+          c1 = u1[UDOF];
+          c2 = u2[UDOF];
+          c3 = u3[UDOF];
+        }
+      }
       T s1 = 0, s2 = 0, s3 = 0;
       const int *__restrict__ kc = op.a_kcol + k0;
       if constexpr (SH && sizeof(T) == 4) {
@@ -693,11 +708,18 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
         out[ip[r]] = f1 - s1;
         out[ip[NU_LOC + r]] = f2 - s2;
         out[ip[2 * NU_LOC + r]] = f3 - s3;
+        if (uloc) {
+          T *uo = uloc + long(slot) * RLOC_STRIDE;
+          uo[r] = c1;
+          uo[NU_LOC + r] = c2;
+          uo[2 * NU_LOC + r] = c3;
+        }
       }
     } else {
       const int mm = r - NU_LOC;
       const int PDOF = grow;
       const T fp = f[3 * N + PDOF];
+      const T cp = uloc && lane == 0 ? up[PDOF] : T(0);
       T s = 0;
       if constexpr (sizeof(T) == 4) {
         if (ua)
@@ -708,7 +730,10 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
         row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, u1, u2, u3, s);
       }
       s = red16(s);
-      if (lane == 0) out[ip[3 * NU_LOC + mm]] = fp - s;
+      if (lane == 0) {
+        out[ip[3 * NU_LOC + mm]] = fp - s;
+        if (uloc) uloc[long(slot) * RLOC_STRIDE + 3 * NU_LOC + mm] = cp;
+      }
     }
   }
 }
@@ -939,16 +964,16 @@ __global__ __launch_bounds__(64) void k_vanka_solve(const F *__restrict__ lu, co
 // from the lanes holding it, then u[dof] += damp * x.
 // Local solve with the explicit inverse M = (P A)^{-1}, one wavefront per patch: lane i
 // holds rows i and 64 + i of M, the permuted local residual P r is broadcast by lane reads,
-// x = M (P r) = A^{-1} r in the natural order, u += damp x.
+// x = M (P r) = A^{-1} r in the natural order, u += damp x (the dofs from pos_dofs, their
+// current values from uloc, as the color's residual launch read them).
 // REF: nref refinement steps: rr = r - K x with the FP32 patch matrix K (stored at set-up,
 // staged in LDS once), permuted like the residual (row i to position invperm[i]) and
 // multiplied by M again, x += M (P rr).
 template <typename F, bool REF>
-__global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, const int *__restrict__ cells,
+__global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, const int *__restrict__ pos_dofs,
                                                     int ncells, int pos0, const int *__restrict__ slots,
-                                                    const float *__restrict__ rloc, const int *__restrict__ u_global,
-                                                    const int *__restrict__ p_global, int N, float *__restrict__ u,
-                                                    float *__restrict__ ua, float damp,
+                                                    const float *__restrict__ rloc, const float *__restrict__ uloc,
+                                                    int N, float *__restrict__ u, float *__restrict__ ua, float damp,
                                                     const float *__restrict__ kref,
                                                     const uint8_t *__restrict__ invperm, int nref) {
   constexpr int NV = MINV_LD * int(sizeof(F)) / 16;   // 16-byte vectors per row
@@ -971,12 +996,12 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
     m1[q] = v4u{0u, 0u, 0u, 0u};
     if (two) m1[q] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(mp + minv_off<F>(64 + lane, q * PER)));
   }
-  const int cell = cells[pos];
+  const int *__restrict__ pd = pos_dofs + long(pos) * POS_DOFS_STRIDE;
   // offsets in u (and in ua, -1: a pressure row) of this lane's rows
   int o0, o1 = 0, a0i, a1i = -1;
   {
     const int c = lane / NU_LOC, j = lane - c * NU_LOC;
-    const int g = u_global[long(cell) * NU_LOC + j];
+    const int g = pd[j];
     o0 = c * N + g;
     a0i = 4 * g + c;
   }
@@ -984,16 +1009,20 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
     const int i = 64 + lane;
     if (i < 3 * NU_LOC) {
       const int c = i / NU_LOC, j = i - c * NU_LOC;
-      const int g = u_global[long(cell) * NU_LOC + j];
+      const int g = pd[j];
       o1 = c * N + g;
       a1i = 4 * g + c;
     } else {
-      o1 = 3 * N + p_global[long(cell) * NP_LOC + (i - 3 * NU_LOC)];
+      o1 = 3 * N + pd[i - 2 * NU_LOC];
     }
   }
   const float *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
   const float r0 = rs[lane];
   const float r1 = two ? rs[64 + lane] : 0.0f;
+  // the values to update, as the residual launch read them (natural local order)
+  const float *__restrict__ us = uloc + long(slot) * RLOC_STRIDE;
+  const float v0 = us[lane];
+  const float v1 = two ? us[64 + lane] : 0.0f;
   if constexpr (REF) {
     if (nref > 0) {
       // the stored matrix has the LDS layout: a straight 16-byte copy, 8 loads in flight per
@@ -1017,9 +1046,6 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
       }
     }
   }
-  // the values to update, loaded before the products (no round trip after them)
-  const float v0 = u[o0];
-  const float v1 = two ? u[o1] : 0.0f;
   // (acc0, acc1) = rows lane and 64 + lane of M times the vector held as (q0, q1)
   auto apply = [&](float q0, float q1, float &acc0, float &acc1) {
     acc0 = 0.0f;
@@ -1090,15 +1116,17 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
 // REF: nref refinement steps after the first solve: the local residual r - K x with the
 // FP32 patch matrix K (A, BT_c and B_c as the set-up gathered them, stored beside the
 // inverse data), solved with the stored data and added to x.
-// AO (ua only): the velocity is read from and written to the interleaved copy ua alone (one
-// 16-byte access per dof instead of three 4-byte ones into u and three into ua); the sweep
-// copies ua back into u after its last color.
+// AO (ua only): the velocity is written to the interleaved copy ua alone (one 16-byte store
+// per dof instead of three 4-byte ones into u and three into ua); the sweep copies ua back
+// into u after its last color.
+// Both forms take the patch dofs from pos_dofs and their current values from uloc (written
+// by the color's residual launch), so no load of the update waits on a chain of lookups.
 template <typename F, bool REF, bool AO>
-__global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ sch, const int *__restrict__ cells,
-                                                          int ncells, int pos0, const int *__restrict__ slots,
+__global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ sch,
+                                                          const int *__restrict__ pos_dofs, int ncells, int pos0,
+                                                          const int *__restrict__ slots,
                                                           const float *__restrict__ rloc,
-                                                          const int *__restrict__ u_global,
-                                                          const int *__restrict__ p_global, int N,
+                                                          const float *__restrict__ uloc, int N,
                                                           float *__restrict__ u, float *__restrict__ ua, float damp,
                                                           const float *__restrict__ kref, int nref) {
   constexpr int PER = 16 / int(sizeof(F));
@@ -1128,7 +1156,8 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
   const float *__restrict__ rs = rloc + long(slot) * RLOC_STRIDE;
   r[lane] = rs[lane];
   if (lane < NLOC - 64) r[64 + lane] = rs[64 + lane];
-  const int cell = cells[pos];
+  const int *__restrict__ pd = pos_dofs + long(pos) * POS_DOFS_STRIDE;
+  const float *__restrict__ us = uloc + long(slot) * RLOC_STRIDE;
   constexpr int KA = 0, KBT = KREF_BT, KB = KREF_B;
   if constexpr (REF) {
     if (nref > 0) {
@@ -1146,23 +1175,17 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
         if (lane + 64 * k < NQ) kl[lane + 64 * k] = tq[k];
     }
   }
+  // the dofs to update and their values as the residual launch read them
   int g0 = 0, op_ = 0;
   float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, vp = 0.0f;
   if (lane < NU_LOC) {
-    g0 = u_global[long(cell) * NU_LOC + lane];
-    if constexpr (AO) {
-      const float4 va = reinterpret_cast<const float4 *>(ua)[g0];
-      v0 = va.x;
-      v1 = va.y;
-      v2 = va.z;
-    } else {
-      v0 = u[g0];
-      v1 = u[N + g0];
-      v2 = u[2 * N + g0];
-    }
+    g0 = pd[lane];
+    v0 = us[lane];
+    v1 = us[NU_LOC + lane];
+    v2 = us[2 * NU_LOC + lane];
   } else if (lane < NU_LOC + NP_LOC) {
-    op_ = 3 * N + p_global[long(cell) * NP_LOC + (lane - NU_LOC)];
-    vp = u[op_];
+    op_ = 3 * N + pd[lane];
+    vp = us[2 * NU_LOC + lane];
   }
   __syncthreads();
   // x = M^{-1} rhs with the stored data; lane j < 27 returns x_c[j] (c = 0..2) in x[0..2],
@@ -1323,15 +1346,16 @@ void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s, float *kref
 }
 
 template <typename F>
-void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
-                              const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, const float *kref, int nref, hipStream_t s, bool ua_only) {
+void launch_vanka_apply_schur(const F *sch, const int *pos_dofs, int ncells, int pos0, const int *slots,
+                              const float *rloc, const float *uloc, int n_udof, float *u, float *ua, float damp,
+                              const float *kref, int nref, hipStream_t s, bool ua_only) {
   if (ncells <= 0) return;
   if (ua_only && !ua) throw DeviceError("launch_vanka_apply_schur: ua_only without the interleaved copy");
+  if (!pos_dofs || !uloc) throw DeviceError("launch_vanka_apply_schur: patch dofs or their values missing");
   auto go = [&](auto ref_c, auto ao_c) {
     constexpr bool R = decltype(ref_c)::value, A = decltype(ao_c)::value;
-    k_vanka_apply_schur<F, R, A><<<ncells, 64, 0, s>>>(sch, cells, ncells, pos0, slots, rloc, u_global, p_global,
-                                                       n_udof, u, ua, damp, R ? kref : nullptr, R ? nref : 0);
+    k_vanka_apply_schur<F, R, A><<<ncells, 64, 0, s>>>(sch, pos_dofs, ncells, pos0, slots, rloc, uloc, n_udof, u,
+                                                       ua, damp, R ? kref : nullptr, R ? nref : 0);
   };
   using T1 = std::true_type;
   using F0 = std::false_type;
@@ -1342,36 +1366,37 @@ void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int po
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_apply_schur<float>(const float *, const int *, int, int, const int *, const float *,
-                                              const int *, const int *, int, float *, float *, float, const float *,
-                                              int, hipStream_t, bool);
+                                              const float *, int, float *, float *, float, const float *, int,
+                                              hipStream_t, bool);
 template void launch_vanka_apply_schur<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
-                                                 const int *, const int *, int, float *, float *, float,
-                                                 const float *, int, hipStream_t, bool);
+                                                 const float *, int, float *, float *, float, const float *, int,
+                                                 hipStream_t, bool);
 
 template <typename F>
-void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
-                        const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
+void launch_vanka_apply(const F *minv, const int *pos_dofs, int ncells, int pos0, const int *slots,
+                        const float *rloc, const float *uloc, int n_udof, float *u, float *ua, float damp,
                         const float *kref, const uint8_t *invperm, int nref, hipStream_t s) {
   if (ncells <= 0) return;
+  if (!pos_dofs || !uloc) throw DeviceError("launch_vanka_apply: patch dofs or their values missing");
   if (nref > 0)
-    k_vanka_apply<F, true><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global, n_udof,
-                                                 u, ua, damp, kref, invperm, nref);
+    k_vanka_apply<F, true><<<ncells, 64, 0, s>>>(minv, pos_dofs, ncells, pos0, slots, rloc, uloc, n_udof, u, ua,
+                                                 damp, kref, invperm, nref);
   else
-    k_vanka_apply<F, false><<<ncells, 64, 0, s>>>(minv, cells, ncells, pos0, slots, rloc, u_global, p_global,
-                                                  n_udof, u, ua, damp, nullptr, invperm, 0);
+    k_vanka_apply<F, false><<<ncells, 64, 0, s>>>(minv, pos_dofs, ncells, pos0, slots, rloc, uloc, n_udof, u, ua,
+                                                  damp, nullptr, invperm, 0);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template void launch_vanka_apply<float>(const float *, const int *, int, int, const int *, const float *, const int *,
-                                        const int *, int, float *, float *, float, const float *, const uint8_t *, int,
-                                        hipStream_t);
+template void launch_vanka_apply<float>(const float *, const int *, int, int, const int *, const float *,
+                                        const float *, int, float *, float *, float, const float *, const uint8_t *,
+                                        int, hipStream_t);
 template void launch_vanka_apply<uint16_t>(const uint16_t *, const int *, int, int, const int *, const float *,
-                                           const int *, const int *, int, float *, float *, float, const float *,
+                                           const float *, int, float *, float *, float, const float *,
                                            const uint8_t *, int, hipStream_t);
 
 template <typename T>
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
                            const int *items, const int *desc, const int *xg, const T *u, const float *ua,
-                           const T *f, T *rloc, hipStream_t s, int per_block) {
+                           const T *f, T *rloc, T *uloc, hipStream_t s, int per_block) {
   if (n_items <= 0) return;
   // small colors (coarse levels) are latency bound: one work item per 16-lane group there,
   // two on the big levels
@@ -1379,17 +1404,17 @@ void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const ui
   int grid = chunk_grid(n_items, per_block, 8192);
   if (xg) grid = (grid + 7) / 8 * 8;
   if (op.a1)
-    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc);
+    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc, uloc);
   else
-    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc);
+    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc, uloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,
                                             const int *, const int *, const double *, const float *, const double *,
-                                            double *, hipStream_t, int);
+                                            double *, double *, hipStream_t, int);
 template void launch_vanka_residual<float>(const OpView<float> &, long, int, const uint8_t *, const int *,
                                            const int *, const int *, const float *, const float *, const float *,
-                                           float *, hipStream_t, int);
+                                           float *, float *, hipStream_t, int);
 
 template <typename T, typename F>
 void launch_vanka_solve(const F *lu, const int *cells, int ncells, int pos0, const int *slots, const T *rloc,

@@ -657,7 +657,8 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
       Timed tm(c, &c->tm.vanka_res, rb, 0.0, &D == &c->L.back() ? &c->tm.vanka_res_finest : nullptr);
       const int pi = sizeof(T) == 8 ? 0 : 1;
       const int *xg = !D.compacted && D.h_xcd_grouped[col] && D.tune_res_xcd[pi] ? D.xcd_goff + 9 * col : nullptr;
-      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream,
+      T *uloc = sizeof(T) == 4 ? reinterpret_cast<T *>(D.uloc32) : nullptr;
+      launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, uloc, c->stream,
                                D.tune_res_pb[pi]);
     }
     // refinement steps (stored inverses only): the stored FP32 patch matrix read once per
@@ -673,17 +674,17 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
                                          D.n_udof, u, damp, c->stream);
     } else {
       if (D.schur && D.minv16)
-        launch_vanka_apply_schur<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                           D.n_udof, u, ua, damp, D.kref, nref, c->stream, ua != nullptr);
+        launch_vanka_apply_schur<uint16_t>(D.minv16, D.pos_dofs, nc, p0, slots, rloc, D.uloc32, D.n_udof, u, ua,
+                                           damp, D.kref, nref, c->stream, ua != nullptr);
       else if (D.schur && D.minv32)
-        launch_vanka_apply_schur<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
-                                        D.n_udof, u, ua, damp, D.kref, nref, c->stream, ua != nullptr);
+        launch_vanka_apply_schur<float>(D.minv32, D.pos_dofs, nc, p0, slots, rloc, D.uloc32, D.n_udof, u, ua, damp,
+                                        D.kref, nref, c->stream, ua != nullptr);
       else if (D.minv16)
-        launch_vanka_apply<uint16_t>(D.minv16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof,
-                                     u, ua, damp, D.kref, D.invperm, nref, c->stream);
+        launch_vanka_apply<uint16_t>(D.minv16, D.pos_dofs, nc, p0, slots, rloc, D.uloc32, D.n_udof, u, ua, damp,
+                                     D.kref, D.invperm, nref, c->stream);
       else if (D.minv32)
-        launch_vanka_apply<float>(D.minv32, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global, D.n_udof, u,
-                                  ua, damp, D.kref, D.invperm, nref, c->stream);
+        launch_vanka_apply<float>(D.minv32, D.pos_dofs, nc, p0, slots, rloc, D.uloc32, D.n_udof, u, ua, damp,
+                                  D.kref, D.invperm, nref, c->stream);
       else if (c->prm.precision == PMG_PREC_FP32_BF16)
         launch_vanka_solve<float, uint16_t>(D.lu16, D.cell_reorder, nc, p0, slots, rloc, D.u_global, D.p_global,
                                             D.n_udof, u, damp, c->stream);
@@ -1862,6 +1863,17 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
     for (int k = 0; k < nc; ++k) D.max_cells_per_color = std::max(D.max_cells_per_color, ptr[k + 1] - ptr[k]);
     D.cell_reorder = dalloc<int>(c, D.n_cells); h2d(c, D.cell_reorder, reo.data(), D.n_cells);
     {
+      std::vector<int> pd(size_t(D.n_cells) * POS_DOFS_STRIDE, 0);
+      for (int q = 0; q < D.n_cells; ++q) {
+        const int cell = reo[q];
+        std::copy(&ug[size_t(cell) * NU_LOC], &ug[size_t(cell) * NU_LOC] + NU_LOC, &pd[size_t(q) * POS_DOFS_STRIDE]);
+        std::copy(&pg[size_t(cell) * NP_LOC], &pg[size_t(cell) * NP_LOC] + NP_LOC,
+                  &pd[size_t(q) * POS_DOFS_STRIDE + NU_LOC]);
+      }
+      D.pos_dofs = dalloc<int>(c, pd.size());
+      h2d(c, D.pos_dofs, pd.data(), pd.size());
+    }
+    {
       // residual work items of each color sorted by the operator row they stream
       std::vector<int> items(size_t(D.n_cells) * (NU_LOC + NP_LOC)), rows(items.size());
       // Large colors of a single problem (>= XCD_GROUP_MIN cells) are split into 8 groups of
@@ -2012,6 +2024,7 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
       D.patch_info = dalloc<int>(c, D.n_cells);
       D.rloc = dalloc<double>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
       if (mixed(c)) D.rloc32 = dalloc<float>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
+      if (D.minv32 || D.minv16) D.uloc32 = dalloc<float>(c, size_t(D.max_cells_per_color) * RLOC_STRIDE);
       // transfers from the coarser level
       const DevLevel &C = c->L[lvl - 1];
         for (int fc = 0; fc < D.n_cells; ++fc) {
@@ -2234,6 +2247,7 @@ static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
     const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
     const int *desc = D.res_desc + 4 * size_t(p0) * (NU_LOC + NP_LOC);
     const bool grouped = D.h_xcd_grouped[0];
+    T *uloc = sizeof(T) == 4 ? reinterpret_cast<T *>(D.uloc32) : nullptr;
     // three rounds over the candidates, each candidate's fastest round counts (the rounds
     // interleave the candidates, so a slow phase of the device does not favour one)
     std::vector<std::pair<int, int>> cand;
@@ -2245,7 +2259,7 @@ static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
         const int *xg = grouped && cand[k].first ? D.xcd_goff : nullptr;
         const int pb = cand[k].second;
         best[k] = std::min(best[k], time_launches(c, reps, [&] {
-          launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, c->stream, pb);
+          launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, uloc, c->stream, pb);
         }));
       }
     const size_t kb = size_t(std::min_element(best.begin(), best.end()) - best.begin());

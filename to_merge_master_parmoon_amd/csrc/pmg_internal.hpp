@@ -35,6 +35,7 @@ constexpr int SCH_AINV = 0, SCH_W = 729, SCH_B = SCH_W + 324, SCH_SINV = SCH_B +
 constexpr int SCH_STRIDE = 1408;    // 1393 padded to a multiple of 16 entries
 static_assert(SCH_SINV + 16 <= SCH_STRIDE && SCH_STRIDE <= MINV_STRIDE, "block-form patch stride");
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
+constexpr int POS_DOFS_STRIDE = 32; // dofs of the patch at a color position (27 + 4 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
 constexpr int RED_BLOCKS = 4096;    // max blocks of the kernels with a last-block reduction
 constexpr int ROW_PAD = 64;         // zeroed tail entries of the streamed operator arrays
@@ -143,6 +144,9 @@ struct DevLevel {
   int n_colors = 0, max_cells_per_color = 0;
   std::vector<int> ptr_colors;
   int *cell_reorder = nullptr;
+  // the patch dofs at every color position (velocity dofs 0..26, pressure dofs 27..30):
+  // the local solves read them without the cell -> dof map chain
+  int *pos_dofs = nullptr;
   // patch factors in color order (position = index into cell_reorder)
   double *lu = nullptr;
   float *lu32 = nullptr;
@@ -177,6 +181,9 @@ struct DevLevel {
   float *sol32 = nullptr, *rhs32 = nullptr, *aux0_32 = nullptr, *aux1_32 = nullptr;
   double *rloc = nullptr;            // max_cells_per_color * RLOC_STRIDE
   float *rloc32 = nullptr;
+  // stored inverses: u at each patch's dofs as the residual read it (natural local order,
+  // RLOC_STRIDE per slot), so the local solve does not gather it again
+  float *uloc32 = nullptr;
   // host copies kept for value updates / coarse assembly
   std::vector<int> h_a_rowptr, h_a_kcol, h_b_rowptr, h_b_kcol, h_bt_rowptr, h_bt_kcol;
   // algorithmic bytes (FP64 operator) of the per-color residual launches and of one defect
@@ -285,28 +292,30 @@ void launch_patch_lu(const DevLevel &L, int *info, hipStream_t s);
 void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s, float *kref = nullptr);
 // nref > 0: that many refinement steps with the FP32 patch matrices kref (KREF_FULL per
 // patch) after the first product (residual permuted by invperm like the local residual)
+// pos_dofs: the level's patch dofs by color position; uloc: u at the patch dofs as the
+// color's residual launch wrote it (natural local order, RLOC_STRIDE per slot)
 template <typename F>
-void launch_vanka_apply(const F *minv, const int *cells, int ncells, int pos0, const int *slots, const float *rloc,
-                        const int *u_global, const int *p_global, int n_udof, float *u, float *ua, float damp,
+void launch_vanka_apply(const F *minv, const int *pos_dofs, int ncells, int pos0, const int *slots,
+                        const float *rloc, const float *uloc, int n_udof, float *u, float *ua, float damp,
                         const float *kref, const uint8_t *invperm, int nref, hipStream_t s);
 // block form (shared velocity block): set-up into minv32 / minv16 with SCH_STRIDE per patch,
 // and its local solves
 void launch_patch_schur(const DevLevel &L, int *info, hipStream_t s, float *kref = nullptr);
 template <typename F>
-void launch_vanka_apply_schur(const F *sch, const int *cells, int ncells, int pos0, const int *slots,
-                              const float *rloc, const int *u_global, const int *p_global, int n_udof, float *u,
-                              float *ua, float damp, const float *kref, int nref, hipStream_t s,
-                              bool ua_only = false);
+void launch_vanka_apply_schur(const F *sch, const int *pos_dofs, int ncells, int pos0, const int *slots,
+                              const float *rloc, const float *uloc, int n_udof, float *u, float *ua, float damp,
+                              const float *kref, int nref, hipStream_t s, bool ua_only = false);
 // the velocity of the interleaved copy ua back into u (u_c[i] = ua[4 i + c])
 void launch_ua_to_u(const float *ua, float *u, int N, hipStream_t s);
 template <typename T>
 // n_items (slot * 32 + local row) work items of one color starting at position pos0, and
 // the global row of each (velocity or pressure dof)
 // xg (may be null): the 9 item offsets of the color's 8 XCD groups; per_block: work items
-// per workgroup (0: 16 below 65,536 items, 32 above)
+// per workgroup (0: 16 below 65,536 items, 32 above); uloc (may be null): receives u at
+// every patch dof (natural local order) for the stored-inverse local solves
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
                            const int *items, const int *desc, const int *xg, const T *u, const float *ua,
-                           const T *f, T *rloc, hipStream_t s, int per_block = 0);
+                           const T *f, T *rloc, T *uloc, hipStream_t s, int per_block = 0);
 // FP32 sweep start: Dirichlet slices of u from f, ua[4 i + c] = u_c(i) (c < 3)
 void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s);
 // descriptors of residual work items (4 ints each) from their items and global rows
