@@ -510,12 +510,13 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const double *__restrict__ f
 
 // y[c*ys + r] = sum_k val[k] x[c*xs + col[k]] for the NC components, 16 lanes per row
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void k_csr(int n_rows, const int *__restrict__ rp, const int *__restrict__ col,
-                                             const double *__restrict__ val, const T *__restrict__ x, long xs,
-                                             T *__restrict__ y, long ys) {
+__global__ __launch_bounds__(256) void k_csr(int n_rows, const int *__restrict__ rows, const int *__restrict__ rp,
+                                             const int *__restrict__ col, const double *__restrict__ val,
+                                             const T *__restrict__ x, long xs, T *__restrict__ y, long ys) {
   const int lane = threadIdx.x & 15;
   const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4, ngrp = gridDim.x * 16;
-  for (int r = grp; r < n_rows; r += ngrp) {
+  for (int i = grp; i < n_rows; i += ngrp) {
+    const int r = rows ? rows[i] : i;
     const int k0 = rp[r], k1 = rp[r + 1];
     T s[NC];
 #pragma unroll
@@ -625,16 +626,20 @@ void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, doubl
 }
 
 template <typename T>
-void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long xs, T *y, long ys, hipStream_t s) {
-  if (m.n_rows <= 0) return;
-  const int grid = std::min(8192, (m.n_rows + 15) / 16);
+void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long xs, T *y, long ys, hipStream_t s, const int *rows,
+                     int n_rows) {
+  const int n = rows ? n_rows : m.n_rows;
+  if (n <= 0) return;
+  const int grid = std::min(8192, (n + 15) / 16);
   if (ncomp == 3)
-    k_csr<T, 3><<<grid, 256, 0, s>>>(m.n_rows, m.rowptr, m.col, m.val, x, xs, y, ys);
+    k_csr<T, 3><<<grid, 256, 0, s>>>(n, rows, m.rowptr, m.col, m.val, x, xs, y, ys);
   else
-    k_csr<T, 1><<<grid, 256, 0, s>>>(m.n_rows, m.rowptr, m.col, m.val, x, xs, y, ys);
+    k_csr<T, 1><<<grid, 256, 0, s>>>(n, rows, m.rowptr, m.col, m.val, x, xs, y, ys);
   PMG_HIP_CHECK(hipGetLastError());
 }
-template void launch_csr_spmv<double>(const Csr &, int, const double *, long, double *, long, hipStream_t);
-template void launch_csr_spmv<float>(const Csr &, int, const float *, long, float *, long, hipStream_t);
+template void launch_csr_spmv<double>(const Csr &, int, const double *, long, double *, long, hipStream_t,
+                                      const int *, int);
+template void launch_csr_spmv<float>(const Csr &, int, const float *, long, float *, long, hipStream_t, const int *,
+                                     int);
 
 }  // namespace pmg

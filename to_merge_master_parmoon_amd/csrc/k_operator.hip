@@ -80,10 +80,11 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
   const T *__restrict__ x3 = x + 2 * N;
   const T *__restrict__ xp = x + 3 * N;
   double pc = 0.0, sq = 0.0;
-  // this block's chunk of rows
+  // this block's chunk of rows (of the listed rows, when the level runs a subset)
   int r0, r1;
-  chunk_of_block(long(N) + NP, r0, r1);
-  for (int r = r0 + grp; r < r1; r += 16) {
+  chunk_of_block(op.rows ? long(op.n_rows) : long(N) + NP, r0, r1);
+  for (int ri = r0 + grp; ri < r1; ri += 16) {
+    const int r = op.rows ? op.rows[ri] : ri;
     if (r < N) {
       T s1 = 0, s2 = 0, s3 = 0;
       const int k0 = op.a_rowptr[r], n = op.a_rowptr[r + 1] - k0;
@@ -322,9 +323,13 @@ inline int grid_for(long n, int bs = 256, int cap = 8192) {
 template <typename T>
 void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project, double *res,
                   RedScratch &red, hipStream_t s) {
-  const int grid = chunk_grid(long(op.n_udof) + op.n_pdof, op.rpb > 0 ? op.rpb : 64, RED_BLOCKS);
+  const long n_rows = op.rows ? long(op.n_rows) : long(op.n_udof) + op.n_pdof;
   const int want_norm = res != nullptr;
   const int proj = (mode != 0 && project) ? 1 : 0;
+  if (op.rows && (want_norm || proj))
+    throw DeviceError("launch_type4: a row subset is supported without the norm and the projection only");
+  if (n_rows <= 0) return;
+  const int grid = chunk_grid(n_rows, op.rpb > 0 ? op.rpb : 64, RED_BLOCKS);
   auto go = [&](auto mode_c, int pr, int wn, double *rs, double *sl) {
     constexpr int M = decltype(mode_c)::value;
     if (op.a1)

@@ -68,6 +68,13 @@ struct pmg_ctx {
   int4 *seg_ch = nullptr;                  // FGMRES chunk table over the merged finest vector
   int n_seg_ch = 0;
   int *seg_cptr = nullptr;                 // chunks of problem b: [seg_cptr[b], seg_cptr[b+1])
+  // the chunks the FGMRES steps run: all of them, or (finished problems left out) those of
+  // the unfinished problems in the same order, with the per-problem ranges into that list
+  int4 *seg_ch_act = nullptr, *seg_ch_live = nullptr;
+  int n_seg_ch_act = 0;
+  int *seg_cptr_act = nullptr, *seg_cptr_live = nullptr;
+  std::vector<int4> h_seg_ch;
+  std::vector<int> h_seg_cptr;
   double *seg_part = nullptr;              // one partial sum per chunk
   double *bX = nullptr, *bB = nullptr;     // merged finest solution and right-hand side
   double *bH = nullptr;                    // per problem: MGS coefficients of one iteration
@@ -717,14 +724,20 @@ void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, doub
 template <typename T>
 void restrict_defect(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *d, T *fc) {
   Timed tm(c, &c->tm.transfer, F.transfer_bytes, 0.0);
-  launch_csr_spmv<T>(F.ru, 3, d, F.n_udof, fc, C.n_udof, c->stream);
-  launch_csr_spmv<T>(F.rp, 1, d + 3L * F.n_udof, 0, fc + 3L * C.n_udof, 0, c->stream);
+  // (batch with finished problems) the coarse rows of the unfinished problems only
+  const bool sub = C.act_lists;
+  launch_csr_spmv<T>(F.ru, 3, d, F.n_udof, fc, C.n_udof, c->stream, sub ? C.act_u : nullptr, C.n_act_u);
+  launch_csr_spmv<T>(F.rp, 1, d + 3L * F.n_udof, 0, fc + 3L * C.n_udof, 0, c->stream, sub ? C.act_p : nullptr,
+                     C.n_act_p);
 }
 template <typename T>
 void prolongate(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *uc, T *d) {
   Timed tm(c, &c->tm.transfer, F.transfer_bytes, 0.0);
-  launch_csr_spmv<T>(F.pu, 3, uc, C.n_udof, d, F.n_udof, c->stream);
-  launch_csr_spmv<T>(F.pp, 1, uc + 3L * C.n_udof, 0, d + 3L * F.n_udof, 0, c->stream);
+  // (batch with finished problems) the fine rows of the unfinished problems only
+  const bool sub = F.act_lists;
+  launch_csr_spmv<T>(F.pu, 3, uc, C.n_udof, d, F.n_udof, c->stream, sub ? F.act_u : nullptr, F.n_act_u);
+  launch_csr_spmv<T>(F.pp, 1, uc + 3L * C.n_udof, 0, d + 3L * F.n_udof, 0, c->stream, sub ? F.act_p : nullptr,
+                     F.n_act_p);
 }
 
 // one coarse visit: the block form (explicit inverses of the blocks) or the LU sweeps
@@ -768,7 +781,7 @@ void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
 template <typename T>
 void batch_coarse(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
   for (int k : D.coarse_insts) {
-    if (!c->b_live.empty() && !c->b_live[k]) continue;   // finished: its input is zero
+    if (!c->b_live.empty() && !c->b_live[k]) continue;   // finished: left out, its result is unused
     BatchProblem &b = c->bi[k];
     if (b.co.pending) {
       PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, b.co.ev_done, 0));
@@ -1117,6 +1130,11 @@ void ensure_fgmres(pmg_ctx *c) {
   c->V = dalloc<double>(c, size_t(m + 1) * n);
   c->Z = dalloc<double>(c, size_t(m + 1) * n);
   c->W = dalloc<double>(c, n);
+  // (batch) the FGMRES steps leave the parts of finished problems untouched once they are
+  // left out, so those parts only ever hold zeros or earlier finite values
+  launch_fill<double>(c->V, 0.0, long(m + 1) * n, c->stream);
+  launch_fill<double>(c->Z, 0.0, long(m + 1) * n, c->stream);
+  launch_fill<double>(c->W, 0.0, n, c->stream);
   c->Hdev = dalloc<double>(c, m + 2);
   c->coef = dalloc<double>(c, m + 1);
   if (2 * m + 8 > c->hpin_len) {
@@ -1275,8 +1293,8 @@ void ensure_hpin(pmg_ctx *c, long n) {
 
 // per-problem <x, y> -> out[b * ldo] (device), deterministic
 void seg_dot(pmg_ctx *c, const double *x, const double *y, double *out, int ldo) {
-  launch_seg_dot(x, y, c->seg_ch, c->n_seg_ch, c->seg_part, c->stream);
-  launch_seg_reduce(c->seg_part, c->seg_cptr, int(c->bi.size()), out, ldo, c->stream);
+  launch_seg_dot(x, y, c->seg_ch_act, c->n_seg_ch_act, c->seg_part, c->stream);
+  launch_seg_reduce(c->seg_part, c->seg_cptr_act, int(c->bi.size()), out, ldo, c->stream);
 }
 
 struct BState {
@@ -1296,8 +1314,10 @@ void back_solve(std::vector<std::vector<double>> &H, std::vector<double> &s, int
   }
 }
 
-// Leave finished problems out of the merged sweeps and coarse solves.  Their FGMRES input
-// is zero from then on, so their V-cycle output stays zero either way; the lists are only
+// Leave finished problems out of the merged sweeps, coarse solves, operator rows,
+// transfers and FGMRES vector steps.  The merged levels are block diagonal, so the rows of
+// the unfinished problems never read a finished problem's entries, whatever those hold;
+// each finished problem's solution was combined when it stopped.  The lists are only
 // rebuilt (and the captured graph dropped) once the live patches fall below 3/4 of those
 // at the last rebuild, or when every problem is live again.
 void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
@@ -1309,16 +1329,64 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
   }
   const bool all = cells == total;
   if (all) {
-    bool any = false;
-    for (size_t l = 1; l < c->L.size(); ++l) any = any || c->L[l].compacted;
+    bool any = c->seg_ch_act != c->seg_ch;
+    for (const auto &D : c->L) any = any || D.compacted || D.act_lists;
     c->b_live.assign(live.size(), 1);
     c->b_live_cells = total;
     if (!any) return;
-    for (size_t l = 1; l < c->L.size(); ++l) c->L[l].compacted = false;
+    for (auto &D : c->L) D.compacted = D.act_lists = false;
+    c->seg_ch_act = c->seg_ch;
+    c->n_seg_ch_act = c->n_seg_ch;
+    c->seg_cptr_act = c->seg_cptr;
     drop_graph(c);
     return;
   }
   if (c->b_live_cells > 0 && cells * 4 > c->b_live_cells * 3) return;
+  // the operator rows and transfer rows of the unfinished problems on every level (the
+  // levels are block diagonal: those rows read only their own problems' entries)
+  for (auto &D : c->L) {
+    if (D.h_udof_problem.empty()) continue;
+    if (!D.act_u) {
+      D.act_u = dalloc<int>(c, D.n_udof);
+      D.act_p = dalloc<int>(c, D.n_pdof);
+      D.act_oprows = dalloc<int>(c, size_t(D.n_udof) + D.n_pdof);
+    }
+    std::vector<int> u, p, rows;
+    for (int i = 0; i < D.n_udof; ++i)
+      if (live[D.h_udof_problem[i]]) u.push_back(i);
+    for (int i = 0; i < D.n_pdof; ++i)
+      if (live[D.h_pdof_problem[i]]) p.push_back(i);
+    rows = u;
+    for (int i : p) rows.push_back(D.n_udof + i);
+    h2d(c, D.act_u, u.data(), u.size());
+    h2d(c, D.act_p, p.data(), p.size());
+    h2d(c, D.act_oprows, rows.data(), rows.size());
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    D.n_act_u = int(u.size());
+    D.n_act_p = int(p.size());
+    D.act_lists = true;
+  }
+  {
+    // the FGMRES chunks of the unfinished problems, each problem's in their original order
+    // (so its partial sums and their reduction order stay those of the full table)
+    std::vector<int4> ch;
+    std::vector<int> cp(1, 0);
+    for (size_t b = 0; b < c->bi.size(); ++b) {
+      if (live[b])
+        for (int k = c->h_seg_cptr[b]; k < c->h_seg_cptr[b + 1]; ++k) ch.push_back(c->h_seg_ch[k]);
+      cp.push_back(int(ch.size()));
+    }
+    if (!c->seg_ch_live) {
+      c->seg_ch_live = dalloc<int4>(c, size_t(c->n_seg_ch));
+      c->seg_cptr_live = dalloc<int>(c, c->bi.size() + 1);
+    }
+    h2d(c, c->seg_ch_live, ch.data(), ch.size());
+    h2d(c, c->seg_cptr_live, cp.data(), cp.size());
+    PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
+    c->seg_ch_act = c->seg_ch_live;
+    c->n_seg_ch_act = int(ch.size());
+    c->seg_cptr_act = c->seg_cptr_live;
+  }
   for (size_t l = 1; l < c->L.size(); ++l) {
     DevLevel &D = c->L[l];
     if (!D.act_items) {
@@ -1410,7 +1478,7 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
   auto combine = [&](int m) {
     PMG_HIP_CHECK(hipMemcpyAsync(dC, hC, sizeof(double) * nb * ldc, hipMemcpyHostToDevice, c->stream));
     PMG_HIP_CHECK(hipMemcpyAsync(c->bmask, hM, sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
-    launch_seg_lincomb(c->bX, c->Z, n, dC, ldc, m, c->bmask, c->seg_ch, c->n_seg_ch, c->stream);
+    launch_seg_lincomb(c->bX, c->Z, n, dC, ldc, m, c->bmask, c->seg_ch_act, c->n_seg_ch_act, c->stream);
   };
   std::vector<char> live(nb, 1);
   batch_compact(c, live);
@@ -1434,7 +1502,7 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
   while (j <= maxit && any_active()) {
     for (int b = 0; b < nb; ++b) hS[b] = B[b].active ? 1.0 / B[b].beta : 0.0;
     PMG_HIP_CHECK(hipMemcpyAsync(dS, hS, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
-    launch_seg_scale(v(0), v(0), dS, c->seg_ch, c->n_seg_ch, c->stream);
+    launch_seg_scale(v(0), v(0), dS, c->seg_ch_act, c->n_seg_ch_act, c->stream);
     for (auto &S : B) {
       std::fill(S.s.begin(), S.s.end(), 0.0);
       S.s[0] = S.beta;
@@ -1449,8 +1517,8 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
       seg_dot(c, c->W, v(0), c->bH, ldh);
       for (int k = 0; k <= i; ++k) {
         const double *vn = (k < i) ? v(k + 1) : c->W;
-        launch_seg_axpy_dot(c->W, c->bH + k, ldh, v(k), vn, c->seg_ch, c->n_seg_ch, c->seg_part, c->stream);
-        launch_seg_reduce(c->seg_part, c->seg_cptr, nb, c->bH + k + 1, ldh, c->stream);
+        launch_seg_axpy_dot(c->W, c->bH + k, ldh, v(k), vn, c->seg_ch_act, c->n_seg_ch_act, c->seg_part, c->stream);
+        launch_seg_reduce(c->seg_part, c->seg_cptr_act, nb, c->bH + k + 1, ldh, c->stream);
       }
       PMG_HIP_CHECK(hipMemcpyAsync(hH, c->bH, sizeof(double) * nb * ldh, hipMemcpyDeviceToHost, c->stream));
       PMG_HIP_CHECK(hipStreamSynchronize(c->stream));
@@ -1490,7 +1558,7 @@ void fgmres_batch(pmg_ctx *c, std::vector<int> &iters, std::vector<double> &fina
       }
       // v_{i+1} = w / H[i+1][i] (a zero norm copies w, as the single-problem path does)
       PMG_HIP_CHECK(hipMemcpyAsync(dS, hS, sizeof(double) * nb, hipMemcpyHostToDevice, c->stream));
-      launch_seg_scale(v(i + 1), c->W, dS, c->seg_ch, c->n_seg_ch, c->stream);
+      launch_seg_scale(v(i + 1), c->W, dS, c->seg_ch_act, c->n_seg_ch_act, c->stream);
       if (finish) combine(i + 1);
       recompact();
     }
@@ -2648,6 +2716,16 @@ int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg
       if (int r = add_level(c, &M.L[l].d, &lvl, true)) throw Fail{r, c->err};
       DevLevel &D = c->L[l];
       D.coarse_insts = M.L[l].coarse;
+      {
+        const MergedLevel &ML = M.L[l];
+        D.h_udof_problem.assign(D.n_udof, 0);
+        D.h_pdof_problem.assign(D.n_pdof, 0);
+        for (int cell = 0; cell < D.n_cells; ++cell) {
+          const int b = ML.cell_problem[cell];
+          for (int j = 0; j < NU_LOC; ++j) D.h_udof_problem[ML.u_global[ML.u_begin[cell] + j]] = b;
+          for (int j = 0; j < NP_LOC; ++j) D.h_pdof_problem[ML.p_global[ML.p_begin[cell] + j]] = b;
+        }
+      }
       if (l > 0) {
         std::vector<int> reo(D.n_cells);
         PMG_HIP_CHECK(hipMemcpy(reo.data(), D.cell_reorder, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost));
@@ -2691,6 +2769,11 @@ int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg
     h2d(c, c->seg_ch, chunks.data(), chunks.size());
     c->seg_cptr = dalloc<int>(c, cptr.size());
     h2d(c, c->seg_cptr, cptr.data(), cptr.size());
+    c->seg_ch_act = c->seg_ch;
+    c->n_seg_ch_act = c->n_seg_ch;
+    c->seg_cptr_act = c->seg_cptr;
+    c->h_seg_ch = chunks;
+    c->h_seg_cptr = cptr;
     c->seg_part = dalloc<double>(c, chunks.size());
     const long n = c->L.back().n_dof;
     c->bX = dalloc<double>(c, n);

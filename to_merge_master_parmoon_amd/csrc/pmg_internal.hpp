@@ -87,6 +87,10 @@ struct OpView {
   // the active rows (the Galerkin Oseen form); nullptr: the nine blocks differ
   const T *a1;
   int rpb;                  // operator rows per workgroup of the type-4 kernels (0: default)
+  // (batched levels with finished problems) the type-4 kernels run these rows only: a
+  // velocity dof r < n_udof, or n_udof + a pressure dof; nullptr: every row
+  const int *rows = nullptr;
+  int n_rows = 0;
 };
 
 // Streaming kernels split their n items (rows, residual work items) into one contiguous
@@ -205,6 +209,13 @@ struct DevLevel {
   std::vector<int> h_pos_problem;    // problem of the cell at each color position
   std::vector<long> act_items_off, act_slots_off;   // per color (n_colors + 1)
   int *act_items = nullptr, *act_rows = nullptr, *act_slots = nullptr, *act_desc = nullptr;
+  // the problem of every velocity and pressure dof of a merged level, and (act_lists) the
+  // dofs of the unfinished problems in ascending order: velocity dofs act_u, pressure dofs
+  // act_p, both as type-4 operator rows act_oprows (velocity dof, n_udof + pressure dof)
+  std::vector<int> h_udof_problem, h_pdof_problem;
+  bool act_lists = false;
+  int *act_u = nullptr, *act_p = nullptr, *act_oprows = nullptr;
+  int n_act_u = 0, n_act_p = 0;
   double *geom = nullptr;            // n_cells * 8 corners * 3 (device re-assembly)
 
   // launch shapes chosen by pmg_autotune (0: the built-in defaults): residual work items per
@@ -215,14 +226,24 @@ struct DevLevel {
   int tune_op_rpb[2] = {0, 0};
 
   OpView<double> op64() const {
-    return OpView<double>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
-                          b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz,
-                          av64, bv64, btv64, a_shared ? a : nullptr, tune_op_rpb[0]};
+    OpView<double> o{n_udof, n_active, n_pdof, a_rowptr, a_kcol, a, a_nnz,
+                     b_rowptr, b_kcol, b, b_nnz, bt_rowptr, bt_kcol, bt, bt_nnz,
+                     av64, bv64, btv64, a_shared ? a : nullptr, tune_op_rpb[0]};
+    if (act_lists) {
+      o.rows = act_oprows;
+      o.n_rows = n_act_u + n_act_p;
+    }
+    return o;
   }
   OpView<float> op32() const {
-    return OpView<float>{n_udof, n_active, n_pdof, a_rowptr, a_kcol, nullptr, a_nnz,
-                         b_rowptr, b_kcol, nullptr, b_nnz, bt_rowptr, bt_kcol, nullptr, bt_nnz,
-                         av32, bv32, btv32, a_shared ? a1_32 : nullptr, tune_op_rpb[1]};
+    OpView<float> o{n_udof, n_active, n_pdof, a_rowptr, a_kcol, nullptr, a_nnz,
+                    b_rowptr, b_kcol, nullptr, b_nnz, bt_rowptr, bt_kcol, nullptr, bt_nnz,
+                    av32, bv32, btv32, a_shared ? a1_32 : nullptr, tune_op_rpb[1]};
+    if (act_lists) {
+      o.rows = act_oprows;
+      o.n_rows = n_act_u + n_act_p;
+    }
+    return o;
   }
 };
 
@@ -349,10 +370,11 @@ void launch_dirichlet_slices(T *u, const T *f, int n_udof, int start, int count,
 template <typename T>
 void launch_l20(T *p, int n_pdof, RedScratch &red, hipStream_t s);
 
-// transfers: y[c*ny + r] = sum_k val[k] x[c*nx + col[k]], c < ncomp
+// transfers: y[c*ny + r] = sum_k val[k] x[c*nx + col[k]], c < ncomp, for every row r, or
+// (rows non-null) for the n_rows rows listed
 template <typename T>
 void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long x_stride, T *y, long y_stride,
-                     hipStream_t s);
+                     hipStream_t s, const int *rows = nullptr, int n_rows = 0);
 
 // dense coarse LU / solve
 void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s);
