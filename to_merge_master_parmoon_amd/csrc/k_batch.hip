@@ -56,35 +56,87 @@ __global__ __launch_bounds__(256) void k_seg_dot(const double *__restrict__ x, c
                                                  const int4 *__restrict__ ch, double *__restrict__ part) {
   __shared__ double sh[4];
   const int4 c = ch[blockIdx.x];
+  const double *__restrict__ xc = x + c.x;
+  const double *__restrict__ yc = y + c.x;
   double s = 0;
-  for (int i = threadIdx.x; i < c.y; i += 256) s += x[c.x + i] * y[c.x + i];
+  // four of a thread's entries loaded before they are added in their order (a chunk is
+  // latency bound when few chunks share the device)
+  int i = threadIdx.x;
+  for (; i + 3 * 256 < c.y; i += 4 * 256) {
+    double a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a[j] = xc[i + j * 256];
+      b[j] = yc[i + j * 256];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s += a[j] * b[j];
+  }
+  for (; i < c.y; i += 256) s += xc[i] * yc[i];
   s = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// w -= h[problem * ldh] v ; partial of <w, vn>
-__global__ __launch_bounds__(256) void k_seg_axpy_dot(double *__restrict__ w, const double *__restrict__ h, int ldh,
-                                                      const double *__restrict__ v, const double *__restrict__ vn,
+// w -= h[problem * ldh] v ; partial of <w, vn> (vn may be w itself: the new norm)
+__global__ __launch_bounds__(256) void k_seg_axpy_dot(double *w, const double *__restrict__ h, int ldh,
+                                                      const double *__restrict__ v, const double *vn,
                                                       const int4 *__restrict__ ch, double *__restrict__ part) {
   __shared__ double sh[4];
   const int4 c = ch[blockIdx.x];
   const double a = -h[long(c.z) * ldh];
+  const bool self = vn == w;
+  double *__restrict__ wc = w + c.x;
+  const double *__restrict__ vc = v + c.x;
+  const double *__restrict__ nc = (vn && !self) ? vn + c.x : nullptr;
   double s = 0;
-  for (int i = threadIdx.x; i < c.y; i += 256) {
-    const double wi = w[c.x + i] + a * v[c.x + i];
-    w[c.x + i] = wi;
-    if (vn) s += wi * vn[c.x + i];
+  // as k_seg_dot: four of a thread's entries in flight, added in their order
+  int i = threadIdx.x;
+  for (; i + 3 * 256 < c.y; i += 4 * 256) {
+    double wv[4], vv[4], nv[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      wv[j] = wc[i + j * 256];
+      vv[j] = vc[i + j * 256];
+      nv[j] = nc ? nc[i + j * 256] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double wi = wv[j] + a * vv[j];
+      wc[i + j * 256] = wi;
+      if (self)
+        s += wi * wi;
+      else if (nc)
+        s += wi * nv[j];
+    }
+  }
+  for (; i < c.y; i += 256) {
+    const double wi = wc[i] + a * vc[i];
+    wc[i] = wi;
+    if (self)
+      s += wi * wi;
+    else if (nc)
+      s += wi * nc[i];
   }
   if (!vn) return;
   s = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// out[b * ldo] = sum of problem b's partials, in chunk order
+// out[b * ldo] = sum of problem b's partials, in chunk order (eight loads in flight ahead
+// of the additions, which keep that order)
 __global__ void k_seg_reduce(const double *__restrict__ part, const int *__restrict__ cptr, int nb,
                              double *__restrict__ out, int ldo) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
+  const int k1 = cptr[b + 1];
+  int k = cptr[b];
   double s = 0;
-  for (int k = cptr[b]; k < cptr[b + 1]; ++k) s += part[k];
+  for (; k + 8 <= k1; k += 8) {
+    double t[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[j] = part[k + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += t[j];
+  }
+  for (; k < k1; ++k) s += part[k];
   out[long(b) * ldo] = s;
 }
 // y = coef[problem] x; a zero coefficient writes zeros whatever x holds

@@ -1352,10 +1352,13 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
       D.act_oprows = dalloc<int>(c, size_t(D.n_udof) + D.n_pdof);
     }
     std::vector<int> u, p, rows;
-    for (int i = 0; i < D.n_udof; ++i)
-      if (live[D.h_udof_problem[i]]) u.push_back(i);
-    for (int i = 0; i < D.n_pdof; ++i)
-      if (live[D.h_pdof_problem[i]]) p.push_back(i);
+    for (const auto &r : D.h_urun)
+      if (live[r.prob])
+        for (long i = r.b; i < r.e; ++i) u.push_back(int(i));
+    for (const auto &r : D.h_prun)
+      if (live[r.prob])
+        for (long i = r.b; i < r.e; ++i) p.push_back(int(i));
+    rows.reserve(u.size() + p.size());
     rows = u;
     for (int i : p) rows.push_back(D.n_udof + i);
     h2d(c, D.act_u, u.data(), u.size());
@@ -1402,12 +1405,11 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
     D.act_slots_off.assign(1, 0);
     for (int col = 0; col < D.n_colors; ++col) {
       const int p0 = D.ptr_colors[col], p1 = D.ptr_colors[col + 1];
-      for (long q = long(p0) * (NU_LOC + NP_LOC); q < long(p1) * (NU_LOC + NP_LOC); ++q) {
-        const int it = D.h_res_items[q];
-        if (live[D.h_pos_problem[p0 + (it >> 5)]]) {
-          items.push_back(it);
-          rows.push_back(D.h_res_rows[q]);
-        }
+      for (long k = D.h_irun_off[col]; k < D.h_irun_off[col + 1]; ++k) {
+        const auto &r = D.h_irun[k];
+        if (!live[r.prob]) continue;
+        items.insert(items.end(), D.h_res_items.begin() + r.b, D.h_res_items.begin() + r.e);
+        rows.insert(rows.end(), D.h_res_rows.begin() + r.b, D.h_res_rows.begin() + r.e);
       }
       for (int sl = 0; sl < p1 - p0; ++sl)
         if (live[D.h_pos_problem[p0 + sl]]) slots.push_back(sl);
@@ -2731,6 +2733,32 @@ int pmg_batch_create(int device, const pmg_params *params, int n_inst, const pmg
         PMG_HIP_CHECK(hipMemcpy(reo.data(), D.cell_reorder, sizeof(int) * D.n_cells, hipMemcpyDeviceToHost));
         D.h_pos_problem.resize(D.n_cells);
         for (int q = 0; q < D.n_cells; ++q) D.h_pos_problem[q] = M.L[l].cell_problem[reo[q]];
+      }
+      {
+        // runs of one problem: the merged numbering keeps each problem's dofs together
+        // (active, Dirichlet and pressure ranges), and a color's work items are sorted by
+        // row, so there are a few runs per problem and a rebuild copies runs
+        auto runs = [](long n, std::vector<DevLevel::Run> &out, auto &&prob) {
+          for (long i = 0; i < n;) {
+            const int b = prob(i);
+            long j = i + 1;
+            while (j < n && prob(j) == b) ++j;
+            out.push_back({i, j, b});
+            i = j;
+          }
+        };
+        runs(D.n_udof, D.h_urun, [&](long i) { return D.h_udof_problem[i]; });
+        runs(D.n_pdof, D.h_prun, [&](long i) { return D.h_pdof_problem[i]; });
+        D.h_irun_off.assign(1, 0);
+        for (int col = 0; l > 0 && col < D.n_colors; ++col) {
+          const long q0 = long(D.ptr_colors[col]) * (NU_LOC + NP_LOC);
+          const long q1 = long(D.ptr_colors[col + 1]) * (NU_LOC + NP_LOC);
+          const int p0 = D.ptr_colors[col];
+          std::vector<DevLevel::Run> cr;
+          runs(q1 - q0, cr, [&](long k) { return D.h_pos_problem[p0 + (D.h_res_items[q0 + k] >> 5)]; });
+          for (auto &r : cr) D.h_irun.push_back({q0 + r.b, q0 + r.e, r.prob});
+          D.h_irun_off.push_back(long(D.h_irun.size()));
+        }
       }
       D.n_l20_seg = int(M.L[l].l20.size() / 2);
       if (D.n_l20_seg) {

@@ -213,6 +213,14 @@ struct DevLevel {
   // dofs of the unfinished problems in ascending order: velocity dofs act_u, pressure dofs
   // act_p, both as type-4 operator rows act_oprows (velocity dof, n_udof + pressure dof)
   std::vector<int> h_udof_problem, h_pdof_problem;
+  // runs of consecutive entries of one problem (first, end, problem): velocity dofs,
+  // pressure dofs and, per color (h_irun_off, n_colors + 1), the residual work items
+  struct Run {
+    long b, e;
+    int prob;
+  };
+  std::vector<Run> h_urun, h_prun, h_irun;
+  std::vector<long> h_irun_off;
   bool act_lists = false;
   int *act_u = nullptr, *act_p = nullptr, *act_oprows = nullptr;
   int n_act_u = 0, n_act_p = 0;
