@@ -28,6 +28,7 @@
 // result register.
 #include "pmg_internal.hpp"
 #include "pmg_rows.hpp"
+#include "coarse_rows.hpp"
 
 namespace pmg {
 
@@ -468,34 +469,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, double al
 __global__ __launch_bounds__(256) void k_cb_phase1(const double *__restrict__ Ainv, int lda,
                                                    const double *__restrict__ P1, const double *__restrict__ rr,
                                                    double *__restrict__ y, double *__restrict__ p, int m, int q) {
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row < m) {
-    const double *ar = Ainv + long(row) * lda;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int c = lane; c < m; c += 64) {
-      const double v = ar[c];
-      s0 = fma(v, rr[c], s0);
-      s1 = fma(v, rr[m + c], s1);
-      s2 = fma(v, rr[2 * m + c], s2);
-    }
-    s0 = red64(s0);
-    s1 = red64(s1);
-    s2 = red64(s2);
-    if (lane == 0) {
-      y[row] = s0;
-      y[m + row] = s1;
-      y[2 * m + row] = s2;
-    }
-  } else if (row < m + q) {
-    const int i = row - m;
-    const long nr = 3L * m + q;
-    const double *pr = P1 + long(i) * nr;
-    double s = 0.0;
-    for (long k = lane; k < nr; k += 64) s = fma(pr[k], rr[k], s);
-    s = red64(s);
-    if (lane == 0) p[i] = s;
-  }
+  cb_phase1_row(Ainv, lda, P1, rr, y, p, m, q, blockIdx.x * 4 + (threadIdx.x >> 6), threadIdx.x & 63);
 }
 
 // phase 2: x_b = y_b - W_b p into the level vector u (active velocity dofs, one wavefront per
@@ -505,39 +479,47 @@ __global__ __launch_bounds__(256) void k_cb_phase2(OpView<double> op, const doub
                                                    const double *__restrict__ y, const double *__restrict__ p,
                                                    const double *__restrict__ f, double *__restrict__ u, int m,
                                                    int q, int pin, int vel_blocks) {
-  const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
+  auto put = [&](long j, double v) { u[j] = v; };
   if (int(blockIdx.x) < vel_blocks) {
-    const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= m) return;
-    const double *wr = W + long(r) * ldw;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = lane; i < q; i += 64) {
-      const double pi = p[i];
-      s0 = fma(wr[i], pi, s0);
-      s1 = fma(wr[q + i], pi, s1);
-      s2 = fma(wr[2 * q + i], pi, s2);
-    }
-    s0 = red64(s0);
-    s1 = red64(s1);
-    s2 = red64(s2);
-    if (lane == 0) {
-      u[r] = y[r] - s0;
-      u[long(N) + r] = y[m + r] - s1;
-      u[2L * N + r] = y[2 * m + r] - s2;
-    }
+    if (r < m) cb_phase2_vel(op, W, ldw, y, p, put, m, q, r, threadIdx.x & 63);
     return;
   }
-  const long idx = long(blockIdx.x - vel_blocks) * 256 + threadIdx.x;
-  const long nd = 3L * (N - NA);
-  if (idx < nd) {
-    const int b = int(idx / (N - NA)), c = NA + int(idx - long(b) * (N - NA));
-    const int k = op.a_rowptr[c];   // Dirichlet rows hold only their diagonal
-    u[long(b) * N + c] = f[long(b) * N + c] / op.a[(4 * b) * op.a_nnz + k];
-  } else if (idx < nd + NP) {
-    const int i = int(idx - nd);
-    u[3L * N + i] = (pin && i == 0) ? 0.0 : p[i - pin];
+  cb_phase2_rest(op, p, [&](long j) { return f[j]; }, put, pin, long(blockIdx.x - vel_blocks) * 256 + threadIdx.x);
+}
+
+// The same three steps for the block-form coarse systems of several problems in one launch
+// each (blockIdx.y = problem): the right-hand side read from, and the solution written to,
+// the merged level's vectors through each problem's coarse dof map.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cbb_rhs(const CbJob *__restrict__ jobs, const T *__restrict__ f) {
+  const CbJob &J = jobs[blockIdx.y];
+  const int lane = threadIdx.x & 15;
+  const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;
+  if (i >= 3 * J.m + J.q) return;
+  const int *__restrict__ idx = J.cidx;
+  const double v = coarse_rhs_row(J.op, [&](long j) { return double(f[idx[j]]); }, i, J.pin, lane);
+  if (lane == 0) J.rr[i] = v;
+}
+__global__ __launch_bounds__(256) void k_cbb_phase1(const CbJob *__restrict__ jobs) {
+  const CbJob &J = jobs[blockIdx.y];
+  cb_phase1_row(J.X, J.m + 3 * J.q, J.P1, J.rr, J.y, J.p, J.m, J.q, blockIdx.x * 4 + (threadIdx.x >> 6),
+                threadIdx.x & 63);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_cbb_phase2(const CbJob *__restrict__ jobs, const T *__restrict__ f,
+                                                    T *__restrict__ u) {
+  const CbJob &J = jobs[blockIdx.y];
+  const int *__restrict__ idx = J.cidx;
+  auto put = [&](long j, double v) { u[idx[j]] = T(v); };
+  const int vel_blocks = (J.m + 3) / 4;
+  if (int(blockIdx.x) < vel_blocks) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r < J.m) cb_phase2_vel(J.op, J.X + J.m, J.m + 3 * J.q, J.y, J.p, put, J.m, J.q, r, threadIdx.x & 63);
+    return;
   }
+  cb_phase2_rest(J.op, J.p, [&](long j) { return double(f[idx[j]]); }, put, J.pin,
+                 long(blockIdx.x - vel_blocks) * 256 + threadIdx.x);
 }
 
 }  // namespace
@@ -598,6 +580,26 @@ void coarse_block_solve(CoarseBlock &B, const OpView<double> &op, const double *
   k_cb_phase2<<<vel_blocks + int((rest + 255) / 256), 256, 0, s>>>(op, B.X + m, m + 3 * q, B.y, B.p, f, u, m, q,
                                                                     B.pin, vel_blocks);
   PMG_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_coarse_block_batch(const CbJob *jobs, int n_jobs, const CbGrid &g, const T *f, T *u, hipStream_t s) {
+  if (n_jobs <= 0) return;
+  k_cbb_rhs<T><<<dim3(g.rhs, n_jobs), 256, 0, s>>>(jobs, f);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_cbb_phase1<<<dim3(g.phase1, n_jobs), 256, 0, s>>>(jobs);
+  PMG_HIP_CHECK(hipGetLastError());
+  k_cbb_phase2<T><<<dim3(g.phase2, n_jobs), 256, 0, s>>>(jobs, f, u);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_coarse_block_batch<float>(const CbJob *, int, const CbGrid &, const float *, float *,
+                                               hipStream_t);
+template void launch_coarse_block_batch<double>(const CbJob *, int, const CbGrid &, const double *, double *,
+                                                hipStream_t);
+
+CbGrid coarse_block_grid(const CbJob &J) {
+  const long rest = 3L * (J.op.n_udof - J.op.n_active) + J.op.n_pdof;
+  return CbGrid{(3 * J.m + J.q + 15) / 16, (J.m + J.q + 3) / 4, (J.m + 3) / 4 + int((rest + 255) / 256)};
 }
 
 }  // namespace pmg

@@ -14,6 +14,7 @@
 // order); each visit runs blocked column sweeps in the dgetrs order.
 #include "pmg_internal.hpp"
 #include "pmg_rows.hpp"
+#include "coarse_rows.hpp"
 
 namespace pmg {
 
@@ -350,39 +351,14 @@ __global__ __launch_bounds__(256) void k_transpose(const double *__restrict__ a,
 // ------------------------------------------------------------------ per-visit solve
 // reduced right-hand side, already permuted: b[i] = (f_I - K_ID x_D)[sigma[i]] with
 // x_D = f_D / diag(A_cc) on the Dirichlet velocity dofs
-__device__ inline double dirichlet_value(const OpView<double> &op, const double *f, int b, int c) {
-  const int k = op.a_rowptr[c];   // Dirichlet rows hold only their diagonal
-  return f[long(b) * op.n_udof + c] / op.a[(4 * b) * op.a_nnz + k];
-}
 __global__ __launch_bounds__(256) void k_coarse_rhs(OpView<double> op, const double *__restrict__ f,
                                                     const int *__restrict__ sigma, double *__restrict__ bout, int n,
                                                     int pin) {
   const int lane = threadIdx.x & 15;
   const int i = (blockIdx.x * 256 + threadIdx.x) >> 4;  // 16 lanes per reduced row
   if (i >= n) return;
-  const int k = sigma[i];
-  const int N = op.n_udof, NA = op.n_active;
-  double v = 0.0;
-  double f0;
-  if (k < 3 * NA) {
-    const int br = k / NA, r = k - br * NA;
-    f0 = f[long(br) * N + r];
-    for (int q = op.a_rowptr[r] + lane; q < op.a_rowptr[r + 1]; q += 16) {
-      const int c = op.a_kcol[q];
-      if (c < NA) continue;
-      for (int bc = 0; bc < 3; ++bc) v += op.a[(3 * br + bc) * op.a_nnz + q] * dirichlet_value(op, f, bc, c);
-    }
-  } else {
-    const int pi = k - 3 * NA + pin;
-    f0 = f[3L * N + pi];
-    for (int q = op.b_rowptr[pi] + lane; q < op.b_rowptr[pi + 1]; q += 16) {
-      const int c = op.b_kcol[q];
-      if (c < NA) continue;
-      for (int b = 0; b < 3; ++b) v += op.b[b * op.b_nnz + q] * dirichlet_value(op, f, b, c);
-    }
-  }
-  v = red16(v);
-  if (lane == 0) bout[i] = f0 - v;
+  const double v = coarse_rhs_row(op, [&](long j) { return f[j]; }, sigma[i], pin, lane);
+  if (lane == 0) bout[i] = v;
 }
 // scatter the reduced solution back: kept dofs, Dirichlet dofs f_D/diag, pinned pressure 0
 __global__ void k_coarse_out(OpView<double> op, const double *__restrict__ f, const double *__restrict__ xr,
@@ -391,7 +367,7 @@ __global__ void k_coarse_out(OpView<double> op, const double *__restrict__ f, co
   const int N = op.n_udof, NA = op.n_active, NP = op.n_pdof;
   if (i < 3L * N) {
     const int b = int(i / N), r = int(i - long(b) * N);
-    u[i] = r < NA ? xr[cvel(b, r, NA)] : dirichlet_value(op, f, b, r);
+    u[i] = r < NA ? xr[cvel(b, r, NA)] : coarse_dirichlet(op, [&](long j) { return f[j]; }, b, r);
   } else if (i < 3L * N + NP) {
     const int q = int(i - 3L * N);
     u[i] = (pin && q == 0) ? 0.0 : xr[cpre(q, NA, pin)];

@@ -33,6 +33,7 @@ struct BatchProblem {
   int *cidx = nullptr, *fidx = nullptr;    // problem dof -> merged index (coarsest, finest level)
   double *cf = nullptr, *cu = nullptr;     // FP64 coarse right-hand side and solution
   double *io = nullptr;                    // staging of one problem vector (host memory calls)
+  bool in_job = false;                     // its coarse visits run in the level's batched launch
 };
 
 struct pmg_ctx {
@@ -777,12 +778,64 @@ void coarse_solve_t(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
   }
 }
 
-// the direct solves (SolveExact) of the problems whose coarsest level is this merged level
+// The job tables of the batched coarse visits: on every merged level, the problems whose
+// coarsest level it is, that are still iterating and whose coarse system has the block
+// form.  Rebuilt after every set-up and whenever the unfinished problems are recounted (never
+// inside a graph capture).
+void build_cb_jobs(pmg_ctx *c) {
+  for (auto &b : c->bi) b.in_job = false;
+  for (auto &D : c->L) {
+    D.n_cb_jobs = 0;
+    D.cb_job_prob.clear();
+    if (D.coarse_insts.empty()) continue;
+    std::vector<CbJob> jobs;
+    CbGrid g{0, 0, 0};
+    for (int k : D.coarse_insts) {
+      BatchProblem &b = c->bi[k];
+      if (!b.co.block || (!c->b_live.empty() && !c->b_live[k])) continue;
+      const CoarseBlock &B = b.co.cb;
+      CbJob J{b.cl.op64(), B.X, B.P1, B.rr, B.y, B.p, b.cidx, B.m, B.q, B.pin};
+      const CbGrid gj = coarse_block_grid(J);
+      g.rhs = std::max(g.rhs, gj.rhs);
+      g.phase1 = std::max(g.phase1, gj.phase1);
+      g.phase2 = std::max(g.phase2, gj.phase2);
+      jobs.push_back(J);
+      D.cb_job_prob.push_back(k);
+      b.in_job = true;
+    }
+    if (jobs.empty()) continue;
+    if (!D.cb_jobs) D.cb_jobs = dalloc<CbJob>(c, D.coarse_insts.size());
+    PMG_HIP_CHECK(hipMemcpy(D.cb_jobs, jobs.data(), sizeof(CbJob) * jobs.size(), hipMemcpyHostToDevice));
+    D.n_cb_jobs = int(jobs.size());
+    D.cb_grid[0] = g.rhs;
+    D.cb_grid[1] = g.phase1;
+    D.cb_grid[2] = g.phase2;
+  }
+}
+
+// the direct solves (SolveExact) of the problems whose coarsest level is this merged level:
+// the block-form ones in one batched visit (three launches), the others one by one
 template <typename T>
 void batch_coarse(pmg_ctx *c, const DevLevel &D, T *u, const T *f) {
+  if (D.n_cb_jobs > 0) {
+    double by = 0.0, fl = 0.0;
+    for (int k : D.cb_job_prob) {
+      BatchProblem &b = c->bi[k];
+      if (b.co.pending) {
+        PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, b.co.ev_done, 0));
+        b.co.pending = false;
+      }
+      by += coarse_visit_bytes(b.co);
+      fl += coarse_visit_flops(b.co);
+    }
+    Timed tm(c, &c->tm.coarse_solve, by, fl);
+    launch_coarse_block_batch<T>(D.cb_jobs, D.n_cb_jobs, CbGrid{D.cb_grid[0], D.cb_grid[1], D.cb_grid[2]}, f, u,
+                                 c->stream);
+  }
   for (int k : D.coarse_insts) {
     if (!c->b_live.empty() && !c->b_live[k]) continue;   // finished: left out, its result is unused
     BatchProblem &b = c->bi[k];
+    if (b.in_job) continue;
     if (b.co.pending) {
       PMG_HIP_CHECK(hipStreamWaitEvent(c->stream, b.co.ev_done, 0));
       b.co.pending = false;
@@ -1114,6 +1167,7 @@ void setup_factors(pmg_ctx *c) {
   }
   c->tm.patch_lu.ms += t_patch;
   for (Coarse *C : todo) C->ready = true;   // usable once ev_done completes; checked in check_deferred
+  if (c->batch) build_cb_jobs(c);
   c->setup_done = true;
 }
 
@@ -1338,6 +1392,7 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
     c->seg_ch_act = c->seg_ch;
     c->n_seg_ch_act = c->n_seg_ch;
     c->seg_cptr_act = c->seg_cptr;
+    build_cb_jobs(c);
     drop_graph(c);
     return;
   }
@@ -1425,6 +1480,7 @@ void batch_compact(pmg_ctx *c, const std::vector<char> &live) {
   }
   c->b_live = live;
   c->b_live_cells = cells;
+  build_cb_jobs(c);
   drop_graph(c);
 }
 

@@ -117,6 +117,8 @@ struct Csr {
   float *val32 = nullptr;
 };
 
+struct CbJob;
+
 struct DevLevel {
   int n_udof = 0, n_active = 0, hanging_bound = 0, n_dirichlet = 0, n_pdof = 0, n_dof = 0;
   int n_cells = 0;
@@ -221,6 +223,12 @@ struct DevLevel {
   };
   std::vector<Run> h_urun, h_prun, h_irun;
   std::vector<long> h_irun_off;
+  // the block-form coarse visits of the problems whose coarsest level this is, batched
+  // (device job table, the problems it covers, the largest grids)
+  CbJob *cb_jobs = nullptr;
+  int n_cb_jobs = 0;
+  std::vector<int> cb_job_prob;
+  int cb_grid[3] = {0, 0, 0};
   bool act_lists = false;
   int *act_u = nullptr, *act_p = nullptr, *act_oprows = nullptr;
   int n_act_u = 0, n_act_p = 0;
@@ -397,6 +405,24 @@ void launch_cb_build(const DevLevel &L, CoarseBlock &B, hipStream_t s);
 void coarse_block_factor(CoarseBlock &B, hipStream_t s);
 void coarse_block_solve(CoarseBlock &B, const OpView<double> &op, const double *rr, const double *f, double *u,
                         hipStream_t s);
+// one problem's block-form coarse visit inside a batched launch (independent problems merged
+// into one level): its own coarse operator and stored data, its work vectors, and the map
+// of its coarse dofs into the merged level's vectors
+struct CbJob {
+  OpView<double> op;
+  const double *X, *P1;
+  double *rr, *y, *p;
+  const int *cidx;
+  int m, q, pin;
+};
+struct CbGrid {
+  int rhs, phase1, phase2;   // workgroups per problem of the three launches
+};
+CbGrid coarse_block_grid(const CbJob &J);
+// the visits of n_jobs problems, three launches in all; f and u are the merged level's
+// right-hand side and solution; g holds the largest grids of the jobs
+template <typename T>
+void launch_coarse_block_batch(const CbJob *jobs, int n_jobs, const CbGrid &g, const T *f, T *u, hipStream_t s);
 void coarse_lu_factor(Coarse &C, hipStream_t s);   // asynchronous; C.info gets the dgetrf info
 void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, double *u, hipStream_t s);
 
