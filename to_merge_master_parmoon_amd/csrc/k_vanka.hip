@@ -625,10 +625,7 @@ __global__ void k_res_desc(const int *__restrict__ a_rowptr, const int *__restri
   __builtin_memcpy(desc + 4 * i, &d, 16);
 }
 
-// UA: A-row entries per lane per round of loads in the interleaved-copy form (4, or 8 on
-// small colors, where one round covers a whole vertex row and the chain of dependent loads
-// is shorter; the lanes add their entries in the same order either way)
-template <typename T, bool SH, int UA>
+template <typename T, bool SH>
 __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, int pos0,
                                                    const uint8_t *__restrict__ invperm,
                                                    const int *__restrict__ items, const int *__restrict__ desc,
@@ -693,8 +690,11 @@ __global__ __launch_bounds__(256) void k_vanka_res(OpView<T> op, long n_items, i
       T s1 = 0, s2 = 0, s3 = 0;
       const int *__restrict__ kc = op.a_kcol + k0;
       if constexpr (SH && sizeof(T) == 4) {
+        // eight A-row entries per lane per round of loads: a whole vertex row (125 entries)
+        // in one round, so one dependent round trip less than four per round (measured:
+        // 8^3 5.13 -> 4.86 us, 16^3 6.99 -> 6.61 us, 32^3 32.3 -> 31.3 us per color)
         if (ua)
-          row_a1_aos<UA>(kc, op.a1 + k0, n, lane, ua, s1, s2, s3);
+          row_a1_aos<8>(kc, op.a1 + k0, n, lane, ua, s1, s2, s3);
         else
           row_a1<T>(kc, op.a1 + k0, n, lane, u1, u2, u3, s1, s2, s3);
       } else if constexpr (SH)
@@ -1406,13 +1406,10 @@ void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const ui
   if (per_block <= 0) per_block = n_items < 65536 ? 16 : 32;
   int grid = chunk_grid(n_items, per_block, 8192);
   if (xg) grid = (grid + 7) / 8 * 8;
-  if (op.a1 && sizeof(T) == 4 && ua && n_items < 65536)
-    k_vanka_res<T, true, 8><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc, uloc);
-  else if (op.a1)
-    k_vanka_res<T, true, 4><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc, uloc);
+  if (op.a1)
+    k_vanka_res<T, true><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc, uloc);
   else
-    k_vanka_res<T, false, 4><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc,
-                                                  uloc);
+    k_vanka_res<T, false><<<grid, 256, 0, s>>>(op, n_items, pos0, invperm, items, desc, xg, u, ua, f, rloc, uloc);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_vanka_residual<double>(const OpView<double> &, long, int, const uint8_t *, const int *,
