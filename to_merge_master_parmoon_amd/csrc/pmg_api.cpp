@@ -2368,12 +2368,21 @@ static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
         launch_sweep_start(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, ua, c->stream);
       }
     }
-    const int p0 = D.ptr_colors[0];
-    const long n_items = long(D.ptr_colors[1] - p0) * (NU_LOC + NP_LOC);
-    const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
-    const int *desc = D.res_desc + 4 * size_t(p0) * (NU_LOC + NP_LOC);
     const bool grouped = D.h_xcd_grouped[0];
     T *uloc = sizeof(T) == 4 ? reinterpret_cast<T *>(D.uloc32) : nullptr;
+    // one pass = the residual launches of every color in sweep order (each color reads its
+    // own rows, as in a sweep; one color repeated would find its rows in the caches)
+    const int passes = std::max(1, reps / std::max(1, D.n_colors));
+    auto pass = [&](int xcd, int pb) {
+      for (int col = 0; col < D.n_colors; ++col) {
+        const int p0 = D.ptr_colors[col];
+        const long n_items = long(D.ptr_colors[col + 1] - p0) * (NU_LOC + NP_LOC);
+        const int *items = D.res_items + size_t(p0) * (NU_LOC + NP_LOC);
+        const int *desc = D.res_desc + 4 * size_t(p0) * (NU_LOC + NP_LOC);
+        const int *xg = xcd && D.h_xcd_grouped[col] ? D.xcd_goff + 9 * col : nullptr;
+        launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, uloc, c->stream, pb);
+      }
+    };
     // three rounds over the candidates, each candidate's fastest round counts (the rounds
     // interleave the candidates, so a slow phase of the device does not favour one)
     std::vector<std::pair<int, int>> cand;
@@ -2381,13 +2390,8 @@ static void tune_level(pmg_ctx *c, DevLevel &D, int reps, bool finest) {
       for (int pb : {16, 32, 64}) cand.emplace_back(xcd, pb);
     std::vector<float> best(cand.size(), 1e30f);
     for (int round = 0; round < 3; ++round)
-      for (size_t k = 0; k < cand.size(); ++k) {
-        const int *xg = grouped && cand[k].first ? D.xcd_goff : nullptr;
-        const int pb = cand[k].second;
-        best[k] = std::min(best[k], time_launches(c, reps, [&] {
-          launch_vanka_residual<T>(op, n_items, p0, D.invperm, items, desc, xg, u, ua, f, rloc, uloc, c->stream, pb);
-        }));
-      }
+      for (size_t k = 0; k < cand.size(); ++k)
+        best[k] = std::min(best[k], time_launches(c, passes, [&] { pass(cand[k].first, cand[k].second); }));
     const size_t kb = size_t(std::min_element(best.begin(), best.end()) - best.begin());
     D.tune_res_xcd[pi] = cand[kb].first;
     D.tune_res_pb[pi] = cand[kb].second;
