@@ -2063,13 +2063,18 @@ static int add_level(pmg_ctx *c, const pmg_level_desc *d, int *level_out, bool m
           D.res_bytes[sh][pi].assign(nc, 0.0);
           for (int k = 0; k < nc; ++k) {
             double b = 0;
+            // with stored inverses (FP32) the launch also writes u at the patch dofs
+            const double uo = (pi == 1 && c->prm.patch_inverse) ? double(vs) : 0.0;
             for (int q = ptr[k]; q < ptr[k + 1]; ++q) {
               const int cell = reo[q];
-              b += 4 + (NU_LOC + NP_LOC) * 4.0;  // cell index, work items
-              // row, f, local residual, dof index, inverse-permutation byte
-              for (int j = 0; j < NU_LOC; ++j)
-                b += rb.vel(d, ug[size_t(cell) * NU_LOC + j], vs, sh) + 3 * vs + 3 * vs + 4 + 3;
-              for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], vs) + vs + vs + 4 + 1;
+              b += (NU_LOC + NP_LOC) * 20.0;   // work items and their 16-byte descriptors
+              // rows (their extents come from the descriptors, not the row pointers), f,
+              // local residual (and u) written, inverse-permutation bytes
+              for (int j = 0; j < NU_LOC; ++j) {
+                const int g = ug[size_t(cell) * NU_LOC + j];
+                b += rb.vel(d, g, vs, sh) - (g < d->n_active ? 16 : 8) + 3 * vs + 3 * vs + 3 * uo + 3;
+              }
+              for (int j = 0; j < NP_LOC; ++j) b += rb.pre(d, pg[size_t(cell) * NP_LOC + j], vs) - 8 + vs + vs + uo + 1;
             }
             D.res_bytes[sh][pi][k] = b;
           }
