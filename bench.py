@@ -564,7 +564,7 @@ def main():
     cfg = {"workload": f"oseen3d_{args.n}^3_q2p1disc_fgmres30_vcycle22", "levels": prob.n_levels,
            "dofs": sz["n_dof"], "precision": args.precision, "nu": 1e-2,
            "shared_block": SHARED_BLOCK, "patch_inverse": PATCH_INVERSE,
-           "local_solves": ("stored block form (A^-1, A^-1 BT_c, B_c, S^-1) on 32^3, full inverses on 16^3/8^3"
+           "local_solves": ("stored block form (A^-1, A^-1 BT_c, [-S^-1 B_c A^-1 | S^-1]) on 32^3, full inverses on 16^3/8^3"
                             if prec and PATCH_INVERSE and SHARED_BLOCK else
                             "stored full inverses" if prec and PATCH_INVERSE else "LU factors and triangular sweeps"),
            "coarse_solve": ("FP64 block form (A^-1, S^-1 and products), two GEMV phases per visit"

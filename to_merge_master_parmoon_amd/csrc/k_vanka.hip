@@ -411,9 +411,10 @@ __global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const in
 // gives exactly these entries: zero coupling blocks, Dirichlet rows with the diagonal only
 // and no BT).  A is inverted by FP32 Gauss-Jordan with partial pivoting (the column swaps
 // undone at the end, so A^{-1} itself), then W_c = A^{-1} BT_c, the Schur complement
-// S = -sum_c B_c W_c and S^{-1} (4 x 4, pivoted); A^{-1}, W, B and S^{-1} are stored in the
-// factor precision (SCH_* layout).  The local residual is then used in its natural order
-// (invperm = identity).
+// S = -sum_c B_c W_c, S^{-1} (4 x 4, pivoted) and the pressure rows of the patch inverse
+// P1 = [-S^{-1} B_c A^{-1} | S^{-1}]; A^{-1}, W and P1 are stored in the factor precision
+// (SCH_* layout), so a local solve is p = P1 r and x_c = A^{-1} r_c - W_c p.  The local
+// residual is used in its natural order (invperm = identity).
 template <typename F>
 __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int *__restrict__ cells,
                                                      const int *__restrict__ pmap, int ncells, F *__restrict__ sch,
@@ -579,15 +580,35 @@ __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int
         }
   }
   __syncthreads();
+  // V_c = B_c A^{-1} (4 x 27 each), kept in BT's storage (no longer needed) as BT[c][j][k]
+  for (int e = t; e < 3 * NP_LOC * NU_LOC; e += 128) {
+    const int c = e / (NP_LOC * NU_LOC), r = e - c * NP_LOC * NU_LOC, k = r / NU_LOC, j = r - NU_LOC * k;
+    float acc = 0.0f;
+    for (int m = 0; m < NU_LOC; ++m) acc = fmaf(Bm[c][k][m], A[m][j], acc);
+    BT[c][j][k] = acc;
+  }
+  __syncthreads();
   F *out = sch + long(pos) * SCH_STRIDE;
   for (int e = t; e < NU_LOC * NU_LOC; e += 128) out[SCH_AINV + e] = StoreF<F>::get(A[e / NU_LOC][e % NU_LOC]);
   for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
     const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC;
     out[SCH_W + e] = StoreF<F>::get(W[c][r / NP_LOC][r % NP_LOC]);
-    out[SCH_B + e] = StoreF<F>::get(Bm[c][r / NU_LOC][r % NU_LOC]);
   }
-  if (t < NP_LOC * NP_LOC) out[SCH_SINV + t] = StoreF<F>::get(S[t / NP_LOC][t % NP_LOC]);
-  for (int e = SCH_SINV + NP_LOC * NP_LOC + t; e < SCH_STRIDE; e += 128) out[e] = StoreF<F>::get(0.0f);
+  // P1 row k: -S^{-1} V (entries c * 27 + j), then S^{-1} (entries 81 + l)
+  for (int e = t; e < NP_LOC * NLOC; e += 128) {
+    const int k = e / NLOC, q = e - NLOC * k;
+    float v;
+    if (q < 3 * NU_LOC) {
+      const int c = q / NU_LOC, j = q - NU_LOC * c;
+      float acc = 0.0f;
+      for (int l = 0; l < NP_LOC; ++l) acc = fmaf(S[k][l], BT[c][j][l], acc);
+      v = -acc;
+    } else {
+      v = S[k][q - 3 * NU_LOC];
+    }
+    out[SCH_P1 + e] = StoreF<F>::get(v);
+  }
+  for (int e = SCH_END + t; e < SCH_STRIDE; e += 128) out[e] = StoreF<F>::get(0.0f);
   if (t < NLOC) invperm[long(pos) * PERM_STRIDE + t] = uint8_t(t);
   if (t == 0) info[pos] = info_s;
 }
@@ -1135,7 +1156,7 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
   constexpr int PER = 16 / int(sizeof(F));
   constexpr int NVEC = SCH_STRIDE / PER;
   __shared__ float M[SCH_STRIDE];
-  __shared__ float r[RLOC_STRIDE], y[3][NU_LOC], g[NP_LOC], pp[NP_LOC];
+  __shared__ float r[RLOC_STRIDE], pp[NP_LOC];
   __shared__ __align__(16) float Kf[REF ? KREF_BLK : 4];   // A, BT_c, B_c (FP32), the stored layout
   __shared__ float xs[REF ? NLOC : 1], rr[REF ? NLOC : 1];
   const int lane = threadIdx.x;
@@ -1192,46 +1213,35 @@ __global__ __launch_bounds__(64) void k_vanka_apply_schur(const F *__restrict__ 
   }
   __syncthreads();
   // x = M^{-1} rhs with the stored data; lane j < 27 returns x_c[j] (c = 0..2) in x[0..2],
-  // lanes 27..30 return p[lane - 27] in x[0]
+  // lanes 27..30 return p[lane - 27] in x[0].  Two phases: p = P1 rhs (16 lanes per pressure
+  // row) beside y_c = A^{-1} rhs_c (lane j, the three components with one read of row j of
+  // A^{-1}), then x_c = y_c - W_c p.
   auto solve = [&](const float *rhs, float (&x)[3]) {
+    {
+      const int k = lane >> 4, part = lane & 15;
+      float acc = 0.0f;
+      for (int q = part; q < NLOC; q += 16) acc = fmaf(M[SCH_P1 + k * NLOC + q], rhs[q], acc);
+      acc += __shfl_xor(acc, 8, 16);
+      acc += __shfl_xor(acc, 4, 16);
+      acc += __shfl_xor(acc, 2, 16);
+      acc += __shfl_xor(acc, 1, 16);
+      if (part == 0) pp[k] = acc;
+    }
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
     if (lane < NU_LOC) {
-      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
       for (int m = 0; m < NU_LOC; ++m) {
         const float w = M[SCH_AINV + lane * NU_LOC + m];
         a0 = fmaf(w, rhs[m], a0);
         a1 = fmaf(w, rhs[NU_LOC + m], a1);
         a2 = fmaf(w, rhs[2 * NU_LOC + m], a2);
       }
-      y[0][lane] = a0;
-      y[1][lane] = a1;
-      y[2][lane] = a2;
-    }
-    __syncthreads();
-    {
-      // g = r_p - sum_c B_c y_c: the 16 lanes of group k share pressure row k's 81 terms
-      const int k = lane >> 4, part = lane & 15;
-      float acc = 0.0f;
-      for (int q = part; q < 3 * NU_LOC; q += 16) {
-        const int c = q / NU_LOC, j = q - NU_LOC * c;
-        acc = fmaf(M[SCH_B + (c * NP_LOC + k) * NU_LOC + j], y[c][j], acc);
-      }
-      acc += __shfl_xor(acc, 8, 16);
-      acc += __shfl_xor(acc, 4, 16);
-      acc += __shfl_xor(acc, 2, 16);
-      acc += __shfl_xor(acc, 1, 16);
-      if (part == 0) g[k] = rhs[3 * NU_LOC + k] - acc;
-    }
-    __syncthreads();
-    if (lane < NP_LOC) {
-      float acc = 0.0f;
-      for (int l = 0; l < NP_LOC; ++l) acc = fmaf(M[SCH_SINV + lane * NP_LOC + l], g[l], acc);
-      pp[lane] = acc;
     }
     __syncthreads();
     x[0] = x[1] = x[2] = 0.0f;
     if (lane < NU_LOC) {
+      const float y[3] = {a0, a1, a2};
       for (int c = 0; c < 3; ++c) {
-        float acc = y[c][lane];
+        float acc = y[c];
         for (int k = 0; k < NP_LOC; ++k) acc = fmaf(-M[SCH_W + (c * NU_LOC + lane) * NP_LOC + k], pp[k], acc);
         x[c] = acc;
       }

@@ -31,9 +31,11 @@ static_assert(NLOC * MINV_LD <= MINV_STRIDE, "inverse exceeds the patch stride")
 // is [A 0 0 BT1; 0 A 0 BT2; 0 0 A BT3; B1 B2 B3 0] with one 27 x 27 block A, so its solve
 // needs A^{-1} (729), W_c = A^{-1} BT_c (3 x 27 x 4), B_c (3 x 4 x 27) and the inverse of the
 // 4 x 4 Schur complement S = -sum_c B_c W_c (16): 1393 entries per patch
-constexpr int SCH_AINV = 0, SCH_W = 729, SCH_B = SCH_W + 324, SCH_SINV = SCH_B + 324;
+// block-form patch data: A^{-1} (27 x 27), W_c = A^{-1} BT_c (c = 1..3, 27 x 4 each) and
+// P1 = [-S^{-1} B_1 A^{-1}, -S^{-1} B_2 A^{-1}, -S^{-1} B_3 A^{-1} | S^{-1}] (4 x 85)
+constexpr int SCH_AINV = 0, SCH_W = 729, SCH_P1 = SCH_W + 324, SCH_END = SCH_P1 + 4 * 85;
 constexpr int SCH_STRIDE = 1408;    // 1393 padded to a multiple of 16 entries
-static_assert(SCH_SINV + 16 <= SCH_STRIDE && SCH_STRIDE <= MINV_STRIDE, "block-form patch stride");
+static_assert(SCH_END <= SCH_STRIDE && SCH_STRIDE <= MINV_STRIDE, "block-form patch stride");
 constexpr int RLOC_STRIDE = 96;     // local residual slot (85 used)
 constexpr int POS_DOFS_STRIDE = 32; // dofs of the patch at a color position (27 + 4 used)
 constexpr int PERM_STRIDE = 96;     // uint8 inverse row permutation per patch
