@@ -488,11 +488,18 @@ __global__ __launch_bounds__(256) void k_trsv_chain(const double *__restrict__ f
 template <typename T, int NC>
 __global__ __launch_bounds__(256) void k_csr(int n_rows, const int *__restrict__ rows, const int *__restrict__ rp,
                                              const int *__restrict__ col, const double *__restrict__ val,
-                                             const T *__restrict__ x, long xs, T *__restrict__ y, long ys) {
+                                             const T *__restrict__ x, long xs, T *__restrict__ y, long ys,
+                                             int zero_from) {
   const int lane = threadIdx.x & 15;
   const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4, ngrp = gridDim.x * 16;
   for (int i = grp; i < n_rows; i += ngrp) {
     const int r = rows ? rows[i] : i;
+    if (r >= zero_from) {
+      if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c * ys + r] = T(0);
+      continue;
+    }
     const int k0 = rp[r], k1 = rp[r + 1];
     T s[NC];
 #pragma unroll
@@ -603,19 +610,19 @@ void coarse_lu_solve(Coarse &C, const OpView<double> &op, const double *f, doubl
 
 template <typename T>
 void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long xs, T *y, long ys, hipStream_t s, const int *rows,
-                     int n_rows) {
+                     int n_rows, int zero_from) {
   const int n = rows ? n_rows : m.n_rows;
   if (n <= 0) return;
   const int grid = std::min(8192, (n + 15) / 16);
   if (ncomp == 3)
-    k_csr<T, 3><<<grid, 256, 0, s>>>(n, rows, m.rowptr, m.col, m.val, x, xs, y, ys);
+    k_csr<T, 3><<<grid, 256, 0, s>>>(n, rows, m.rowptr, m.col, m.val, x, xs, y, ys, zero_from);
   else
-    k_csr<T, 1><<<grid, 256, 0, s>>>(n, rows, m.rowptr, m.col, m.val, x, xs, y, ys);
+    k_csr<T, 1><<<grid, 256, 0, s>>>(n, rows, m.rowptr, m.col, m.val, x, xs, y, ys, zero_from);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_csr_spmv<double>(const Csr &, int, const double *, long, double *, long, hipStream_t,
-                                      const int *, int);
+                                      const int *, int, int);
 template void launch_csr_spmv<float>(const Csr &, int, const float *, long, float *, long, hipStream_t, const int *,
-                                     int);
+                                     int, int);
 
 }  // namespace pmg

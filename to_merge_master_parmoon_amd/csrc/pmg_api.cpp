@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -610,7 +611,10 @@ template <typename T>
 // (nothing touched them in between): with the interleaved velocity copy and no global
 // relaxation, that copy is still current and the Dirichlet values are still set, so the
 // sweep start (and the previous sweep's copy back into u) are left out
-void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, bool last = true, bool first = true) {
+// close = false: the closing SetDirichletNodes of the last sweep is left out (the caller
+// overwrites the same slice next)
+void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, bool last = true, bool first = true,
+                 bool close = true) {
   const bool prof = c->profiling;
   auto t0 = std::chrono::steady_clock::now();
   if constexpr (sizeof(T) == 8) {
@@ -707,7 +711,7 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
     if (ua && D.schur && (last || relax)) launch_ua_to_u(ua, u, D.n_udof, c->stream);
   }
   if (relax) launch_relax<T>(u, aux, T(alpha), D.n_dof, c->stream);
-  if (last) set_dirichlet_t<T>(c, D, u, f);
+  if (last && close) set_dirichlet_t<T>(c, D, u, f);
   if (prof) {
     c->tm.sweep_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->tm.sweep_calls += 1;
@@ -723,11 +727,15 @@ void mg_defect(pmg_ctx *c, const DevLevel &D, const T *u, const T *f, T *d, doub
 }
 
 template <typename T>
-void restrict_defect(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *d, T *fc) {
+// zero_dirichlet: the Dirichlet rows of the coarse velocity get 0 instead of their sum (the
+// CorrectDefect that follows a restriction, in the same launch)
+void restrict_defect(pmg_ctx *c, const DevLevel &F, const DevLevel &C, const T *d, T *fc,
+                     bool zero_dirichlet = false) {
   Timed tm(c, &c->tm.transfer, F.transfer_bytes, 0.0);
   // (batch with finished problems) the coarse rows of the unfinished problems only
   const bool sub = C.act_lists;
-  launch_csr_spmv<T>(F.ru, 3, d, F.n_udof, fc, C.n_udof, c->stream, sub ? C.act_u : nullptr, C.n_act_u);
+  launch_csr_spmv<T>(F.ru, 3, d, F.n_udof, fc, C.n_udof, c->stream, sub ? C.act_u : nullptr, C.n_act_u,
+                     zero_dirichlet ? C.n_active : INT_MAX);
   launch_csr_spmv<T>(F.rp, 1, d + 3L * F.n_udof, 0, fc + 3L * C.n_udof, 0, c->stream, sub ? C.act_p : nullptr,
                      C.n_act_p);
 }
@@ -872,8 +880,11 @@ void cycle(pmg_ctx *c, int i) {
   DevLevel &D = c->L[i];
   T *u = Vec<T>::sol(D), *f = Vec<T>::rhs(D), *d = Vec<T>::aux0(D), *aux = Vec<T>::aux1(D);
   // CorrectNodes then SetDirichletNodes (NSE_MultiGrid.C:501-502): both act on the same
-  // slice of u and the second overwrites what the first writes, so only the second runs
-  set_dirichlet_t<T>(c, D, u, f);
+  // slice of u and the second overwrites what the first writes, so only the second runs;
+  // it is left out where the next write of that slice comes before any read: a single
+  // problem's coarse solve writes all of u, the first pre-smoothing sweep starts with it
+  const int pre = c->prm.sc_pre_smooth_saddle;
+  if ((i == 0 && c->batch) || (i > 0 && pre == 0)) set_dirichlet_t<T>(c, D, u, f);
   if (i == 0) {
     if (c->batch)
       batch_coarse<T>(c, D, u, f);
@@ -886,13 +897,18 @@ void cycle(pmg_ctx *c, int i) {
   // them unchanged (their prolongated correction rows are empty)
   if (!D.coarse_insts.empty()) batch_coarse<T>(c, D, u, f);
   DevLevel &C = c->L[i - 1];
-  for (int j = 0; j < c->prm.sc_pre_smooth_saddle; ++j)
-    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == c->prm.sc_pre_smooth_saddle, j == 0);
+  // the last pre-smoothing sweep's closing SetDirichletNodes is overwritten by the
+  // CorrectNodes right after it
+  for (int j = 0; j < pre; ++j)
+    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == pre, j == 0, j + 1 < pre);
   correct_nodes_t<T>(c, D, u);
   mg_defect<T>(c, D, u, f, d, nullptr);
-  restrict_defect<T>(c, D, C, d, Vec<T>::rhs(C));
-  correct_defect_t<T>(c, C, Vec<T>::rhs(C));
-  launch_fill<T>(Vec<T>::sol(C), T(0), C.n_dof, c->stream);
+  // CorrectDefect of the coarse right-hand side: its Dirichlet rows zeroed by the
+  // restriction itself, then the pressure projection
+  restrict_defect<T>(c, D, C, d, Vec<T>::rhs(C), true);
+  project_pressure<T>(c, C, Vec<T>::rhs(C));
+  // the zero start of the coarser level (a single problem's coarse solve writes all of u)
+  if (i - 1 > 0 || c->batch) launch_fill<T>(Vec<T>::sol(C), T(0), C.n_dof, c->stream);
   cycle<T>(c, i - 1);
   prolongate<T>(c, D, C, Vec<T>::sol(C), d);
   // CorrectNodes of the correction and of u (NSE_MultiGrid.C:1548, 1580) write only the

@@ -389,10 +389,10 @@ template <typename T>
 void launch_l20(T *p, int n_pdof, RedScratch &red, hipStream_t s);
 
 // transfers: y[c*ny + r] = sum_k val[k] x[c*nx + col[k]], c < ncomp, for every row r, or
-// (rows non-null) for the n_rows rows listed
+// (rows non-null) for the n_rows rows listed; rows r >= zero_from get 0
 template <typename T>
 void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long x_stride, T *y, long y_stride,
-                     hipStream_t s, const int *rows = nullptr, int n_rows = 0);
+                     hipStream_t s, const int *rows = nullptr, int n_rows = 0, int zero_from = 0x7fffffff);
 
 // dense coarse LU / solve
 void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s);
