@@ -518,6 +518,49 @@ __global__ __launch_bounds__(256) void k_csr(int n_rows, const int *__restrict__
   }
 }
 
+// The prolongation and the coarse-grid correction in one launch, for the fine rows r (all,
+// or the listed ones): velocity (NC = 3) u_c[r] = f_c[r] on the Dirichlet rows [ds, ds + dc)
+// and u_c[r] + a (P x)_c[r] elsewhere, and (ua non-null) ua[r] = (u_1, u_2, u_3, 0);
+// pressure (NC = 1) u[r] += a (P x)[r].  The same values as the prolongation into a vector,
+// the axpy and the sweep start that set the Dirichlet slice and build ua.
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_csr_update(int n_rows, const int *__restrict__ rows,
+                                                    const int *__restrict__ rp, const int *__restrict__ col,
+                                                    const double *__restrict__ val, const T *__restrict__ x, long xs,
+                                                    T *__restrict__ u, long us, T a, const T *__restrict__ f, int ds,
+                                                    int dc, float4 *__restrict__ ua) {
+  const int lane = threadIdx.x & 15;
+  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4, ngrp = gridDim.x * 16;
+  for (int i = grp; i < n_rows; i += ngrp) {
+    const int r = rows ? rows[i] : i;
+    const int k0 = rp[r], k1 = rp[r + 1];
+    T s[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] = 0;
+    for (int k = k0 + lane; k < k1; k += 16) {
+      const int j = col[k];
+      const T v = T(val[k]);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) s[c] += v * x[c * xs + j];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s[c] = red16(s[c]);
+    if (lane == 0) {
+      const bool dir = r >= ds && r < ds + dc;
+      T w[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        T y = u[c * us + r];
+        y += a * s[c];
+        w[c] = dir ? f[c * us + r] : y;
+        u[c * us + r] = w[c];
+      }
+      if constexpr (NC == 3)
+        if (ua) ua[r] = make_float4(float(w[0]), float(w[1]), float(w[2]), 0.0f);
+    }
+  }
+}
+
 }  // namespace
 
 void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s) {
@@ -624,5 +667,28 @@ template void launch_csr_spmv<double>(const Csr &, int, const double *, long, do
                                       const int *, int, int);
 template void launch_csr_spmv<float>(const Csr &, int, const float *, long, float *, long, hipStream_t, const int *,
                                      int, int);
+
+
+template <typename T>
+void launch_prolong_update(const Csr &pu, const Csr &pp, const T *xc, long nxc, T *u, long nu, T a, const T *f,
+                           int ds, int dc, float *ua, hipStream_t s, const int *urows, int n_urows, const int *prows,
+                           int n_prows) {
+  const int nv = urows ? n_urows : pu.n_rows, np = prows ? n_prows : pp.n_rows;
+  if (nv > 0)
+    k_csr_update<T, 3><<<std::min(8192, (nv + 15) / 16), 256, 0, s>>>(nv, urows, pu.rowptr, pu.col, pu.val, xc, nxc,
+                                                                       u, nu, a, f, ds, dc,
+                                                                       reinterpret_cast<float4 *>(ua));
+  if (np > 0)
+    k_csr_update<T, 1><<<std::min(8192, (np + 15) / 16), 256, 0, s>>>(np, prows, pp.rowptr, pp.col, pp.val,
+                                                                       xc + 3 * nxc, 0, u + 3 * nu, 0, a, nullptr,
+                                                                       0, 0, nullptr);
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_prolong_update<double>(const Csr &, const Csr &, const double *, long, double *, long, double,
+                                            const double *, int, int, float *, hipStream_t, const int *, int,
+                                            const int *, int);
+template void launch_prolong_update<float>(const Csr &, const Csr &, const float *, long, float *, long, float,
+                                           const float *, int, int, float *, hipStream_t, const int *, int,
+                                           const int *, int);
 
 }  // namespace pmg

@@ -604,6 +604,16 @@ void nodal_sweep(pmg_ctx *c, DevLevel &D, double *u, const double *f, double *au
   if (last) set_dirichlet(c, D, u, f);
 }
 
+// the interleaved velocity copy a level's FP32 sweeps gather from (stored inverses and the
+// shared velocity block), or null
+template <typename T>
+float *sweep_copy(const DevLevel &D) {
+  if constexpr (sizeof(T) == 4) {
+    if (D.ua32 && D.a_shared && (D.minv16 || D.minv32)) return D.ua32;
+  }
+  return nullptr;
+}
+
 template <typename T>
 // last = false: another sweep with the same f follows at once and sets the Dirichlet values
 // again before anything reads them, so this sweep's closing SetDirichletNodes is left out
@@ -612,9 +622,10 @@ template <typename T>
 // relaxation, that copy is still current and the Dirichlet values are still set, so the
 // sweep start (and the previous sweep's copy back into u) are left out
 // close = false: the closing SetDirichletNodes of the last sweep is left out (the caller
-// overwrites the same slice next)
+// overwrites the same slice next); started: the caller has done the sweep's start (the
+// Dirichlet slice of u set from f and, FP32 with inverses, the interleaved copy built)
 void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, bool last = true, bool first = true,
-                 bool close = true) {
+                 bool close = true, bool started = false) {
   const bool prof = c->profiling;
   auto t0 = std::chrono::steady_clock::now();
   if constexpr (sizeof(T) == 8) {
@@ -634,15 +645,13 @@ void vanka_sweep(pmg_ctx *c, DevLevel &D, T *u, const T *f, T *aux, T *rloc, boo
   const int fb = sizeof(T) == 8 ? 8 : (c->prm.precision == PMG_PREC_FP32_BF16 ? 2 : 4);
   // FP32 with inverses and the shared block: the residual gathers the velocity from the
   // interleaved copy ua, built here and kept current by the local updates
-  float *ua = nullptr;
-  if constexpr (sizeof(T) == 4) {
-    if (D.ua32 && D.a_shared && (D.minv16 || D.minv32)) ua = D.ua32;
+  float *ua = sweep_copy<T>(D);
+  if (!started) {
     if (ua && !(first == false && !relax))
-      launch_sweep_start(u, f, D.n_udof, D.hanging_bound, D.n_dirichlet, ua, c->stream);
+      launch_sweep_start(reinterpret_cast<float *>(u), reinterpret_cast<const float *>(f), D.n_udof, D.hanging_bound,
+                         D.n_dirichlet, ua, c->stream);
     else if (!ua)
       set_dirichlet_t<T>(c, D, u, f);
-  } else {
-    set_dirichlet_t<T>(c, D, u, f);
   }
   if (relax) launch_copy<T>(aux, u, D.n_dof, c->stream);
   const OpView<T> op = opview<T>(D);
@@ -910,16 +919,30 @@ void cycle(pmg_ctx *c, int i) {
   // the zero start of the coarser level (a single problem's coarse solve writes all of u)
   if (i - 1 > 0 || c->batch) launch_fill<T>(Vec<T>::sol(C), T(0), C.n_dof, c->stream);
   cycle<T>(c, i - 1);
-  prolongate<T>(c, D, C, Vec<T>::sol(C), d);
   // CorrectNodes of the correction and of u (NSE_MultiGrid.C:1548, 1580) write only the
   // Dirichlet slice of u, which the first post-smoothing sweep sets from f before reading
   // it: with post-smoothing they are left out (same results)
-  const bool post = c->prm.sc_post_smooth_saddle > 0;
+  const int post = c->prm.sc_post_smooth_saddle;
+  const T damp = T(c->prm.sc_gmg_damp_factor_saddle);
+  if (post > 0 && !D.nodal) {
+    // the prolongation, the update of u and the first post-smoothing sweep's start in one
+    // pass over the fine rows
+    {
+      Timed tm(c, &c->tm.transfer, D.transfer_bytes, 0.0);
+      const bool sub = D.act_lists;
+      launch_prolong_update<T>(D.pu, D.pp, Vec<T>::sol(C), C.n_udof, u, D.n_udof, damp, f, D.hanging_bound,
+                               D.n_dirichlet, sweep_copy<T>(D), c->stream, sub ? D.act_u : nullptr, D.n_act_u,
+                               sub ? D.act_p : nullptr, D.n_act_p);
+    }
+    for (int j = 0; j < post; ++j)
+      vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == post, j == 0, true, j == 0);
+    return;
+  }
+  prolongate<T>(c, D, C, Vec<T>::sol(C), d);
   if (!post) correct_nodes_t<T>(c, D, d);
-  launch_axpy<T>(u, T(c->prm.sc_gmg_damp_factor_saddle), d, D.n_dof, c->stream);
+  launch_axpy<T>(u, damp, d, D.n_dof, c->stream);
   if (!post) correct_nodes_t<T>(c, D, u);
-  for (int j = 0; j < c->prm.sc_post_smooth_saddle; ++j)
-    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == c->prm.sc_post_smooth_saddle, j == 0);
+  for (int j = 0; j < post; ++j) vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == post, j == 0);
 }
 
 // the main stream waits for coarse factorizations still running on the second stream

@@ -394,6 +394,15 @@ template <typename T>
 void launch_csr_spmv(const Csr &m, int ncomp, const T *x, long x_stride, T *y, long y_stride,
                      hipStream_t s, const int *rows = nullptr, int n_rows = 0, int zero_from = 0x7fffffff);
 
+// the prolongation of the coarse correction xc with the update of the fine u in one pass:
+// u += a P xc (velocity rows of pu, pressure rows of pp), the velocity Dirichlet rows
+// [ds, ds + dc) set from f, and (ua non-null) the interleaved velocity copy rebuilt; urows /
+// prows (may be null) restrict it to the listed rows
+template <typename T>
+void launch_prolong_update(const Csr &pu, const Csr &pp, const T *xc, long nxc, T *u, long nu, T a, const T *f,
+                           int ds, int dc, float *ua, hipStream_t s, const int *urows = nullptr, int n_urows = 0,
+                           const int *prows = nullptr, int n_prows = 0);
+
 // dense coarse LU / solve
 void launch_coarse_build(const DevLevel &L, Coarse &C, hipStream_t s);
 // reduced right-hand side b[i] = (f_I - K_ID x_D)[sigma[i]] of the coarse system (n unknowns)
