@@ -1134,9 +1134,10 @@ __global__ __launch_bounds__(64) void k_vanka_apply(const F *__restrict__ minv, 
 }
 
 // Local solve in the block form, one wavefront per patch: the patch's 1408 entries are
-// staged in LDS (non-temporal 16-byte loads), then y_c = A^{-1} r_c (lanes 0..26, three
-// components), g = r_p - sum_c B_c y_c, p = S^{-1} g (lanes 0..3), x_c = y_c - W_c p, and
-// u += damp * (x, p) (with the interleaved velocity copy when ua is given).
+// staged in LDS (non-temporal 16-byte loads), then p = P1 r (P1 = [-S^{-1} B_c A^{-1} |
+// S^{-1}], 16 lanes per pressure row) beside y_c = A^{-1} r_c (lanes 0..26, three
+// components), then x_c = y_c - W_c p, and u += damp * (x, p) (with the interleaved velocity
+// copy when ua is given).
 // REF: nref refinement steps after the first solve: the local residual r - K x with the
 // FP32 patch matrix K (A, BT_c and B_c as the set-up gathered them, stored beside the
 // inverse data), solved with the stored data and added to x.
