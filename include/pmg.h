@@ -99,8 +99,9 @@ typedef struct {
                                                   Vanka solve is a product with data stored once per
                                                   operator in the factor precision: with the shared
                                                   velocity block and colors of more than 2048
-                                                  patches A^{-1} of the 27 x 27 block, A^{-1} BT_c,
-                                                  B_c and the inverse of the 4 x 4 Schur complement,
+                                                  patches A^{-1} of the 27 x 27 block, A^{-1} BT_c
+                                                  and the pressure rows [-S^{-1} B_c A^{-1} | S^{-1}]
+                                                  of the patch inverse (S the 4 x 4 Schur complement),
                                                   otherwise the explicit inverse of the patch matrix
                                                   (FP32 Gauss-Jordan with the pivots of dgetf2); 0:
                                                   the stored LU factors and triangular sweeps.  FP64
