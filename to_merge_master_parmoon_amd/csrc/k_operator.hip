@@ -214,6 +214,23 @@ __global__ void k_sweep_start(float *__restrict__ u, const float *__restrict__ f
   ua[i] = make_float4(v1, v2, v3, 0.0f);
 }
 
+// A zero start: u = 0 with the velocity Dirichlet slices from f, and (ua non-null) the
+// interleaved copy of that velocity
+template <typename T>
+__global__ void k_start_zero(T *__restrict__ u, const T *__restrict__ f, int N, int NP, int start, int count,
+                             float4 *__restrict__ ua) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) {
+    const bool dir = i >= start && i < start + count;
+    const T v1 = dir ? f[i] : T(0), v2 = dir ? f[N + i] : T(0), v3 = dir ? f[2 * N + i] : T(0);
+    u[i] = v1;
+    u[N + i] = v2;
+    u[2 * N + i] = v3;
+    if (ua) ua[i] = make_float4(float(v1), float(v2), float(v3), 0.0f);
+  } else if (i < N + NP) {
+    u[2 * N + i] = T(0);
+  }
+}
 __global__ void k_ua_to_u(const float4 *__restrict__ ua, float *__restrict__ u, int N) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -479,6 +496,15 @@ void launch_ua_to_u(const float *ua, float *u, int N, hipStream_t s) {
   k_ua_to_u<<<(N + 255) / 256, 256, 0, s>>>(reinterpret_cast<const float4 *>(ua), u, N);
   PMG_HIP_CHECK(hipGetLastError());
 }
+
+template <typename T>
+void launch_start_zero(T *u, const T *f, int N, int NP, int start, int count, float *ua, hipStream_t s) {
+  if (N + NP <= 0) return;
+  k_start_zero<T><<<(N + NP + 255) / 256, 256, 0, s>>>(u, f, N, NP, start, count, reinterpret_cast<float4 *>(ua));
+  PMG_HIP_CHECK(hipGetLastError());
+}
+template void launch_start_zero<float>(float *, const float *, int, int, int, int, float *, hipStream_t);
+template void launch_start_zero<double>(double *, const double *, int, int, int, int, float *, hipStream_t);
 
 void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s) {
   if (N <= 0) return;

@@ -885,9 +885,24 @@ struct Vec<float> {
 // residual printout (:503, :1550, :1753 and the coarse-level one) are skipped: they do not
 // change the iterate.
 template <typename T>
-void cycle(pmg_ctx *c, int i) {
+// zero: the level starts from u = 0 (its right-hand side is new)
+void cycle(pmg_ctx *c, int i, bool zero) {
   DevLevel &D = c->L[i];
   T *u = Vec<T>::sol(D), *f = Vec<T>::rhs(D), *d = Vec<T>::aux0(D), *aux = Vec<T>::aux1(D);
+  // the zero start: with pre-smoothing, one pass that also does the first sweep's start (the
+  // Dirichlet slice from f, the interleaved copy); a single problem's coarse solve writes
+  // all of u, so it needs none.  (Not on merged levels where problems are solved directly:
+  // their solution is written into u after the start, and the sweeps' interleaved copy must
+  // be built from it.)
+  bool started = false;
+  if (zero) {
+    if (i > 0 && c->prm.sc_pre_smooth_saddle > 0 && !D.nodal && D.coarse_insts.empty()) {
+      launch_start_zero<T>(u, f, D.n_udof, D.n_pdof, D.hanging_bound, D.n_dirichlet, sweep_copy<T>(D), c->stream);
+      started = true;
+    } else if (i > 0 || c->batch) {
+      launch_fill<T>(u, T(0), D.n_dof, c->stream);
+    }
+  }
   // CorrectNodes then SetDirichletNodes (NSE_MultiGrid.C:501-502): both act on the same
   // slice of u and the second overwrites what the first writes, so only the second runs;
   // it is left out where the next write of that slice comes before any read: a single
@@ -909,16 +924,14 @@ void cycle(pmg_ctx *c, int i) {
   // the last pre-smoothing sweep's closing SetDirichletNodes is overwritten by the
   // CorrectNodes right after it
   for (int j = 0; j < pre; ++j)
-    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == pre, j == 0, j + 1 < pre);
+    vanka_sweep<T>(c, D, u, f, aux, Vec<T>::rloc(D), j + 1 == pre, j == 0, j + 1 < pre, j == 0 && started);
   correct_nodes_t<T>(c, D, u);
   mg_defect<T>(c, D, u, f, d, nullptr);
   // CorrectDefect of the coarse right-hand side: its Dirichlet rows zeroed by the
   // restriction itself, then the pressure projection
   restrict_defect<T>(c, D, C, d, Vec<T>::rhs(C), true);
   project_pressure<T>(c, C, Vec<T>::rhs(C));
-  // the zero start of the coarser level (a single problem's coarse solve writes all of u)
-  if (i - 1 > 0 || c->batch) launch_fill<T>(Vec<T>::sol(C), T(0), C.n_dof, c->stream);
-  cycle<T>(c, i - 1);
+  cycle<T>(c, i - 1, true);
   // CorrectNodes of the correction and of u (NSE_MultiGrid.C:1548, 1580) write only the
   // Dirichlet slice of u, which the first post-smoothing sweep sets from f before reading
   // it: with post-smoothing they are left out (same results)
@@ -968,15 +981,10 @@ void drop_graph(pmg_ctx *c) {
 void vcycle_body(pmg_ctx *c) {
   DevLevel &F = c->L.back();
   const int top = int(c->L.size()) - 1;
-  if (!mixed(c)) {
-    launch_fill<double>(F.sol, 0.0, F.n_dof, c->stream);
-    set_dirichlet(c, F, F.sol, F.rhs);
-    cycle<double>(c, top);
-  } else {
-    launch_fill<float>(F.sol32, 0.0f, F.n_dof, c->stream);
-    set_dirichlet_t<float>(c, F, F.sol32, F.rhs32);
-    cycle<float>(c, top);
-  }
+  if (!mixed(c))
+    cycle<double>(c, top, true);
+  else
+    cycle<float>(c, top, true);
 }
 
 // Capture vcycle_body into a graph.  Every launch of the cycle is a kernel (fills and

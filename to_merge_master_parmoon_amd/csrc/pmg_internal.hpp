@@ -355,6 +355,11 @@ template <typename T>
 void launch_vanka_residual(const OpView<T> &op, long n_items, int pos0, const uint8_t *invperm,
                            const int *items, const int *desc, const int *xg, const T *u, const float *ua,
                            const T *f, T *rloc, T *uloc, hipStream_t s, int per_block = 0);
+// zero start of a level: u = 0 except the velocity Dirichlet slices [start, start + count)
+// from f, and (ua non-null) the interleaved copy of that velocity; N velocity, NP pressure
+// dofs
+template <typename T>
+void launch_start_zero(T *u, const T *f, int N, int NP, int start, int count, float *ua, hipStream_t s);
 // FP32 sweep start: Dirichlet slices of u from f, ua[4 i + c] = u_c(i) (c < 3)
 void launch_sweep_start(float *u, const float *f, int N, int start, int count, float *ua, hipStream_t s);
 // descriptors of residual work items (4 ints each) from their items and global rows
