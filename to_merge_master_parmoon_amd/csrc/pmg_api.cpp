@@ -968,6 +968,12 @@ void resolve_pending(pmg_ctx *c) {
   wait(c->coarse);
   for (auto &b : c->bi) wait(b.co);
 }
+bool coarse_pending(const pmg_ctx *c) {
+  if (c->coarse.pending) return true;
+  for (const auto &b : c->bi)
+    if (b.co.pending) return true;
+  return false;
+}
 
 void drop_graph(pmg_ctx *c) {
   if (c->vexec) (void)hipGraphExecDestroy(c->vexec);
@@ -1012,12 +1018,16 @@ void vcycle_dev(pmg_ctx *c, const double *v, double *z) {
     launch_copy<double>(F.rhs, v, F.n_dof, c->stream);
   else
     launch_d2f(F.rhs32, v, F.n_dof, c->stream);
-  if (c->prm.use_graph && !c->profiling) {
-    resolve_pending(c);
+  if (c->prm.use_graph && !c->profiling && !coarse_pending(c)) {
     if (!c->vexec) capture_vcycle(c);
     PMG_HIP_CHECK(hipGraphLaunch(c->vexec, c->stream));
   } else {
+    // eager while a coarse factorization still runs on the second stream (the first cycle
+    // after a set-up): the main stream then waits for it at the coarse visit, and the
+    // smoothing and restrictions of the finer levels before that visit overlap it (a graph
+    // launch would have to wait for it before its first kernel)
     vcycle_body(c);
+    resolve_pending(c);
   }
   if (!mixed(c))
     launch_copy<double>(z, F.sol, F.n_dof, c->stream);
