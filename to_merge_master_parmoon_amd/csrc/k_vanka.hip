@@ -433,14 +433,57 @@ __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int
   const int t = threadIdx.x;
   const int *__restrict__ mp = pmap + long(cells[pos]) * PM_STRIDE;
   if (t == 0) info_s = 0;
-  for (int e = t; e < NU_LOC * NU_LOC; e += 128) {
-    const int j = e / NU_LOC, m = e - NU_LOC * j;
-    A[j][m] = patch_entry<float>(op, mp, j, m);
-  }
-  for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
-    const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC, j = r / NP_LOC, k = r - NP_LOC * j;
-    BT[c][j][k] = patch_entry<float>(op, mp, NU_LOC * c + j, 3 * NU_LOC + k);
-    Bm[c][k][j] = patch_entry<float>(op, mp, 3 * NU_LOC + k, NU_LOC * c + j);
+  {
+    // the entries patch_entry gives for this form (the shared block A: value av[off] or 0;
+    // BT_c and B_c: block c of the row, off + c |len|), with every load of a thread issued
+    // before any entry is stored: two memory round trips in all (the map, then the values)
+    // instead of two per entry
+    constexpr int EA = NU_LOC * NU_LOC, IA = (EA + 127) / 128;
+    constexpr int EB = 3 * NU_LOC * NP_LOC, IB = (EB + 127) / 128;
+    int oa[IA], ob[IB], lb[IB], oc[IB], lc[IB];
+#pragma unroll
+    for (int i = 0; i < IA; ++i) {
+      const int e = t + 128 * i;
+      oa[i] = e < EA ? mp[PM_A + e] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      const int e = t + 128 * i;
+      const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC, j = r / NP_LOC, k = r - NP_LOC * j;
+      const bool in = e < EB;
+      ob[i] = in ? mp[PM_BT + j * NP_LOC + k] : -1;
+      lb[i] = in ? mp[PM_MT + j] : 0;
+      oc[i] = in ? mp[PM_B + k * NU_LOC + j] : -1;
+      lc[i] = in ? mp[PM_NB + k] : 0;
+    }
+    float va[IA], vb[IB], vc[IB];
+#pragma unroll
+    for (int i = 0; i < IA; ++i) {
+      const float v = op.av[oa[i] >= 0 ? oa[i] : 0];
+      va[i] = oa[i] >= 0 ? v : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      const int c = (t + 128 * i) / (NU_LOC * NP_LOC);
+      const float x = op.btv[ob[i] >= 0 ? ob[i] + c * abs(lb[i]) : 0];
+      const float y = op.bv[oc[i] >= 0 ? oc[i] + c * abs(lc[i]) : 0];
+      vb[i] = ob[i] >= 0 ? x : 0.0f;
+      vc[i] = oc[i] >= 0 ? y : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < IA; ++i) {
+      const int e = t + 128 * i, j = e / NU_LOC, m = e - NU_LOC * j;
+      if (e < EA) A[j][m] = va[i];
+    }
+#pragma unroll
+    for (int i = 0; i < IB; ++i) {
+      const int e = t + 128 * i;
+      const int c = e / (NU_LOC * NP_LOC), r = e - c * NU_LOC * NP_LOC, j = r / NP_LOC, k = r - NP_LOC * j;
+      if (e < EB) {
+        BT[c][j][k] = vb[i];
+        Bm[c][k][j] = vc[i];
+      }
+    }
   }
   __syncthreads();
   if (kref) {
