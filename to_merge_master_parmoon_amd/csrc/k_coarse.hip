@@ -212,23 +212,33 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
       if (lane < nk && src != row) sig[row] = sg;
     }
     __syncthreads();
-    // interchanges on every column outside the panel: the <= 32 rows involved, staged
-    // through LDS (the U block row buffer) 256 columns at a time, read before written
+    // interchanges on every column outside the panel: the <= 32 rows involved, staged in
+    // registers (a thread's loads all in flight together), every row read before any is
+    // written; only the rows whose content changes are read and written
     {
       const int nk = nk_s;
-      double *stage = &Us[0][0];   // 32 x 256
-      for (int cb = 0; cb < n; cb += 256) {
-        for (int idx = t; idx < nk * 256; idx += LU_T) {
-          const int k = idx >> 8, c = cb + (idx & 255);
-          stage[idx] = c < n ? a[long(src_s[k]) * ld + c] : 0.0;
+      // two halves of <= 256 columns (n <= LU_T); thread t moves column (t % 256) of the
+      // half and every second row from row t / 256
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int c = 256 * h + (t & 255), k0 = t >> 8;
+        const bool col = c < n && (c < j0 || c >= j0 + w);
+        double xv[LU_NB];
+#pragma unroll
+        for (int q = 0; q < LU_NB; ++q) {
+          const int k = k0 + 2 * q;
+          const bool mv = col && k < nk && src_s[k] != rows_s[k];
+          xv[q] = mv ? a[long(src_s[k]) * ld + c] : 0.0;
         }
         __syncthreads();
-        for (int idx = t; idx < nk * 256; idx += LU_T) {
-          const int k = idx >> 8, c = cb + (idx & 255);
-          if (c < n && (c < j0 || c >= j0 + w) && src_s[k] != rows_s[k]) a[long(rows_s[k]) * ld + c] = stage[idx];
+#pragma unroll
+        for (int q = 0; q < LU_NB; ++q) {
+          const int k = k0 + 2 * q;
+          const bool mv = col && k < nk && src_s[k] != rows_s[k];
+          if (mv) a[long(rows_s[k]) * ld + c] = xv[q];
         }
-        __syncthreads();
       }
+      __syncthreads();
     }
     const int j1 = j0 + w, nt = n - j1;
     if (nt <= 0) break;
