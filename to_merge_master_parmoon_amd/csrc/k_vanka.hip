@@ -415,7 +415,14 @@ __global__ __launch_bounds__(256, 6) void k_patch_inv(OpView<float> op, const in
 // P1 = [-S^{-1} B_c A^{-1} | S^{-1}]; A^{-1}, W and P1 are stored in the factor precision
 // (SCH_* layout), so a local solve is p = P1 r and x_c = A^{-1} r_c - W_c p.  The local
 // residual is used in its natural order (invperm = identity).
-template <typename F>
+//
+// FULL: the same steps, then the full 85 x 85 inverse expanded from them in the storage of
+// k_patch_inv (minv_off, invperm = identity): with T_c = W_c S^{-1},
+//   M(c, c') = delta(c, c') A^{-1} + T_c V_c',  M(c, p) = -T_c,  M(p, c') = -S^{-1} V_c',
+//   M(p, p) = S^{-1},
+// so the levels whose local solves use the full inverse get it without an 85 x 85
+// Gauss-Jordan (the shared velocity block only, no refinement-step matrices).
+template <typename F, bool FULL = false>
 __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int *__restrict__ cells,
                                                      const int *__restrict__ pmap, int ncells, F *__restrict__ sch,
                                                      uint8_t *__restrict__ invperm, int *__restrict__ info,
@@ -486,7 +493,7 @@ __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int
     }
   }
   __syncthreads();
-  if (kref) {
+  if (!FULL && kref) {
     // the FP32 block form for the refinement steps: A row-major, BT_c[j][k], B_c[k][j]
     float *kr = kref + long(pos) * KREF_BLK;
     for (int e = t; e < NU_LOC * NU_LOC; e += 128) kr[e] = A[e / NU_LOC][e % NU_LOC];
@@ -631,6 +638,71 @@ __global__ __launch_bounds__(128) void k_patch_schur(OpView<float> op, const int
     BT[c][j][k] = acc;
   }
   __syncthreads();
+  if constexpr (FULL) {
+    // T_c = W_c S^{-1} into W's storage (each thread reads its row of W before the barrier)
+    float tv[2][NP_LOC];
+    for (int h = 0; h < 2; ++h) {
+      const int e = t + 128 * h;
+      if (e < 3 * NU_LOC) {
+        const int c = e / NU_LOC, i = e - NU_LOC * c;
+        for (int l = 0; l < NP_LOC; ++l) {
+          float acc = 0.0f;
+          for (int m = 0; m < NP_LOC; ++m) acc = fmaf(W[c][i][m], S[m][l], acc);
+          tv[h][l] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    for (int h = 0; h < 2; ++h) {
+      const int e = t + 128 * h;
+      if (e < 3 * NU_LOC)
+        for (int l = 0; l < NP_LOC; ++l) W[e / NU_LOC][e % NU_LOC][l] = tv[h][l];
+    }
+    __syncthreads();
+    constexpr int PER = 16 / int(sizeof(F));
+    F *out = sch + long(pos) * MINV_STRIDE;
+    for (int o = t; o < NLOC * MINV_LD; o += 128) {
+      // (row, column) of storage entry o (the inverse of minv_off)
+      int r, cc;
+      if (o < 64 * MINV_LD) {
+        const int v = o / PER;
+        r = v % 64;
+        cc = (v / 64) * PER + o % PER;
+      } else {
+        const int o2 = o - 64 * MINV_LD, v = o2 / PER;
+        r = 64 + v % (NLOC - 64);
+        cc = (v / (NLOC - 64)) * PER + o2 % PER;
+      }
+      float v = 0.0f;
+      if (cc < NLOC) {
+        if (r < 3 * NU_LOC) {
+          const int ci = r / NU_LOC, i = r - NU_LOC * ci;
+          if (cc < 3 * NU_LOC) {
+            const int cj = cc / NU_LOC, j = cc - NU_LOC * cj;
+            float acc = ci == cj ? A[i][j] : 0.0f;
+            for (int l = 0; l < NP_LOC; ++l) acc = fmaf(W[ci][i][l], BT[cj][j][l], acc);
+            v = acc;
+          } else {
+            v = -W[ci][i][cc - 3 * NU_LOC];
+          }
+        } else {
+          const int k = r - 3 * NU_LOC;
+          if (cc < 3 * NU_LOC) {
+            const int cj = cc / NU_LOC, j = cc - NU_LOC * cj;
+            float acc = 0.0f;
+            for (int l = 0; l < NP_LOC; ++l) acc = fmaf(S[k][l], BT[cj][j][l], acc);
+            v = -acc;
+          } else {
+            v = S[k][cc - 3 * NU_LOC];
+          }
+        }
+      }
+      out[o] = StoreF<F>::get(v);
+    }
+    if (t < NLOC) invperm[long(pos) * PERM_STRIDE + t] = uint8_t(t);
+    if (t == 0) info[pos] = info_s;
+    return;
+  }
   F *out = sch + long(pos) * SCH_STRIDE;
   for (int e = t; e < NU_LOC * NU_LOC; e += 128) out[SCH_AINV + e] = StoreF<F>::get(A[e / NU_LOC][e % NU_LOC]);
   for (int e = t; e < 3 * NU_LOC * NP_LOC; e += 128) {
@@ -1382,6 +1454,17 @@ void launch_res_desc(const DevLevel &L, const int *items, const int *rows, long 
 
 void launch_patch_inv(const DevLevel &L, int *info, hipStream_t s, float *kref) {
   if (L.n_cells <= 0) return;
+  if (L.a_shared && !kref) {
+    // the full inverse expanded from the block form (no 85 x 85 elimination)
+    if (L.minv32)
+      k_patch_schur<float, true><<<L.n_cells, 128, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv32,
+                                                           L.invperm, info, nullptr);
+    else
+      k_patch_schur<uint16_t, true><<<L.n_cells, 128, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells,
+                                                              L.minv16, L.invperm, info, nullptr);
+    PMG_HIP_CHECK(hipGetLastError());
+    return;
+  }
   if (L.minv32)
     k_patch_inv<float><<<L.n_cells, 256, 0, s>>>(L.op32(), L.cell_reorder, L.pmap, L.n_cells, L.minv32, L.invperm,
                                                  info, kref);
