@@ -38,6 +38,39 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ inline d4 mfma16(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
+// (bv, bi) := the larger value, the lower index among equals (the pivot rule of dgetf2)
+__device__ inline void amax_take(double &bv, int &bi, double ov, int oi) {
+  const bool take = ov > bv || (ov == bv && oi < bi);
+  bv = take ? ov : bv;
+  bi = take ? oi : bi;
+}
+// one step of the reduction with the partner lane that the DPP pattern CTRL selects
+template <int CTRL>
+__device__ inline void amax_dpp(double &bv, int &bi) {
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(bv), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(bv), CTRL, 0xF, 0xF, false);
+  const int oi = __builtin_amdgcn_mov_dpp(bi, CTRL, 0xF, 0xF, false);
+  amax_take(bv, bi, __hiloint2double(hi, lo), oi);
+}
+// the wavefront's largest value and its lowest index in every lane (all 64 lanes active):
+// within each row of 16 lanes by DPP (quad xor 1, quad xor 2, half-row mirror, row
+// mirror), then the four rows' results by readlane, taken in lane order
+__device__ inline void wave_argmax(double &bv, int &bi) {
+  amax_dpp<0xB1>(bv, bi);
+  amax_dpp<0x4E>(bv, bi);
+  amax_dpp<0x141>(bv, bi);
+  amax_dpp<0x140>(bv, bi);
+  const int h = __double2hiint(bv), l = __double2loint(bv);
+  double v = __hiloint2double(__builtin_amdgcn_readlane(h, 0), __builtin_amdgcn_readlane(l, 0));
+  int i = __builtin_amdgcn_readlane(bi, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r)
+    amax_take(v, i, __hiloint2double(__builtin_amdgcn_readlane(h, 16 * r), __builtin_amdgcn_readlane(l, 16 * r)),
+              __builtin_amdgcn_readlane(bi, 16 * r));
+  bv = v;
+  bi = i;
+}
+
 constexpr int LU_NB = 16;          // panel width
 constexpr int LU_T = 512;          // threads of the single-workgroup LU (one panel row each)
 constexpr int TRSM_T = 256;        // threads of the substitution kernel
@@ -88,8 +121,8 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
                                                  int *__restrict__ info) {
   __shared__ double Lp[LU_T][LU_NB + 1];     // the panel
   __shared__ double Us[LU_NB][LU_T];         // the U block row of the panel (columns j0 + 16 ..)
-  __shared__ double wv[2][LU_T / 64];
-  __shared__ int wi[2][LU_T / 64];
+  __shared__ __align__(16) double wv[2][LU_T / 64];
+  __shared__ __align__(16) int wi[2][LU_T / 64];
   __shared__ double WB[2][LU_T / 64][LU_NB];   // every wavefront's candidate pivot row
   __shared__ double JB[2][LU_NB];              // row j of the current step
   __shared__ int pivs[LU_NB];
@@ -115,15 +148,7 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
       const int j = j0 + jj, par = jj & 1;
       double bv = (own && r >= j) ? fabs(rv[jj]) : -1.0;
       int bi = (own && r >= j) ? r : n;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o, 64);
-        const int oi = __shfl_xor(bi, o, 64);
-        if (ov > bv || (ov == bv && oi < bi)) {
-          bv = ov;
-          bi = oi;
-        }
-      }
+      wave_argmax(bv, bi);
       if (lane == 0) {
         wv[par][wave] = bv;
         wi[par][wave] = bi;
@@ -135,16 +160,19 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
 #pragma unroll
         for (int cc = 0; cc < LU_NB; ++cc) JB[par][cc] = rv[cc];
       __syncthreads();
-      bv = wv[par][0];
-      bi = wi[par][0];
+      {
+        // every wavefront's candidate loaded first, then taken in wavefront order
+        double cv[LU_T / 64];
+        int ci[LU_T / 64];
 #pragma unroll
-      for (int w2 = 1; w2 < LU_T / 64; ++w2) {
-        const double ov = wv[par][w2];
-        const int oi = wi[par][w2];
-        if (ov > bv || (ov == bv && oi < bi)) {
-          bv = ov;
-          bi = oi;
+        for (int w2 = 0; w2 < LU_T / 64; ++w2) {
+          cv[w2] = wv[par][w2];
+          ci[w2] = wi[par][w2];
         }
+        bv = cv[0];
+        bi = ci[0];
+#pragma unroll
+        for (int w2 = 1; w2 < LU_T / 64; ++w2) amax_take(bv, bi, cv[w2], ci[w2]);
       }
       const int p = bi;
       if (t == 0) {
