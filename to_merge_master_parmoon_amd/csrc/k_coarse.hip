@@ -144,7 +144,7 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
     for (int cc = 0; cc < LU_NB; ++cc) rv[cc] = (own && cc < w) ? a[long(r) * ld + j0 + cc] : 0.0;
 #pragma unroll
     for (int jj = 0; jj < LU_NB; ++jj) {
-      if (jj >= w) break;
+      if (jj >= w) continue;   // (not break: a constant trip count lets the loop unroll fully)
       const int j = j0 + jj, par = jj & 1;
       double bv = (own && r >= j) ? fabs(rv[jj]) : -1.0;
       int bi = (own && r >= j) ? r : n;
