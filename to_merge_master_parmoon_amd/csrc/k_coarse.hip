@@ -129,6 +129,9 @@ __global__ __launch_bounds__(LU_T) void k_lu_wg(double *__restrict__ a, int n, i
   __shared__ int sig[LU_T];
   __shared__ int rows_s[2 * LU_NB], src_s[2 * LU_NB];
   __shared__ int nk_s, info_s;
+  // the whole register file of its SIMDs (2 x 256 VGPRs): no wavefront of a concurrent
+  // kernel shares this CU's issue slots with the pivot chain
+  asm volatile("" ::: "v255");
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   if (t < n) sig[t] = t;
   if (t == 0) info_s = 0;
