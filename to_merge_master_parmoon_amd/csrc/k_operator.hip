@@ -90,7 +90,12 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
       const int k0 = op.a_rowptr[r], n = op.a_rowptr[r + 1] - k0;
       const int *__restrict__ kc = op.a_kcol + k0;
       const T *__restrict__ v = op.av + 9L * k0;
-      if constexpr (SH) {
+      if constexpr (SH && sizeof(T) == 8) {
+        if (op.xa)
+          row_a1_v_aos(kc, op.a1 + k0, n, lane, op.xa, s1, s2, s3);
+        else
+          row_a1<T>(kc, op.a1 + k0, n, lane, x1, x2, x3, s1, s2, s3);
+      } else if constexpr (SH) {
         row_a1<T>(kc, op.a1 + k0, n, lane, x1, x2, x3, s1, s2, s3);
       } else if (r < NA) {
         row_a9<T>(kc, v, n, lane, x1, x2, x3, s1, s2, s3);
@@ -120,7 +125,14 @@ __global__ __launch_bounds__(256) void k_type4(OpView<T> op, const T *__restrict
       const int pr = r - N;
       T s = 0;
       const int k0 = op.b_rowptr[pr], n = op.b_rowptr[pr + 1] - k0;
-      row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, x1, x2, x3, s);
+      if constexpr (SH && sizeof(T) == 8) {
+        if (op.xa)
+          row_b_v_aos(op.b_kcol + k0, op.bv + 3L * k0, n, lane, op.xa, s);
+        else
+          row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, x1, x2, x3, s);
+      } else {
+        row_b<T>(op.b_kcol + k0, op.bv + 3L * k0, n, lane, x1, x2, x3, s);
+      }
       s = red16(s);
       if (lane == 0) {
         const T v = (MODE == 0) ? s : T(bv[3 * N + pr] - s);
@@ -336,6 +348,21 @@ inline int grid_for(long n, int bs = 256, int cap = 8192) {
 }
 
 }  // namespace
+
+namespace {
+__global__ void k_interleave3(double *__restrict__ xa, const double *__restrict__ x, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 p = {x[i], x[n + i]}, q = {x[2L * n + i], 0.0};
+  reinterpret_cast<double2 *>(xa)[2L * i] = p;
+  reinterpret_cast<double2 *>(xa)[2L * i + 1] = q;
+}
+}  // namespace
+void launch_interleave3(double *xa, const double *x, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_interleave3<<<(n + 255) / 256, 256, 0, s>>>(xa, x, n);
+  PMG_HIP_CHECK(hipGetLastError());
+}
 
 template <typename T>
 void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project, double *res,

@@ -53,6 +53,9 @@ struct pmg_ctx {
   // FGMRES workspace
   int restart_alloc = 0;
   double *V = nullptr, *Z = nullptr, *W = nullptr, *Hdev = nullptr, *coef = nullptr;
+  // the FGMRES operator's input velocity interleaved (4 doubles per finest velocity dof)
+  double *xa64 = nullptr;
+  long xa64_n = 0;
   double *hpin = nullptr;   // pinned host scalars
   int hpin_len = 0;
   std::vector<double> hist;
@@ -1229,6 +1232,11 @@ void setup_factors(pmg_ctx *c) {
 }
 
 void ensure_fgmres(pmg_ctx *c) {
+  if (!c->batch && c->xa64_n != c->L.back().n_udof) {
+    dfree(c, c->xa64);
+    c->xa64_n = c->L.back().n_udof;
+    c->xa64 = dalloc<double>(c, 4 * size_t(c->xa64_n));
+  }
   const int m = std::max(1, std::min(c->prm.sc_gmres_restart > 0 ? c->prm.sc_gmres_restart : 10,
                                      std::max(1, c->prm.sc_lin_maxit_saddle)));
   if (m <= c->restart_alloc) return;
@@ -1344,8 +1352,16 @@ int fgmres_dev(pmg_ctx *c, double *sol, const double *rhs, int *iters, double *f
     for (int i = 0; i < restart && j <= maxite; i++, j++) {
       for (int k = 0; k < std::max(1, P.sc_amg_prec_it); ++k) vcycle_dev(c, v(i), z(i));
       {
-        Timed tm(c, &c->tm.matvec, F.matvec_bytes[F.a_shared], 0.0);
-        launch_type4<double>(F.op64(), 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
+        // with the shared velocity block the operator gathers an interleaved copy of the
+        // input's velocity (one pass more: 3 x 8 B read, 4 x 8 B written per velocity dof)
+        OpView<double> o = F.op64();
+        const bool il = F.a_shared && c->xa64;
+        Timed tm(c, &c->tm.matvec, F.matvec_bytes[F.a_shared] + (il ? 56.0 * F.n_udof : 0.0), 0.0);
+        if (il) {
+          launch_interleave3(c->xa64, z(i), F.n_udof, c->stream);
+          o.xa = c->xa64;
+        }
+        launch_type4<double>(o, 0, z(i), nullptr, c->W, false, nullptr, c->red, c->stream);
       }
       // modified Gram-Schmidt, H[k][i] -> Hdev[k], ||w||^2 -> Hdev[i+1]
       launch_dot(c->W, v(0), n, c->Hdev, c->red, c->stream);

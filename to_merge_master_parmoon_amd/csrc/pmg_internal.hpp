@@ -93,6 +93,9 @@ struct OpView {
   // velocity dof r < n_udof, or n_udof + a pressure dof; nullptr: every row
   const int *rows = nullptr;
   int n_rows = 0;
+  // (FP64, shared velocity block) an interleaved copy of the input's velocity, (x1, x2, x3,
+  // 0) per dof, that the type-4 kernels gather instead of the three component arrays
+  const T *xa = nullptr;
 };
 
 // Streaming kernels split their n items (rows, residual work items) into one contiguous
@@ -319,6 +322,8 @@ struct RedScratch {
 // ------------------------------------------------------------------ kernel launchers
 // type-4 operator: mode 0 y = K x, 1 y = b - K x (Defect_NSE4), 2 MG defect (Dirichlet rows 0)
 // project: IntoL20 of the pressure constants; res (device, may be null): ||y||_2
+// xa[4 i + c] = x[c N + i] (c < 3), xa[4 i + 3] = 0 for the n velocity dofs
+void launch_interleave3(double *xa, const double *x, int n, hipStream_t s);
 template <typename T>
 void launch_type4(const OpView<T> &op, int mode, const T *x, const T *b, T *y, bool project,
                   double *res, RedScratch &red, hipStream_t s);

@@ -128,6 +128,61 @@ __device__ inline void row_a1_v(const int *__restrict__ kc, const T *__restrict_
   }
 }
 
+// row_a1_v with x gathered from an interleaved copy (x1, x2, x3, 0 per dof): two 16-byte
+// loads per entry instead of three separate ones; the same entries per lane in the same order
+__device__ inline void row_a1_v_aos(const int *__restrict__ kc, const double *__restrict__ v, int n, int lane,
+                                    const double *__restrict__ xa, double &s1, double &s2, double &s3) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    bool ok[4];
+    const int ld = row_window(j0, n, lane, ok);
+    int c[4];
+    ld4(kc + ld, c);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = ok[u] ? c[u] : 0;
+    double a[4];
+    ld4(v + ld, a);
+    double2 p[4], q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      p[u] = *reinterpret_cast<const double2 *>(xa + 4L * c[u]);
+      q[u] = *reinterpret_cast<const double2 *>(xa + 4L * c[u] + 2);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double w = ok[u] ? a[u] : 0.0;
+      s1 += w * p[u].x;
+      s2 += w * p[u].y;
+      s3 += w * q[u].x;
+    }
+  }
+}
+// row_b_v with x gathered from the interleaved copy
+__device__ inline void row_b_v_aos(const int *__restrict__ kc, const double *__restrict__ v, int n, int lane,
+                                   const double *__restrict__ xa, double &s) {
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    bool ok[4];
+    const int ld = row_window(j0, n, lane, ok);
+    int c[4];
+    ld4(kc + ld, c);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = ok[u] ? c[u] : 0;
+    double b[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) ld4(v + r * n + ld, b[r]);
+    double2 p[4], q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      p[u] = *reinterpret_cast<const double2 *>(xa + 4L * c[u]);
+      q[u] = *reinterpret_cast<const double2 *>(xa + 4L * c[u] + 2);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double z1 = ok[u] ? p[u].x : 0.0, z2 = ok[u] ? p[u].y : 0.0, z3 = ok[u] ? q[u].x : 0.0;
+      s += b[0][u] * z1 + b[1][u] * z2 + b[2][u] * z3;
+    }
+  }
+}
+
 // A row of a Dirichlet dof: only the diagonal blocks A11, A22, A33
 template <typename T, int U>
 __device__ inline void row_a_diag(const int *__restrict__ kc, const T *__restrict__ v, int n, int lane,
