@@ -523,42 +523,36 @@ __global__ __launch_bounds__(256) void k_csr(int n_rows, const int *__restrict__
 // and u_c[r] + a (P x)_c[r] elsewhere, and (ua non-null) ua[r] = (u_1, u_2, u_3, 0);
 // pressure (NC = 1) u[r] += a (P x)[r].  The same values as the prolongation into a vector,
 // the axpy and the sweep start that set the Dirichlet slice and build ua.
-template <typename T, int NC, int L>
+template <typename T, int NC>
 __global__ __launch_bounds__(256) void k_csr_update(int n_rows, const int *__restrict__ rows,
                                                     const int *__restrict__ rp, const int *__restrict__ col,
                                                     const double *__restrict__ val, const T *__restrict__ x, long xs,
                                                     T *__restrict__ u, long us, T a, const T *__restrict__ f, int ds,
                                                     int dc, float4 *__restrict__ ua) {
-  // L lanes per row (rows of P hold 1-27 entries): one row per group, the row's u (and f on
-  // the Dirichlet slice) loaded beside its entries rather than after the sum
-  const int lane = threadIdx.x & (L - 1);
-  const int grp = (blockIdx.x * 256 + threadIdx.x) / L, ngrp = gridDim.x * (256 / L);
+  const int lane = threadIdx.x & 15;
+  const int grp = (blockIdx.x * 256 + threadIdx.x) >> 4, ngrp = gridDim.x * 16;
   for (int i = grp; i < n_rows; i += ngrp) {
     const int r = rows ? rows[i] : i;
     const int k0 = rp[r], k1 = rp[r + 1];
-    const bool dir = r >= ds && r < ds + dc;
-    T y[NC];
-    if (lane == 0)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) y[c] = dir ? f[c * us + r] : u[c * us + r];
     T s[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) s[c] = 0;
-    for (int k = k0 + lane; k < k1; k += L) {
+    for (int k = k0 + lane; k < k1; k += 16) {
       const int j = col[k];
       const T v = T(val[k]);
 #pragma unroll
       for (int c = 0; c < NC; ++c) s[c] += v * x[c * xs + j];
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int o = L / 2; o > 0; o >>= 1) s[c] += __shfl_xor(s[c], o, L);
+    for (int c = 0; c < NC; ++c) s[c] = red16(s[c]);
     if (lane == 0) {
+      const bool dir = r >= ds && r < ds + dc;
       T w[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        w[c] = dir ? y[c] : T(y[c] + a * s[c]);
+        T y = u[c * us + r];
+        y += a * s[c];
+        w[c] = dir ? f[c * us + r] : y;
         u[c * us + r] = w[c];
       }
       if constexpr (NC == 3)
@@ -681,13 +675,13 @@ void launch_prolong_update(const Csr &pu, const Csr &pp, const T *xc, long nxc, 
                            int n_prows) {
   const int nv = urows ? n_urows : pu.n_rows, np = prows ? n_prows : pp.n_rows;
   if (nv > 0)
-    k_csr_update<T, 3, 8><<<std::min(8192, (nv + 31) / 32), 256, 0, s>>>(nv, urows, pu.rowptr, pu.col, pu.val, xc,
-                                                                          nxc, u, nu, a, f, ds, dc,
-                                                                          reinterpret_cast<float4 *>(ua));
+    k_csr_update<T, 3><<<std::min(8192, (nv + 15) / 16), 256, 0, s>>>(nv, urows, pu.rowptr, pu.col, pu.val, xc, nxc,
+                                                                       u, nu, a, f, ds, dc,
+                                                                       reinterpret_cast<float4 *>(ua));
   if (np > 0)
-    k_csr_update<T, 1, 4><<<std::min(8192, (np + 63) / 64), 256, 0, s>>>(np, prows, pp.rowptr, pp.col, pp.val,
-                                                                          xc + 3 * nxc, 0, u + 3 * nu, 0, a, nullptr,
-                                                                          0, 0, nullptr);
+    k_csr_update<T, 1><<<std::min(8192, (np + 15) / 16), 256, 0, s>>>(np, prows, pp.rowptr, pp.col, pp.val,
+                                                                       xc + 3 * nxc, 0, u + 3 * nu, 0, a, nullptr,
+                                                                       0, 0, nullptr);
   PMG_HIP_CHECK(hipGetLastError());
 }
 template void launch_prolong_update<double>(const Csr &, const Csr &, const double *, long, double *, long, double,
