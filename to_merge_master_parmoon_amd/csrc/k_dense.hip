@@ -534,6 +534,12 @@ __global__ __launch_bounds__(256) void k_csr_update(int n_rows, const int *__res
   for (int i = grp; i < n_rows; i += ngrp) {
     const int r = rows ? rows[i] : i;
     const int k0 = rp[r], k1 = rp[r + 1];
+    // the row's u (f on the Dirichlet slice) loaded beside its entries, not after the sum
+    const bool dir = r >= ds && r < ds + dc;
+    T y[NC];
+    if (lane == 0)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) y[c] = dir ? f[c * us + r] : u[c * us + r];
     T s[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) s[c] = 0;
@@ -546,13 +552,12 @@ __global__ __launch_bounds__(256) void k_csr_update(int n_rows, const int *__res
 #pragma unroll
     for (int c = 0; c < NC; ++c) s[c] = red16(s[c]);
     if (lane == 0) {
-      const bool dir = r >= ds && r < ds + dc;
       T w[NC];
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        T y = u[c * us + r];
-        y += a * s[c];
-        w[c] = dir ? f[c * us + r] : y;
+        T yy = y[c];
+        yy += a * s[c];
+        w[c] = dir ? y[c] : yy;
         u[c * us + r] = w[c];
       }
       if constexpr (NC == 3)
