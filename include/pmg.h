@@ -103,7 +103,9 @@ typedef struct {
                                                   and the pressure rows [-S^{-1} B_c A^{-1} | S^{-1}]
                                                   of the patch inverse (S the 4 x 4 Schur complement),
                                                   otherwise the explicit inverse of the patch matrix
-                                                  (FP32 Gauss-Jordan with the pivots of dgetf2); 0:
+                                                  (with the shared block expanded from that block
+                                                  form; nine blocks or refinement steps: FP32
+                                                  Gauss-Jordan with the pivots of dgetf2); 0:
                                                   the stored LU factors and triangular sweeps.  FP64
                                                   always uses the LU factors (dgetrf/dgetrs,
                                                   LinAlg.C:106-130) */
